@@ -1,0 +1,290 @@
+#!/usr/bin/env python3
+"""Benchmark of the RTM propagation-and-imaging path on one MI355X.
+
+Workload (one "step" = one complete FP32 HBC RTM shot: forward propagation
+with upper-border history, backward source reconstruction, receiver
+back-propagation and cross-correlation imaging):
+  512x512x256 salt model at 12.5 m (v(z) gradient, seeded smoothed +-5 %
+  perturbation, seeded ellipsoidal salt at 4500 m/s), HBC border 40
+  (extended 592x592x336), 3000 steps, 25 Hz Ricker, 128x128 surface receivers
+  (stride 4); the shot record is synthesised first by the modeling path.
+
+Reported: Mcell-updates/s of the whole shot (updated cells per wavefield step
+x (3 nt - 2) wavefield steps / shot time), RTM shots/hour, the roofline of the
+dominant kernel (the stencil step: 16 algorithmic bytes per updated cell,
+durations from HIP events around every launch in the timed region), and the
+reference CPU program timed on a bounded sample of the same workload on this
+host's cores.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+import argparse
+import json
+import os
+import re
+import shutil
+import subprocess
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, REPO)
+
+METRIC = ("Mcell-updates/s + RTM shots/hour at image rel-L2≤1e-2, "
+          "batched/learned vs plain FP32")
+HBM_PEAK_GBS = 8000.0
+REFBIN = os.path.join(REPO, "oracle", "_ref", "RTM3D.bin")
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+class Shot:
+    """Device state of one RTM shot on the salt model."""
+
+    def __init__(self, hip, host, workloads, nt):
+        self.hip = hip
+        p = workloads.salt_512_params(nt)
+        self.p = p
+        t = time.time()
+        vel = workloads.salt_model()
+        log("velocity model %s built in %.1f s" % (vel.shape, time.time() - t))
+        b, order = p["blen"], p["stencil_order"]
+        self.b, self.order, self.nt = b, order, nt
+        ve_abc = host.extend_velocity(vel, b, "abc")
+        self.shape = ve_abc.shape
+        self.coefs = host.stencil_coefs(order, p["dx"], p["dy"], p["dz"])
+        self.V = hip.DeviceArray.from_host(host.v2dt2(ve_abc, p["dt"]))
+        del ve_abc
+        self.src = hip.DeviceArray.from_host(host.ricker(nt, p["dt"], p["fpeak"]))
+        self.g = hip.DeviceArray.from_host(host.taper_table(b, p["taper_factor"]))
+        self.geom = hip.RcvGeom(p["receiver_start_x"], p["receiver_start_y"],
+                                p["receiver_depth_z"], p["receiver_distance_x"],
+                                p["receiver_distance_y"], p["receiver_count_x"],
+                                p["receiver_count_y"])
+        self.shot = hip.make_shot(*self.shape, b, order, nt, nt,
+                                  (p["source_start_x"] + b, p["source_start_y"] + b,
+                                   p["source_depth_z"] + b), self.geom)
+        self.F = hip.FieldPair(self.shape)
+        self.R = hip.FieldPair(self.shape)
+        self.rcv = hip.DeviceArray((p["receiver_count_x"], p["receiver_count_y"], nt)).zero()
+        self.img = hip.DeviceArray(self.shape).zero()
+        self.upb = hip.DeviceArray((nt, self.shape[0], self.shape[1], order // 2))
+        # modeling (ABC velocity) synthesises the record; migration uses HBC
+        t = time.time()
+        hip.model_steps(self.shot, self.V, self.coefs, self.g, self.src, self.F, self.rcv, 0, nt)
+        hip.synchronize()
+        self.modeling_s = time.time() - t
+        self.V.upload(host.v2dt2(host.extend_velocity(vel, b, "hbc"), p["dt"]))
+        del vel
+        self.U = float((self.shape[0] - order) * (self.shape[1] - order) * (self.shape[2] - order))
+
+    def migrate(self):
+        hip = self.hip
+        for f in (self.F.a, self.F.b, self.R.a, self.R.b, self.img):
+            f.zero()
+        hip.forward_steps(self.shot, True, self.V, self.coefs, self.g, self.src, self.F,
+                          self.upb, 0, self.nt)
+        hip.backward_steps(self.shot, True, self.V, self.coefs, self.g, self.F, self.R, self.upb,
+                           self.rcv, self.img, self.nt - 1, self.nt, True)
+
+
+def read_traffic():
+    """HBM bytes per stencil launch from the committed rocprofv3 PMC summary."""
+    path = os.path.join(REPO, "profiles", "stencil_salt512_pmc.md")
+    if not os.path.exists(path):
+        return None
+    m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", open(path).read())
+    return float(m.group(1)) if m else None
+
+
+# ---------------------------------------------------------------- CPU leg --
+def _ref_report(stdout, key):
+    m = re.search(r'"%s":\s*([0-9.eE+-]+)' % key, stdout)
+    return float(m.group(1)) if m else None
+
+
+def cpu_reference_sample(workloads, steps, cores):
+    """The reference CPU program (compiled from its sources) on the salt model
+    for `steps` time steps: modeling, then HBC migration of that record."""
+    wd = tempfile.mkdtemp(prefix="rtmcpu_")
+    try:
+        for d in ("vel", "out", "tmp", "dat"):
+            os.makedirs(os.path.join(wd, d))
+        workloads.salt_model().tofile(os.path.join(wd, "vel", "model.bin"))
+        p = workloads.salt_512_params(steps)
+        p["nthreads"] = cores
+        out = {}
+        for mode in ("modeling", "migration"):
+            q = dict(p, modeling=(mode == "modeling"), migration=(mode == "migration"))
+            with open(os.path.join(wd, "in.json"), "w") as f:
+                json.dump(q, f)
+            t = time.time()
+            r = subprocess.run([REFBIN, "in.json"], cwd=wd, capture_output=True, text=True,
+                               timeout=900, env=dict(os.environ, OMP_NUM_THREADS=str(cores)))
+            wall = time.time() - t
+            if r.returncode != 0:
+                raise RuntimeError("reference run failed: " + r.stdout[-500:])
+            out[mode] = {"wall_s": wall,
+                         "kernel_s": _ref_report(r.stdout, "%s_time" % mode)}
+            log("reference %s: %s" % (mode, out[mode]))
+        return out
+    finally:
+        shutil.rmtree(wd, ignore_errors=True)
+
+
+def accuracy_crop(hip, host, workloads):
+    """FP32 device result vs the FP64 oracle on a crop of the salt model
+    (80x80x64 + border 16, 400 steps): relative L2 of the receiver traces and
+    of the migrated HBC image."""
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import pyoracle as po
+
+    full = workloads.salt_model()
+    vel = np.ascontiguousarray(full[216:296, 216:296, :64])
+    del full
+    d, b, order, nt = 12.5, 16, 8, 400
+    p = workloads.params("crop", 80, 80, 64, d, b, nt, workloads.cfl_dt(4500.0, d), 25.0,
+                         (40, 40, 2), (0, 0, 0, 4, 4, 20, 20))
+    geom = hip.RcvGeom(0, 0, 0, 4, 4, 20, 20)
+    coefs = host.stencil_coefs(order, d, d, d)
+    src = host.ricker(nt, p["dt"], p["fpeak"])
+    g = host.taper_table(b, p["taper_factor"])
+    ve = host.extend_velocity(vel, b, "abc")
+    vpe = host.extend_velocity(vel, b, "hbc")
+    shape = ve.shape
+    V = hip.DeviceArray.from_host(host.v2dt2(ve, p["dt"]))
+    dsrc, dg = hip.DeviceArray.from_host(src), hip.DeviceArray.from_host(g)
+    shot = hip.make_shot(*shape, b, order, nt, nt, (40 + b, 40 + b, 2 + b), geom)
+    F = hip.FieldPair(shape)
+    rcv = hip.DeviceArray((20, 20, nt)).zero()
+    hip.model_steps(shot, V, coefs, dg, dsrc, F, rcv, 0, nt)
+    rec32 = rcv.download()
+    V.upload(host.v2dt2(vpe, p["dt"]))
+    S, R = hip.FieldPair(shape), hip.FieldPair(shape)
+    upb = hip.DeviceArray((nt, shape[0], shape[1], 4))
+    img = hip.DeviceArray(shape).zero()
+    hip.forward_steps(shot, True, V, coefs, dg, dsrc, S, upb, 0, nt)
+    hip.backward_steps(shot, True, V, coefs, dg, S, R, upb, rcv, img, nt - 1, nt, True)
+    out = hip.DeviceArray((80, 80, 64))
+    hip.image_finish(img, out, b, host.stencil_coefs(2, d, d, d))
+    img32 = out.download()
+    for x in (V, dsrc, dg, F, rcv, S, R, upb, img, out):
+        x.free()
+    rec64 = po.pipeline_modeling(p, vel, np.float64)[0, 0]
+    q = dict(p, modeling=False, migration=True)
+    img64 = po.pipeline_migration(q, vpe, rec64[None, None], np.float64)
+    rel = lambda a, r: float(np.linalg.norm(a - r) / np.linalg.norm(r))
+    return {"trace_rel_l2": rel(rec32.astype(np.float64), rec64),
+            "image_rel_l2": rel(img32.astype(np.float64), img64),
+            "case": "80x80x64 crop of the salt model, border 16, 400 steps, 20x20 receivers"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--nt", type=int, default=3000)
+    ap.add_argument("--cpu-steps", type=int, default=6)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py runs on one GPU only: the propagation path is measured on a single "
+              "MI355X (one device holds the whole model); --gpus %d is not supported"
+              % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    from rtm3d_demo_amd import hip, host, workloads
+    hip.init(0)
+    sh = Shot(hip, host, workloads, args.nt)
+    log("modeling pass: %.3f s (%.0f Mcell-updates/s)"
+        % (sh.modeling_s, sh.U * args.nt / sh.modeling_s / 1e6))
+    for i in range(args.warmup):
+        sh.migrate()
+        hip.synchronize()
+        log("warmup shot %d done" % i)
+    hip.timing_enable(True)
+    hip.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        sh.migrate()
+        hip.synchronize()
+        log("timed shot %d done at %.2f s" % (i, time.perf_counter() - t0))
+    t1 = time.perf_counter()
+    n0, ms0 = hip.timing_read(0)
+    n1, ms1 = hip.timing_read(1)
+    hip.timing_enable(False)
+    img = sh.img.download()
+    if not np.isfinite(img).all() or not np.abs(img).max() > 0:
+        raise RuntimeError("invalid image")
+    del img
+
+    shot_s = (t1 - t0) / args.steps
+    updates = sh.U * (3 * args.nt - 2)
+    value = updates / shot_s / 1e6
+    bytes_per_launch = 16.0 * sh.U
+    avg0 = ms0 / max(n0, 1) * 1e-3
+    avg1 = ms1 / max(n1, 1) * 1e-3
+    achieved = bytes_per_launch / avg0 / 1e9
+    achieved_fused = (bytes_per_launch + 12.0 * sh.U) / avg1 / 1e9 if n1 else None
+    traffic = read_traffic()
+    line = {
+        "metric": METRIC, "value": round(value, 1), "unit": "Mcell-updates/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(shot_s * 1e3, 2), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "hbc_rtm_salt512_nt%d" % args.nt, "grid": [512, 512, 256],
+                   "extended_grid": list(sh.shape), "dx_m": 12.5, "border": 40,
+                   "nt": args.nt, "stencil_order": 8, "receivers": [128, 128],
+                   "shots_per_step": 1, "parallelism": "single_gpu"},
+        "shots_per_hour": round(3600.0 / shot_s, 1),
+        "modeling_mcell_updates_per_s": round(sh.U * args.nt / sh.modeling_s / 1e6, 1),
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": traffic, "kernel": "k_stencil (plain step)",
+                     "launches": n0, "avg_ms": round(avg0 * 1e3, 4),
+                     "algorithmic_bytes_per_launch": bytes_per_launch},
+        "fused_receiver_step": {"launches": n1, "avg_ms": round(avg1 * 1e3, 4),
+                                "achieved_GBs": round(achieved_fused, 1) if achieved_fused else None,
+                                "algorithmic_bytes_per_launch": 28.0 * sh.U},
+    }
+    for x in (sh.V, sh.src, sh.g, sh.F, sh.R, sh.rcv, sh.img, sh.upb):
+        x.free()
+
+    if not args.no_cpu:
+        cores = min(16, os.cpu_count() or 1)
+        cpu = None
+        try:
+            acc = accuracy_crop(hip, host, workloads)
+            line["accuracy_vs_fp64"] = acc
+        except Exception as e:  # reported, never hidden
+            line["accuracy_vs_fp64"] = {"error": str(e)}
+        if os.path.exists(REFBIN):
+            try:
+                s = args.cpu_steps
+                r = cpu_reference_sample(workloads, s, cores)
+                mig = r["migration"]["kernel_s"] or r["migration"]["wall_s"]
+                cpu_rate = sh.U * (3 * s - 2) / mig / 1e6
+                cpu = {"value": round(cpu_rate, 2), "unit": "Mcell-updates/s", "cores": cores,
+                       "kind": "reference",
+                       "sample": "reference CPU program, salt model 512x512x256 + HBC 40, "
+                                 "%d-step HBC migration (forward+backward+imaging) after a "
+                                 "%d-step modeling run; early steps (small wavefield)" % (s, s),
+                       "migration_s": round(mig, 3),
+                       "shots_per_hour_extrapolated": round(
+                           3600.0 / (mig * (3 * args.nt - 2) / (3 * s - 2)), 3),
+                       "gpu_over_cpu": round(value / cpu_rate, 1)}
+            except Exception as e:
+                cpu = {"error": str(e), "kind": "reference", "cores": cores}
+        line["cpu_baseline"] = cpu
+    hip.finalize()
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

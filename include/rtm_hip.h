@@ -1,0 +1,190 @@
+/*
+ * rtm_hip.h -- C ABI of the MI355X (gfx950) RTM propagation-and-imaging path.
+ *
+ * This is the drop-in boundary below the reference's operator interface
+ * (class RTMPlatform, rtm-3d/include/RTMPlatform.hpp:22-114) and its CUDA
+ * launchers (rtm-3d/kernel/gpu/rtmgpu.hpp:8-65).  Plain C types only: device
+ * pointers are borrowed from the caller (the caller owns every buffer, as the
+ * reference's grid objects own their device buffers), sizes are ints, every
+ * call returns a status code (0 = success) and rtmhip_last_error() returns the
+ * message of the last failure.
+ *
+ * Data layout is the reference's: [x][y][z], z fastest, fp32, extended grids
+ * (nxe = nx + 2*blen, ...), offset = (x*nye + y)*nze + z
+ * (rtm-3d/include/RTMGrid.hpp:217-221).
+ *
+ * All operators enqueue asynchronously on the library stream; copies to the
+ * host and rtmhip_synchronize() wait for it.  Arithmetic follows the
+ * reference CPU platform (rtm-3d/src/platform/host/RTMCPUPlatform.cpp)
+ * operation for operation, so fp32 results are bit-identical to it.
+ */
+#ifndef RTM_HIP_H
+#define RTM_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---- */
+#define RTMHIP_OK               0
+#define RTMHIP_ERR_INVALID      1  /* bad argument: sizes, pointers, order */
+#define RTMHIP_ERR_HIP          2  /* HIP runtime error */
+#define RTMHIP_ERR_DISPERSION   3  /* a NaN was produced by the stencil: the reference's
+                                      "Numerical Dispersion" exception
+                                      (RTMCPUPlatform.cpp:374-391) */
+#define RTMHIP_ERR_NOT_INIT     4  /* rtmhip_init() not called */
+#define RTMHIP_ERR_UNSUPPORTED  5  /* e.g. stencil order above 8 */
+
+/* Receiver geometry in interior coordinates: the JSON keys
+ * receiver_{start_x,start_y,depth_z,distance_x,distance_y,count_x,count_y}
+ * (rtm-3d/src/seismic/RTMSeismic.cpp:4-30). */
+typedef struct {
+    int start_x, start_y, depth_z;
+    int dist_x, dist_y;
+    int count_x, count_y;
+} rtmhip_rcv_geom;
+
+/* One shot on one extended grid: everything the per-step operators of the
+ * time loops need besides buffers.  Source coordinates are extended ones
+ * (interior + blen), as RTMCPUPlatform::rtmApplySource computes them
+ * (RTMCPUPlatform.cpp:103-113). */
+typedef struct {
+    int nxe, nye, nze;   /* extended grid */
+    int blen;            /* absorbing border width */
+    int order;           /* stencil order: 2, 4, 6 or 8 */
+    int nt;              /* total time steps */
+    int ntstep;          /* receiver-chunk length (divides nt) */
+    int sxe, sye, sze;   /* source cell, extended coordinates */
+    rtmhip_rcv_geom rcv; /* receiver geometry */
+} rtmhip_shot;
+
+/* ================= lifecycle =================
+ * Replaces RTMGPUPlatform::initRTMPlatform / destroyRTMPlatform
+ * (rtm-3d/src/platform/gpu/RTMGPUPlatform.cpp:17-57): selects the device,
+ * creates the library stream.  Re-initialising with another device is an
+ * error until rtmhip_finalize(). */
+int rtmhip_init(int device);
+int rtmhip_finalize(void);
+const char *rtmhip_last_error(void);
+int rtmhip_device_count(int *count);
+int rtmhip_synchronize(void);
+/* Library version as major*10000 + minor*100 + patch. */
+int rtmhip_version(void);
+
+/* ================= device-grid slot =================
+ * Replaces GPUDeviceGrid::createDeviceBuffer / removeDeviceBuffer /
+ * moveToDevice / moveFromDevice / devMemSet
+ * (rtm-3d/src/grid/RTMGPUDeviceGrid.cpp:14-58,
+ *  rtm-3d/include/RTMDeviceGrid.hpp:35-39). */
+int rtmhip_alloc(void **dptr, size_t bytes);
+int rtmhip_free(void *dptr);
+int rtmhip_copy_to_device(void *dst, const void *src, size_t bytes);
+int rtmhip_copy_from_device(void *dst, const void *src, size_t bytes);
+int rtmhip_copy_device(void *dst, const void *src, size_t bytes);
+int rtmhip_fill(float *dptr, size_t count, float value);
+
+/* ================= per-step operators =================
+ * One entry per RTMPlatform virtual (rtm-3d/include/RTMPlatform.hpp:28-102).
+ * Host pointers are marked (host); everything else is a device pointer. */
+
+/* rtmStep (RTMPlatform.hpp:50; CPU RTMCPUPlatform.cpp:325-395; CUDA
+ * RTMSTEP_WRAPPER rtmgpu.hpp:8-11): PP <- 2P - PP + V2DT2*lap(P) on
+ * [order/2, n-order/2)^3, in place.  coefs (host) is the [3][order+1] stencil
+ * plane (RTMGrid.hpp:433-446).  A NaN makes a later rtmhip_synchronize()
+ * return RTMHIP_ERR_DISPERSION. */
+int rtmhip_step(const float *P, float *PP, const float *V2DT2, const float *coefs, int nxe,
+                int nye, int nze, int order);
+
+/* rtmTaperAllBorders / rtmTaperUpperBorders (RTMPlatform.hpp:36-43; CPU
+ * taper3D RTMCPUPlatform.cpp:710-920, taperUpper3D :566-657; CUDA
+ * RTMTAPER_WRAPPER rtmgpu.hpp:28-31).  taper: device table g[blen]
+ * (RTMSeismic.hpp:294-305). */
+int rtmhip_taper(float *P, const float *taper, int nxe, int nye, int nze, int blen,
+                 int upper_only);
+
+/* rtmApplySource (RTMPlatform.hpp:67; CPU RTMCPUPlatform.cpp:103-113; CUDA
+ * RTMSOURCE_WRAPPER rtmgpu.hpp:22-26): PP[sxe][sye][sze] += value. */
+int rtmhip_apply_source(float *PP, int nxe, int nye, int nze, int sxe, int sye, int sze,
+                        float value);
+
+/* rtmSaveReceiverData / rtmRestoreReceiverData (RTMPlatform.hpp:72-77; CPU
+ * RTMCPUPlatform.cpp:114-151; CUDA RTMSEISM_WRAPPER rtmgpu.hpp:13-20).
+ * rcv: one chunk [count_x][count_y][ntstep]; restore != 0 SETS the wavefield
+ * (CPU semantics), restore == 0 samples it at time index lt. */
+int rtmhip_receivers(float *PP, float *rcv, int nxe, int nye, int nze, int blen,
+                     const rtmhip_rcv_geom *geom, int ntstep, int lt, int restore);
+
+/* rtmSaveUpperBorder / rtmRestoreUpperBorder (RTMPlatform.hpp:82-88; CPU
+ * RTMCPUPlatform.cpp:153-189; CUDA RTMWUPB_WRAPPER rtmgpu.hpp:33-34).
+ * upb: [ntstep][nxe][nye][order/2] with 64-bit offsets; the layer is
+ * z in [blen - order/2, blen). */
+int rtmhip_upper_border(float *PP, float *upb, int nxe, int nye, int nze, int blen, int order,
+                        int lt, int restore);
+
+/* rtmImageCondition (RTMPlatform.hpp:62; CPU RTMCPUPlatform.cpp:513-540;
+ * CUDA RTMIMG_WRAPPER rtmgpu.hpp:36-38): img += PS*PR on all x, y and
+ * z in [blen, nze-blen) (CPU range). */
+int rtmhip_image_condition(float *img, const float *PS, const float *PR, int nxe, int nye,
+                           int nze, int blen);
+
+/* Image post-processing of the controller (RTMAcousticController.cpp:244-249,
+ * RTMGrid.cpp:129-166 and :226-300): order-2 Laplacian filter with the
+ * [3][3] plane fcoefs (host) followed by border removal; out is the
+ * [nxe-2blen][nye-2blen][nze-2blen] image. */
+int rtmhip_image_finish(const float *img, float *out, int nxe, int nye, int nze, int blen,
+                        const float *fcoefs);
+
+/* dst[i] <- dst[i] + src[i] (RTMCube::stackRegion, RTMGrid.cpp:195-223). */
+int rtmhip_stack(float *dst, const float *src, size_t count);
+
+/* ================= device-resident time loops =================
+ * The reference's per-step loops of RTMFiniteDifferencesKernel /
+ * RTMHBCKernel / RTMRBCKernel with no host synchronisation between steps.
+ * The caller keeps two wavefield pointers per field and passes them by
+ * address; the loops swap them exactly as RTMGRID_SWAP does (RTM.hpp:56-60),
+ * so a run may be split into chunks of ntstep steps.  coefs: host plane as
+ * for rtmhip_step; taper, ricker: device tables. */
+
+/* Modeling, steps it0 .. it0+nsteps-1 of rtmAcousticFiniteDiffModeling
+ * (rtm-3d/src/kernel/RTMFiniteDifferencesKernel.cpp:169-214): swap,
+ * taperAll x2, step, source, save receivers into rcv_chunk at it % ntstep. */
+int rtmhip_model_steps(const rtmhip_shot *shot, const float *V2DT2, const float *coefs,
+                       const float *taper, const float *ricker, float **P, float **PP,
+                       float *rcv_chunk, int it0, int nsteps);
+
+/* Forward propagation of migration, steps it0 .. it0+nsteps-1:
+ * hbc != 0: rtmHBCForward (RTMFiniteDifferencesKernel_HBC.cpp:140-185):
+ *   swap, taperUpper x2, step, source, save UPB at absolute index it.
+ * hbc == 0: rtmRBCForward (RTMFiniteDifferencesKernel_RBC.cpp:96-126). */
+int rtmhip_forward_steps(const rtmhip_shot *shot, int hbc, const float *V2DT2,
+                         const float *coefs, const float *taper, const float *ricker, float **P,
+                         float **PP, float *upb, int it0, int nsteps);
+
+/* Backward propagation with imaging, it from it_hi down to it_hi-nsteps+1
+ * (rtmHBCBackward RTMFiniteDifferencesKernel_HBC.cpp:261-354,
+ * rtmRBCBackward RTMFiniteDifferencesKernel_RBC.cpp:201-266).  On the first
+ * call (it_hi = nt-1) *Ps and *PPs must be the forward's final *P and *PP
+ * (the reference's snap0/snap1); *Pr, *PPr and img must be zero.  rcv_chunk
+ * holds the chunk containing it_hi.  With fused != 0 the receiver step,
+ * receiver injection and imaging condition run as one pass over the grid. */
+int rtmhip_backward_steps(const rtmhip_shot *shot, int hbc, const float *V2DT2,
+                          const float *coefs, const float *taper, float **Ps, float **PPs,
+                          float **Pr, float **PPr, const float *upb, const float *rcv_chunk,
+                          float *img, int it_hi, int nsteps, int fused);
+
+/* ================= measurement =================
+ * When enabled, every launch of the stencil kernels is bracketed by HIP
+ * events on the library stream; rtmhip_timing_read() synchronises and returns
+ * the number of launches and their summed duration per kernel class
+ * (0 = plain stencil step, 1 = stencil step fused with receivers+imaging),
+ * then clears the record. */
+int rtmhip_timing_enable(int on);
+int rtmhip_timing_read(int kernel_class, long long *launches, double *total_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,60 @@
+// kernel.hpp -- per-shot time loops of the RTM path on the device.
+//
+// Mirrors RTMFiniteDifferencesKernel::rtmModel (rtm-3d/src/kernel/
+// RTMFiniteDifferencesKernel.cpp:103-246) and RTMHBCKernel / RTMRBCKernel
+// ::rtmMigrate (rtm-3d/src/kernel/RTMFiniteDifferencesKernel_HBC.cpp:64-365,
+// ..._RBC.cpp:36-274): same operator order, same chunked seismogram files,
+// same outputs.  Differences by design: wavefields, the upper-border history
+// and the receiver chunks stay in device memory for the whole shot (no per
+// operator synchronisation, no UPB spill files), and the receiver step of the
+// backward pass is fused with receiver injection and imaging.
+#pragma once
+#include <string>
+#include <vector>
+
+#include "grid.hpp"
+#include "param.hpp"
+#include "platform.hpp"
+
+namespace rtm3d {
+
+struct ShotPos {
+    int sx, sy, sz;  // interior coordinates
+};
+
+struct KernelReport {
+    double modeling_s = 0, forward_s = 0, backward_s = 0, migration_s = 0;
+    long long modeling_n = 0, migration_n = 0;
+    double cell_updates = 0;  // wavefield cell updates performed
+    std::string to_string() const;
+};
+
+class RTMKernel {
+public:
+    RTMKernel(const RTMParam &p, RTMHIPPlatform &plat);
+    ~RTMKernel();
+    void initKernel();
+    void rtmModel(const ShotPos &s, const DeviceGrid &v2dt2);
+    // Migrates one shot; the shot image (extended grid) is left in img().
+    void rtmMigrate(const ShotPos &s, const DeviceGrid &v2dt2);
+    const DeviceGrid &img() const { return img_; }
+    KernelReport &report() { return report_; }
+
+private:
+    const RTMParam &p_;
+    RTMHIPPlatform &plat_;
+    Dims d_{};
+    bool hbc_ = true;
+    std::vector<float> coefs_;
+    DeviceGrid taper_, ricker_, rcv_, upb_, img_;
+    DeviceGrid f_[4];  // two source and two receiver wavefields
+    KernelReport report_;
+    std::vector<float> host_rcv_;
+
+    rtmhip_shot make_shot(const ShotPos &s) const;
+    rtmhip_rcv_geom geom() const;
+    void snapshot(const float *field, int sx, const std::string &file);
+    void progress(const char *tag, const ShotPos &s, int it, double t0);
+};
+
+}  // namespace rtm3d

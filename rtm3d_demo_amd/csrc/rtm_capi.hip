@@ -1,0 +1,577 @@
+// rtm_capi.hip -- implementation of the C ABI declared in include/rtm_hip.h.
+//
+// One library stream per process; every operator validates its arguments on
+// the host (shapes against what the kernels assume), enqueues its kernels
+// and returns.  The time-loop entry points chain the operators of the
+// reference's per-step loops with no host synchronisation in between.
+#include "rtm_internal.hpp"
+
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/rtm_hip.h"
+
+using namespace rtmhip;
+
+namespace {
+
+struct Timing {
+    std::vector<hipEvent_t> start, stop;
+    size_t used = 0;
+};
+
+struct State {
+    bool init = false;
+    int device = -1;
+    hipStream_t stream = nullptr;
+    int *flag = nullptr;  // NaN ("numerical dispersion") flag written by the stencil
+    bool timing = false;
+    Timing tm[2];
+};
+
+State g_state;
+thread_local std::string g_err;
+
+int fail(int code, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+int hip_fail(hipError_t e, const char *what)
+{
+    return fail(RTMHIP_ERR_HIP, "%s: %s", what, hipGetErrorString(e));
+}
+
+#define HIPCHK(expr, what)                              \
+    do {                                                \
+        hipError_t e__ = (expr);                        \
+        if (e__ != hipSuccess) return hip_fail(e__, what); \
+    } while (0)
+
+#define NEED_INIT()                                                               \
+    do {                                                                          \
+        if (!g_state.init) return fail(RTMHIP_ERR_NOT_INIT, "rtmhip_init() not called"); \
+    } while (0)
+
+int check_grid(int nxe, int nye, int nze)
+{
+    if (nxe <= 0 || nye <= 0 || nze <= 0)
+        return fail(RTMHIP_ERR_INVALID, "grid dimensions must be positive (%d,%d,%d)", nxe, nye,
+                    nze);
+    if ((long long)nye * nze >= (1LL << 31))
+        return fail(RTMHIP_ERR_INVALID, "nye*nze must stay below 2^31");
+    return RTMHIP_OK;
+}
+
+int check_order(int order, int *radius)
+{
+    if (order != 2 && order != 4 && order != 6 && order != 8)
+        return fail(RTMHIP_ERR_UNSUPPORTED, "stencil order %d not supported (2, 4, 6, 8)", order);
+    *radius = order / 2;
+    return RTMHIP_OK;
+}
+
+Coefs make_coefs(const float *c, int order)
+{
+    Coefs cf;
+    memset(&cf, 0, sizeof cf);
+    for (int d = 0; d < 3; ++d)
+        for (int k = 0; k <= order; ++k) cf.c[d][k] = c[d * (order + 1) + k];
+    return cf;
+}
+
+int check_rcv(const rtmhip_rcv_geom *g, int nze, int blen)
+{
+    if (!g) return fail(RTMHIP_ERR_INVALID, "receiver geometry is NULL");
+    if (g->count_x < 0 || g->count_y < 0)
+        return fail(RTMHIP_ERR_INVALID, "negative receiver count");
+    if ((g->count_x > 1 && g->dist_x < 1) || (g->count_y > 1 && g->dist_y < 1))
+        return fail(RTMHIP_ERR_INVALID,
+                    "receiver distances must be >= 1 when more than one receiver per line");
+    if (g->depth_z < 0 || g->depth_z + blen >= nze)
+        return fail(RTMHIP_ERR_INVALID, "receiver depth %d outside the grid", g->depth_z);
+    if (g->start_x < 0 || g->start_y < 0)
+        return fail(RTMHIP_ERR_INVALID, "negative receiver start");
+    return RTMHIP_OK;
+}
+
+hipError_t stencil(int radius, const float *P, float *PP, const float *V, const Coefs &cf,
+                   const Grid &g, const ImageFuse *fuse)
+{
+    Timing *tm = g_state.timing ? &g_state.tm[fuse ? 1 : 0] : nullptr;
+    if (tm) {
+        if (tm->used == tm->start.size()) {
+            hipEvent_t a, b;
+            hipError_t e = hipEventCreate(&a);
+            if (e != hipSuccess) return e;
+            e = hipEventCreate(&b);
+            if (e != hipSuccess) return e;
+            tm->start.push_back(a);
+            tm->stop.push_back(b);
+        }
+        hipError_t e = hipEventRecord(tm->start[tm->used], g_state.stream);
+        if (e != hipSuccess) return e;
+    }
+    hipError_t e = launch_stencil(radius, P, PP, V, cf, g, fuse, g_state.flag, g_state.stream,
+                                  nullptr);
+    if (e != hipSuccess) return e;
+    if (tm) {
+        e = hipEventRecord(tm->stop[tm->used], g_state.stream);
+        if (e != hipSuccess) return e;
+        tm->used++;
+    }
+    return hipSuccess;
+}
+
+int check_dispersion()
+{
+    int h = 0;
+    HIPCHK(hipMemcpyAsync(&h, g_state.flag, sizeof h, hipMemcpyDeviceToHost, g_state.stream),
+           "reading the dispersion flag");
+    HIPCHK(hipStreamSynchronize(g_state.stream), "hipStreamSynchronize");
+    if (h) {
+        HIPCHK(hipMemsetAsync(g_state.flag, 0, sizeof(int), g_state.stream), "clearing the flag");
+        HIPCHK(hipStreamSynchronize(g_state.stream), "hipStreamSynchronize");
+        return fail(RTMHIP_ERR_DISPERSION,
+                    "RTM ERROR: Numerical Dispersion (NaN produced by the stencil update)");
+    }
+    return RTMHIP_OK;
+}
+
+struct ShotCtx {
+    Grid g;
+    int radius, hf;
+    Coefs cf;
+    long long src_off;
+    int rx0, ry0, rz;
+};
+
+int prepare_shot(const rtmhip_shot *s, const float *coefs, ShotCtx *c)
+{
+    if (!s || !coefs) return fail(RTMHIP_ERR_INVALID, "shot or coefs is NULL");
+    int rc = check_grid(s->nxe, s->nye, s->nze);
+    if (rc) return rc;
+    rc = check_order(s->order, &c->radius);
+    if (rc) return rc;
+    c->hf = s->order / 2;
+    if (s->blen < c->hf || 2 * s->blen >= s->nxe || 2 * s->blen >= s->nye || 2 * s->blen >= s->nze)
+        return fail(RTMHIP_ERR_INVALID, "blen %d must be >= order/2 and leave an interior",
+                    s->blen);
+    if (s->nt <= 0 || s->ntstep <= 0 || s->nt % s->ntstep != 0)
+        return fail(RTMHIP_ERR_INVALID, "ntstep (%d) must divide nt (%d)", s->ntstep, s->nt);
+    if (s->sxe < 0 || s->sxe >= s->nxe || s->sye < 0 || s->sye >= s->nye || s->sze < 0 ||
+        s->sze >= s->nze)
+        return fail(RTMHIP_ERR_INVALID, "source (%d,%d,%d) outside the grid", s->sxe, s->sye,
+                    s->sze);
+    rc = check_rcv(&s->rcv, s->nze, s->blen);
+    if (rc) return rc;
+    c->g = dense_grid(s->nxe, s->nye, s->nze);
+    c->cf = make_coefs(coefs, s->order);
+    c->src_off = ((long long)s->sxe * s->nye + s->sye) * s->nze + s->sze;
+    c->rx0 = s->rcv.start_x + s->blen;
+    c->ry0 = s->rcv.start_y + s->blen;
+    c->rz = s->rcv.depth_z + s->blen;
+    return RTMHIP_OK;
+}
+
+int check_chunk(const rtmhip_shot *s, int lo, int hi)
+{
+    if (lo < 0 || hi >= s->nt || lo > hi)
+        return fail(RTMHIP_ERR_INVALID, "time range [%d,%d] outside [0,%d)", lo, hi, s->nt);
+    if (lo / s->ntstep != hi / s->ntstep)
+        return fail(RTMHIP_ERR_INVALID, "time range [%d,%d] crosses a receiver chunk of %d", lo,
+                    hi, s->ntstep);
+    return RTMHIP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int rtmhip_version(void) { return 10000; }
+
+const char *rtmhip_last_error(void) { return g_err.c_str(); }
+
+int rtmhip_device_count(int *count)
+{
+    if (!count) return fail(RTMHIP_ERR_INVALID, "count is NULL");
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess) n = 0;
+    *count = n;
+    return RTMHIP_OK;
+}
+
+int rtmhip_init(int device)
+{
+    if (g_state.init) {
+        if (device == g_state.device) return RTMHIP_OK;
+        return fail(RTMHIP_ERR_INVALID, "already initialised on device %d", g_state.device);
+    }
+    int n = 0;
+    HIPCHK(hipGetDeviceCount(&n), "hipGetDeviceCount");
+    if (device < 0 || device >= n)
+        return fail(RTMHIP_ERR_INVALID, "device %d not present (%d devices)", device, n);
+    HIPCHK(hipSetDevice(device), "hipSetDevice");
+    HIPCHK(hipStreamCreateWithFlags(&g_state.stream, hipStreamNonBlocking), "hipStreamCreate");
+    HIPCHK(hipMalloc((void **)&g_state.flag, sizeof(int)), "hipMalloc(flag)");
+    HIPCHK(hipMemsetAsync(g_state.flag, 0, sizeof(int), g_state.stream), "hipMemset(flag)");
+    HIPCHK(hipStreamSynchronize(g_state.stream), "hipStreamSynchronize");
+    g_state.device = device;
+    g_state.init = true;
+    return RTMHIP_OK;
+}
+
+int rtmhip_finalize(void)
+{
+    if (!g_state.init) return RTMHIP_OK;
+    hipStreamSynchronize(g_state.stream);
+    for (auto &t : g_state.tm) {
+        for (auto e : t.start) hipEventDestroy(e);
+        for (auto e : t.stop) hipEventDestroy(e);
+        t = Timing();
+    }
+    hipFree(g_state.flag);
+    hipStreamDestroy(g_state.stream);
+    g_state = State();
+    return RTMHIP_OK;
+}
+
+int rtmhip_synchronize(void)
+{
+    NEED_INIT();
+    HIPCHK(hipStreamSynchronize(g_state.stream), "hipStreamSynchronize");
+    return check_dispersion();
+}
+
+int rtmhip_alloc(void **dptr, size_t bytes)
+{
+    NEED_INIT();
+    if (!dptr) return fail(RTMHIP_ERR_INVALID, "dptr is NULL");
+    HIPCHK(hipMalloc(dptr, bytes ? bytes : 4), "hipMalloc");
+    return RTMHIP_OK;
+}
+
+int rtmhip_free(void *dptr)
+{
+    NEED_INIT();
+    if (!dptr) return RTMHIP_OK;
+    HIPCHK(hipStreamSynchronize(g_state.stream), "hipStreamSynchronize");
+    HIPCHK(hipFree(dptr), "hipFree");
+    return RTMHIP_OK;
+}
+
+int rtmhip_copy_to_device(void *dst, const void *src, size_t bytes)
+{
+    NEED_INIT();
+    if (!dst || !src) return fail(RTMHIP_ERR_INVALID, "NULL pointer");
+    HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, g_state.stream), "copy H2D");
+    HIPCHK(hipStreamSynchronize(g_state.stream), "hipStreamSynchronize");
+    return RTMHIP_OK;
+}
+
+int rtmhip_copy_from_device(void *dst, const void *src, size_t bytes)
+{
+    NEED_INIT();
+    if (!dst || !src) return fail(RTMHIP_ERR_INVALID, "NULL pointer");
+    HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, g_state.stream), "copy D2H");
+    HIPCHK(hipStreamSynchronize(g_state.stream), "hipStreamSynchronize");
+    return RTMHIP_OK;
+}
+
+int rtmhip_copy_device(void *dst, const void *src, size_t bytes)
+{
+    NEED_INIT();
+    if (!dst || !src) return fail(RTMHIP_ERR_INVALID, "NULL pointer");
+    HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, g_state.stream), "copy D2D");
+    return RTMHIP_OK;
+}
+
+int rtmhip_fill(float *dptr, size_t count, float value)
+{
+    NEED_INIT();
+    if (!dptr && count) return fail(RTMHIP_ERR_INVALID, "NULL pointer");
+    HIPCHK(launch_fill(dptr, count, value, g_state.stream), "fill");
+    return RTMHIP_OK;
+}
+
+int rtmhip_step(const float *P, float *PP, const float *V2DT2, const float *coefs, int nxe,
+                int nye, int nze, int order)
+{
+    NEED_INIT();
+    int radius, rc;
+    if ((rc = check_order(order, &radius))) return rc;
+    if ((rc = check_grid(nxe, nye, nze))) return rc;
+    if (!P || !PP || !V2DT2 || !coefs) return fail(RTMHIP_ERR_INVALID, "NULL pointer");
+    if (P == PP) return fail(RTMHIP_ERR_INVALID, "P and PP must be distinct buffers");
+    const Coefs cf = make_coefs(coefs, order);
+    HIPCHK(stencil(radius, P, PP, V2DT2, cf, dense_grid(nxe, nye, nze), nullptr), "stencil");
+    return RTMHIP_OK;
+}
+
+int rtmhip_taper(float *P, const float *taper, int nxe, int nye, int nze, int blen,
+                 int upper_only)
+{
+    NEED_INIT();
+    int rc;
+    if ((rc = check_grid(nxe, nye, nze))) return rc;
+    if (!P || (!taper && blen > 0)) return fail(RTMHIP_ERR_INVALID, "NULL pointer");
+    if (blen < 0 || 2 * blen >= nxe || 2 * blen >= nye || 2 * blen >= nze)
+        return fail(RTMHIP_ERR_INVALID, "blen %d leaves no interior", blen);
+    HIPCHK(launch_taper(P, taper, dense_grid(nxe, nye, nze), blen, upper_only != 0,
+                        g_state.stream),
+           "taper");
+    return RTMHIP_OK;
+}
+
+int rtmhip_apply_source(float *PP, int nxe, int nye, int nze, int sxe, int sye, int sze,
+                        float value)
+{
+    NEED_INIT();
+    int rc;
+    if ((rc = check_grid(nxe, nye, nze))) return rc;
+    if (!PP) return fail(RTMHIP_ERR_INVALID, "NULL pointer");
+    if (sxe < 0 || sxe >= nxe || sye < 0 || sye >= nye || sze < 0 || sze >= nze)
+        return RTMHIP_OK;  // outside the grid: no-op, as getWithGlobalCoordinate
+    const long long o = ((long long)sxe * nye + sye) * nze + sze;
+    HIPCHK(launch_source(PP, o, nullptr, 0, value, g_state.stream), "source");
+    return RTMHIP_OK;
+}
+
+int rtmhip_receivers(float *PP, float *rcv, int nxe, int nye, int nze, int blen,
+                     const rtmhip_rcv_geom *geom, int ntstep, int lt, int restore)
+{
+    NEED_INIT();
+    int rc;
+    if ((rc = check_grid(nxe, nye, nze))) return rc;
+    if ((rc = check_rcv(geom, nze, blen))) return rc;
+    if (!PP || !rcv) return fail(RTMHIP_ERR_INVALID, "NULL pointer");
+    if (ntstep <= 0 || lt < 0 || lt >= ntstep)
+        return fail(RTMHIP_ERR_INVALID, "lt %d outside [0,%d)", lt, ntstep);
+    HIPCHK(launch_receivers(PP, rcv, dense_grid(nxe, nye, nze), geom->start_x + blen,
+                            geom->start_y + blen, geom->depth_z + blen, geom->dist_x,
+                            geom->dist_y, geom->count_x, geom->count_y, ntstep, lt,
+                            restore != 0, g_state.stream),
+           "receivers");
+    return RTMHIP_OK;
+}
+
+int rtmhip_upper_border(float *PP, float *upb, int nxe, int nye, int nze, int blen, int order,
+                        int lt, int restore)
+{
+    NEED_INIT();
+    int radius, rc;
+    if ((rc = check_grid(nxe, nye, nze))) return rc;
+    if ((rc = check_order(order, &radius))) return rc;
+    if (!PP || !upb) return fail(RTMHIP_ERR_INVALID, "NULL pointer");
+    if (blen < radius || blen > nze) return fail(RTMHIP_ERR_INVALID, "blen %d < order/2", blen);
+    if (lt < 0) return fail(RTMHIP_ERR_INVALID, "negative lt");
+    HIPCHK(launch_upb(PP, upb, dense_grid(nxe, nye, nze), blen, radius, lt, restore != 0,
+                      g_state.stream),
+           "upper border");
+    return RTMHIP_OK;
+}
+
+int rtmhip_image_condition(float *img, const float *PS, const float *PR, int nxe, int nye,
+                           int nze, int blen)
+{
+    NEED_INIT();
+    int rc;
+    if ((rc = check_grid(nxe, nye, nze))) return rc;
+    if (!img || !PS || !PR) return fail(RTMHIP_ERR_INVALID, "NULL pointer");
+    if (blen < 0) return fail(RTMHIP_ERR_INVALID, "negative blen");
+    HIPCHK(launch_image(img, PS, PR, dense_grid(nxe, nye, nze), blen, nze - blen, g_state.stream),
+           "image");
+    return RTMHIP_OK;
+}
+
+int rtmhip_image_finish(const float *img, float *out, int nxe, int nye, int nze, int blen,
+                        const float *fcoefs)
+{
+    NEED_INIT();
+    int rc;
+    if ((rc = check_grid(nxe, nye, nze))) return rc;
+    if (!img || !out || !fcoefs) return fail(RTMHIP_ERR_INVALID, "NULL pointer");
+    if (blen < 1 || 2 * blen >= nxe || 2 * blen >= nye || 2 * blen >= nze)
+        return fail(RTMHIP_ERR_INVALID, "blen %d must be >= 1 and leave an interior", blen);
+    const Coefs fc = make_coefs(fcoefs, 2);
+    HIPCHK(launch_image_finish(img, out, dense_grid(nxe, nye, nze), blen, fc, g_state.stream),
+           "image finish");
+    return RTMHIP_OK;
+}
+
+int rtmhip_stack(float *dst, const float *src, size_t count)
+{
+    NEED_INIT();
+    if ((!dst || !src) && count) return fail(RTMHIP_ERR_INVALID, "NULL pointer");
+    HIPCHK(launch_stack(dst, src, count, g_state.stream), "stack");
+    return RTMHIP_OK;
+}
+
+int rtmhip_model_steps(const rtmhip_shot *shot, const float *V2DT2, const float *coefs,
+                       const float *taper, const float *ricker, float **P, float **PP,
+                       float *rcv_chunk, int it0, int nsteps)
+{
+    NEED_INIT();
+    ShotCtx c;
+    int rc;
+    if ((rc = prepare_shot(shot, coefs, &c))) return rc;
+    if (!V2DT2 || !taper || !ricker || !P || !PP || !*P || !*PP || !rcv_chunk)
+        return fail(RTMHIP_ERR_INVALID, "NULL pointer");
+    if (nsteps <= 0) return RTMHIP_OK;
+    if ((rc = check_chunk(shot, it0, it0 + nsteps - 1))) return rc;
+    hipStream_t s = g_state.stream;
+    for (int it = it0; it < it0 + nsteps; ++it) {
+        float *t = *P;
+        *P = *PP;
+        *PP = t;
+        HIPCHK(launch_taper(*P, taper, c.g, shot->blen, false, s), "taper");
+        HIPCHK(launch_taper(*PP, taper, c.g, shot->blen, false, s), "taper");
+        HIPCHK(stencil(c.radius, *P, *PP, V2DT2, c.cf, c.g, nullptr), "stencil");
+        HIPCHK(launch_source(*PP, c.src_off, ricker, it, 0.f, s), "source");
+        HIPCHK(launch_receivers(*PP, rcv_chunk, c.g, c.rx0, c.ry0, c.rz, shot->rcv.dist_x,
+                                shot->rcv.dist_y, shot->rcv.count_x, shot->rcv.count_y,
+                                shot->ntstep, it % shot->ntstep, false, s),
+               "receivers");
+    }
+    return RTMHIP_OK;
+}
+
+int rtmhip_forward_steps(const rtmhip_shot *shot, int hbc, const float *V2DT2,
+                         const float *coefs, const float *taper, const float *ricker, float **P,
+                         float **PP, float *upb, int it0, int nsteps)
+{
+    NEED_INIT();
+    ShotCtx c;
+    int rc;
+    if ((rc = prepare_shot(shot, coefs, &c))) return rc;
+    if (!V2DT2 || !ricker || !P || !PP || !*P || !*PP || (hbc && (!taper || !upb)))
+        return fail(RTMHIP_ERR_INVALID, "NULL pointer");
+    if (nsteps <= 0) return RTMHIP_OK;
+    if (it0 < 0 || it0 + nsteps > shot->nt)
+        return fail(RTMHIP_ERR_INVALID, "time range outside [0,%d)", shot->nt);
+    hipStream_t s = g_state.stream;
+    for (int it = it0; it < it0 + nsteps; ++it) {
+        float *t = *P;
+        *P = *PP;
+        *PP = t;
+        if (hbc) {
+            HIPCHK(launch_taper(*P, taper, c.g, shot->blen, true, s), "taper");
+            HIPCHK(launch_taper(*PP, taper, c.g, shot->blen, true, s), "taper");
+        }
+        HIPCHK(stencil(c.radius, *P, *PP, V2DT2, c.cf, c.g, nullptr), "stencil");
+        HIPCHK(launch_source(*PP, c.src_off, ricker, it, 0.f, s), "source");
+        if (hbc) HIPCHK(launch_upb(*PP, upb, c.g, shot->blen, c.hf, it, false, s), "upb save");
+    }
+    return RTMHIP_OK;
+}
+
+int rtmhip_backward_steps(const rtmhip_shot *shot, int hbc, const float *V2DT2,
+                          const float *coefs, const float *taper, float **Ps, float **PPs,
+                          float **Pr, float **PPr, const float *upb, const float *rcv_chunk,
+                          float *img, int it_hi, int nsteps, int fused)
+{
+    NEED_INIT();
+    ShotCtx c;
+    int rc;
+    if ((rc = prepare_shot(shot, coefs, &c))) return rc;
+    if (!V2DT2 || !Ps || !PPs || !Pr || !PPr || !*Ps || !*PPs || !*Pr || !*PPr || !rcv_chunk ||
+        !img || (hbc && (!taper || !upb)))
+        return fail(RTMHIP_ERR_INVALID, "NULL pointer");
+    if (nsteps <= 0) return RTMHIP_OK;
+    if ((rc = check_chunk(shot, it_hi - nsteps + 1, it_hi))) return rc;
+    hipStream_t s = g_state.stream;
+    const int nt = shot->nt, b = shot->blen;
+    const int z0 = b, z1 = shot->nze - b;
+    for (int it = it_hi; it > it_hi - nsteps; --it) {
+        // Source wavefield at time it (RTMFiniteDifferencesKernel_HBC.cpp:283-309).
+        const float *S;
+        if (it == nt - 1) {
+            S = *PPs;  // snap1
+        } else if (it == nt - 2) {
+            S = *Ps;   // snap0 (entry state: Ps = snap0, PPs = snap1)
+        } else {
+            HIPCHK(stencil(c.radius, *Ps, *PPs, V2DT2, c.cf, c.g, nullptr), "stencil");
+            if (hbc)
+                HIPCHK(launch_upb(*PPs, (float *)upb, c.g, b, c.hf, it, true, s), "upb restore");
+            float *t = *Ps;
+            *Ps = *PPs;
+            *PPs = t;
+            S = *Ps;
+        }
+        // Receiver wavefield and imaging (:311-345).
+        if (hbc) {
+            HIPCHK(launch_taper(*Pr, taper, c.g, b, true, s), "taper");
+            HIPCHK(launch_taper(*PPr, taper, c.g, b, true, s), "taper");
+        }
+        const int lt = it % shot->ntstep;
+        if (fused) {
+            ImageFuse fz;
+            fz.img = img;
+            fz.ps = S;
+            fz.z0 = z0;
+            fz.z1 = z1;
+            fz.rz = c.rz;
+            fz.rx0 = c.rx0;
+            fz.rdx = shot->rcv.dist_x;
+            fz.rnx = shot->rcv.count_x;
+            fz.ry0 = c.ry0;
+            fz.rdy = shot->rcv.dist_y;
+            fz.rny = shot->rcv.count_y;
+            HIPCHK(stencil(c.radius, *Pr, *PPr, V2DT2, c.cf, c.g, &fz), "fused stencil");
+            HIPCHK(launch_receivers_image(*PPr, rcv_chunk, c.g, c.rx0, c.ry0, c.rz,
+                                          shot->rcv.dist_x, shot->rcv.dist_y, shot->rcv.count_x,
+                                          shot->rcv.count_y, shot->ntstep, lt, img, S, z0, z1,
+                                          c.radius, s),
+                   "receivers+image");
+        } else {
+            HIPCHK(stencil(c.radius, *Pr, *PPr, V2DT2, c.cf, c.g, nullptr), "stencil");
+            HIPCHK(launch_receivers(*PPr, (float *)rcv_chunk, c.g, c.rx0, c.ry0, c.rz,
+                                    shot->rcv.dist_x, shot->rcv.dist_y, shot->rcv.count_x,
+                                    shot->rcv.count_y, shot->ntstep, lt, true, s),
+                   "receivers");
+            HIPCHK(launch_image(img, S, *PPr, c.g, z0, z1, s), "image");
+        }
+        float *t = *Pr;
+        *Pr = *PPr;
+        *PPr = t;
+    }
+    return RTMHIP_OK;
+}
+
+int rtmhip_timing_enable(int on)
+{
+    NEED_INIT();
+    g_state.timing = on != 0;
+    return RTMHIP_OK;
+}
+
+int rtmhip_timing_read(int kernel_class, long long *launches, double *total_ms)
+{
+    NEED_INIT();
+    if (kernel_class < 0 || kernel_class > 1 || !launches || !total_ms)
+        return fail(RTMHIP_ERR_INVALID, "bad arguments");
+    HIPCHK(hipStreamSynchronize(g_state.stream), "hipStreamSynchronize");
+    Timing &t = g_state.tm[kernel_class];
+    double sum = 0.0;
+    for (size_t i = 0; i < t.used; ++i) {
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, t.start[i], t.stop[i]), "hipEventElapsedTime");
+        sum += ms;
+    }
+    *launches = (long long)t.used;
+    *total_ms = sum;
+    t.used = 0;
+    return RTMHIP_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,70 @@
+// rtm_internal.hpp -- shared declarations of the gfx950 kernels and their
+// launchers.  Not part of the public C ABI (include/rtm_hip.h).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+namespace rtmhip {
+
+// Stencil plane [dim][tap] (rtm-3d/include/RTMGrid.hpp:400-446), passed by
+// value so every tap lives in a scalar register.
+struct Coefs {
+    float c[3][9];
+};
+
+// Extended grid with explicit strides: offset = x*sx + y*sy + z.
+struct Grid {
+    int nx, ny, nz;
+    long long sx;  // plane stride (elements)
+    int sy;        // row stride (elements)
+};
+
+inline Grid dense_grid(int nx, int ny, int nz)
+{
+    return Grid{nx, ny, nz, (long long)ny * nz, nz};
+}
+
+// Epilogue of the fused receiver step: img += ps*pp on z in [z0, z1) except
+// at receiver cells, which the receiver kernel handles after injection.
+struct ImageFuse {
+    float *img;
+    const float *ps;
+    int z0, z1;
+    int rz;             // receiver plane (extended z)
+    int rx0, rdx, rnx;  // receiver x = rx0 + i*rdx, i < rnx (extended)
+    int ry0, rdy, rny;
+};
+
+// Tile configuration of the stencil kernel.
+struct StencilTiling {
+    int tz, by, ry;  // lanes along z, thread rows, rows per thread
+    int xchunk;      // planes marched per block (0 = automatic)
+};
+
+// Launchers: enqueue on `s`, return the launch status.
+hipError_t launch_stencil(int radius, const float *P, float *PP, const float *V, const Coefs &cf,
+                          const Grid &g, const ImageFuse *fuse, int *flag, hipStream_t s,
+                          const StencilTiling *tiling);
+hipError_t launch_taper(float *P, const float *taper, const Grid &g, int blen, bool upper_only,
+                        hipStream_t s);
+hipError_t launch_source(float *PP, long long offset, const float *table, int it, float value,
+                         hipStream_t s);
+hipError_t launch_receivers(float *PP, float *rcv, const Grid &g, int rx0, int ry0, int rz,
+                            int rdx, int rdy, int rnx, int rny, int ntstep, int lt, bool restore,
+                            hipStream_t s);
+hipError_t launch_receivers_image(float *PP, const float *rcv, const Grid &g, int rx0, int ry0,
+                                  int rz, int rdx, int rdy, int rnx, int rny, int ntstep, int lt,
+                                  float *img, const float *ps, int z0, int z1, int R,
+                                  hipStream_t s);
+hipError_t launch_upb(float *PP, float *upb, const Grid &g, int blen, int hf, long long lt,
+                      bool restore, hipStream_t s);
+hipError_t launch_image(float *img, const float *PS, const float *PR, const Grid &g, int z0,
+                        int z1, hipStream_t s);
+hipError_t launch_image_finish(const float *img, float *out, const Grid &g, int blen,
+                               const Coefs &fc, hipStream_t s);
+hipError_t launch_fill(float *p, size_t n, float v, hipStream_t s);
+hipError_t launch_stack(float *dst, const float *src, size_t n, hipStream_t s);
+
+}  // namespace rtmhip

@@ -1,0 +1,415 @@
+// rtm_ops.hip -- border, source, receiver, upper-border and imaging kernels
+// for gfx950.  Each restates one operator of the reference CPU platform
+// (rtm-3d/src/platform/host/RTMCPUPlatform.cpp) with the same per-cell
+// arithmetic; they are bandwidth-trivial next to the stencil and are kept
+// simple and coalesced (z on consecutive lanes).
+#include "rtm_internal.hpp"
+
+#pragma clang fp contract(off)
+
+namespace rtmhip {
+namespace {
+
+constexpr int kThreads = 256;
+
+__device__ __forceinline__ long long goff(const Grid &g, int x, int y, int z)
+{
+    return (long long)x * g.sx + (long long)y * g.sy + z;
+}
+
+// Taper factor along one axis of length n: g[b-1-i] in the low border,
+// g[i-(n-b)] in the high border.
+__device__ __forceinline__ float gfac(const float *__restrict__ t, int i, int n, int b)
+{
+    return i < b ? t[b - 1 - i] : t[i - (n - b)];
+}
+
+// taperUpper3D (RTMCPUPlatform.cpp:566-657): every cell with z < blen.
+//   x, y interior           : v*gz
+//   x in a border (any y)   : ((v*gy if y in a border)*gx)*gz
+//   y in a border, x inner  : (v*gy)*gz
+__global__ void __launch_bounds__(kThreads)
+    k_taper_upper(float *__restrict__ P, const float *__restrict__ t, Grid g, int b)
+{
+    const long long n = (long long)g.nx * g.ny * b;
+    const long long i = (long long)blockIdx.x * kThreads + threadIdx.x;
+    if (i >= n) return;
+    const int z = (int)(i % b);
+    const long long r = i / b;
+    const int y = (int)(r % g.ny);
+    const int x = (int)(r / g.ny);
+    const bool bx = x < b || x >= g.nx - b;
+    const bool by = y < b || y >= g.ny - b;
+    const long long o = goff(g, x, y, z);
+    float v = P[o];
+    const float gz = t[b - 1 - z];
+    if (!bx && !by) {
+        v *= gz;
+    } else if (bx) {
+        if (by) v *= gfac(t, y, g.ny, b);
+        v *= gfac(t, x, g.nx, b);
+        v *= gz;
+    } else {
+        v *= gfac(t, y, g.ny, b);
+        v *= gz;
+    }
+    P[o] = v;
+}
+
+// The front-right edge of taper3D (RTMCPUPlatform.cpp:900-904) writes cells
+// x < blen, y >= ny-blen, z interior from the face-tapered cell
+// (blen, y, z).  This kernel runs before k_taper_all so it reads that cell
+// before the face pass rescales it:  ((P(blen,y,z)*gy)*gx)*gy.
+__global__ void __launch_bounds__(kThreads)
+    k_taper_front_right(float *__restrict__ P, const float *__restrict__ t, Grid g, int b)
+{
+    const int lz = g.nz - 2 * b;
+    const long long n = (long long)b * b * lz;
+    const long long i = (long long)blockIdx.x * kThreads + threadIdx.x;
+    if (i >= n) return;
+    const int z = b + (int)(i % lz);
+    const long long r = i / lz;
+    const int iy = (int)(r % b);
+    const int x = (int)(r / b);
+    const int y = g.ny - b + iy;
+    const float gy = t[iy];
+    float v = P[goff(g, b, y, z)];
+    v *= gy;
+    v *= t[b - 1 - x];
+    v *= gy;
+    P[goff(g, x, y, z)] = v;
+}
+
+// taper3D (RTMCPUPlatform.cpp:710-920) for every border cell except the
+// front-right edge.  Work is enumerated as three disjoint slabs:
+//   A: z border (all x, y)      nx*ny*2b cells
+//   B: y border, z inner        nx*2b*lz
+//   C: x border, y and z inner  2b*ly*lz
+// Per-cell factor order follows the reference loop nests.
+__global__ void __launch_bounds__(kThreads)
+    k_taper_all(float *__restrict__ P, const float *__restrict__ t, Grid g, int b)
+{
+    const int lz = g.nz - 2 * b, ly = g.ny - 2 * b;
+    const long long nA = (long long)g.nx * g.ny * 2 * b;
+    const long long nB = (long long)g.nx * 2 * b * lz;
+    const long long nC = (long long)2 * b * ly * lz;
+    long long i = (long long)blockIdx.x * kThreads + threadIdx.x;
+    int x, y, z;
+    if (i < nA) {
+        const int zz = (int)(i % (2 * b));
+        const long long r = i / (2 * b);
+        y = (int)(r % g.ny);
+        x = (int)(r / g.ny);
+        z = zz < b ? zz : g.nz - 2 * b + zz;
+    } else if ((i -= nA) < nB) {
+        const int zi = (int)(i % lz);
+        const long long r = i / lz;
+        const int yy = (int)(r % (2 * b));
+        x = (int)(r / (2 * b));
+        y = yy < b ? yy : g.ny - 2 * b + yy;
+        z = b + zi;
+    } else if ((i -= nB) < nC) {
+        const int zi = (int)(i % lz);
+        const long long r = i / lz;
+        const int yi = (int)(r % ly);
+        const int xx = (int)(r / ly);
+        x = xx < b ? xx : g.nx - 2 * b + xx;
+        y = b + yi;
+        z = b + zi;
+    } else {
+        return;
+    }
+    const bool bx = x < b || x >= g.nx - b;
+    const bool by = y < b || y >= g.ny - b;
+    const bool bz = z < b || z >= g.nz - b;
+    if (bx && by && !bz && x < b && y >= g.ny - b) return;  // front-right edge
+    const long long o = goff(g, x, y, z);
+    float v = P[o];
+    if (bx && bz) {
+        if (by) v *= gfac(t, y, g.ny, b);
+        v *= gfac(t, x, g.nx, b);
+        v *= gfac(t, z, g.nz, b);
+    } else if (by && bz) {
+        v *= gfac(t, y, g.ny, b);
+        v *= gfac(t, z, g.nz, b);
+    } else if (bx && by) {
+        v *= gfac(t, x, g.nx, b);
+        v *= gfac(t, y, g.ny, b);
+    } else if (bz) {
+        v *= gfac(t, z, g.nz, b);
+    } else if (by) {
+        v *= gfac(t, y, g.ny, b);
+    } else {
+        v *= gfac(t, x, g.nx, b);
+    }
+    P[o] = v;
+}
+
+// rtmApplySource (RTMCPUPlatform.cpp:103-113): PP[s] += value (or table[it]).
+__global__ void k_source(float *__restrict__ PP, long long o, const float *__restrict__ table,
+                         int it, float value)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        const float v = table ? table[it] : value;
+        PP[o] = PP[o] + v;
+    }
+}
+
+// rtmSaveReceiverData / rtmRestoreReceiverData (RTMCPUPlatform.cpp:114-151).
+__global__ void __launch_bounds__(kThreads)
+    k_receivers(float *__restrict__ PP, float *__restrict__ rcv, Grid g, int rx0, int ry0, int rz,
+                int rdx, int rdy, int rnx, int rny, int ntstep, int lt, int restore)
+{
+    const int i = blockIdx.x * kThreads + threadIdx.x;
+    if (i >= rnx * rny) return;
+    const int ix = i / rny, iy = i % rny;
+    const int rx = rx0 + ix * rdx, ry = ry0 + iy * rdy;
+    if (rx < 0 || rx >= g.nx || ry < 0 || ry >= g.ny) return;
+    const long long o = goff(g, rx, ry, rz);
+    const long long ro = (long long)i * ntstep + lt;
+    if (restore) PP[o] = rcv[ro];
+    else rcv[ro] = PP[o];
+}
+
+// Epilogue of the fused backward receiver step.  Threads [0, rnx*rny):
+// receiver injection (SET) followed by the imaging condition at the receiver
+// cells.  Remaining threads: the imaging condition on the x/y frame of width
+// R that the stencil kernel does not visit (all x, y are imaged by the
+// reference, RTMCPUPlatform.cpp:513-540), skipping receiver cells.
+__device__ __forceinline__ bool is_receiver(int x, int y, int z, int rx0, int ry0, int rz,
+                                            int rdx, int rdy, int rnx, int rny)
+{
+    const int dx = x - rx0, dy = y - ry0;
+    return z == rz && dx >= 0 && dy >= 0 && (dx % rdx) == 0 && (dy % rdy) == 0 &&
+           (dx / rdx) < rnx && (dy / rdy) < rny;
+}
+
+__global__ void __launch_bounds__(kThreads)
+    k_receivers_image(float *__restrict__ PP, const float *__restrict__ rcv, Grid g, int rx0,
+                      int ry0, int rz, int rdx, int rdy, int rnx, int rny, int ntstep, int lt,
+                      float *__restrict__ img, const float *__restrict__ ps, int z0, int z1,
+                      int R)
+{
+    long long i = (long long)blockIdx.x * kThreads + threadIdx.x;
+    const long long nrcv = (long long)rnx * rny;
+    if (i < nrcv) {
+        const int ix = (int)(i / rny), iy = (int)(i % rny);
+        const int rx = rx0 + ix * rdx, ry = ry0 + iy * rdy;
+        if (rx < 0 || rx >= g.nx || ry < 0 || ry >= g.ny) return;
+        const long long o = goff(g, rx, ry, rz);
+        const float v = rcv[i * ntstep + lt];
+        PP[o] = v;
+        if (rz >= z0 && rz < z1) img[o] = img[o] + ps[o] * v;
+        return;
+    }
+    i -= nrcv;
+    const int lz = z1 - z0;
+    if (lz <= 0) return;
+    const long long nA = (long long)2 * R * g.ny * lz;
+    const long long nB = (long long)(g.nx - 2 * R) * 2 * R * lz;
+    int x, y, z;
+    if (i < nA) {
+        z = z0 + (int)(i % lz);
+        const long long r = i / lz;
+        y = (int)(r % g.ny);
+        const int xx = (int)(r / g.ny);
+        x = xx < R ? xx : g.nx - 2 * R + xx;
+    } else if ((i -= nA) < nB) {
+        z = z0 + (int)(i % lz);
+        const long long r = i / lz;
+        const int yy = (int)(r % (2 * R));
+        x = R + (int)(r / (2 * R));
+        y = yy < R ? yy : g.ny - 2 * R + yy;
+    } else {
+        return;
+    }
+    if (is_receiver(x, y, z, rx0, ry0, rz, rdx, rdy, rnx, rny)) return;
+    const long long o = goff(g, x, y, z);
+    img[o] = img[o] + ps[o] * PP[o];
+}
+
+// rtmSaveUpperBorder / rtmRestoreUpperBorder (RTMCPUPlatform.cpp:153-189):
+// upb[lt][x][y][k] <-> PP[x][y][blen-hf+k], 64-bit offsets.
+__global__ void __launch_bounds__(kThreads)
+    k_upb(float *__restrict__ PP, float *__restrict__ upb, Grid g, int b, int hf, long long lt,
+          int restore)
+{
+    const long long n = (long long)g.nx * g.ny * hf;
+    const long long i = (long long)blockIdx.x * kThreads + threadIdx.x;
+    if (i >= n) return;
+    const int k = (int)(i % hf);
+    const long long r = i / hf;
+    const int y = (int)(r % g.ny);
+    const int x = (int)(r / g.ny);
+    const long long u = lt * n + i;
+    const long long o = goff(g, x, y, b - hf + k);
+    if (restore) PP[o] = upb[u];
+    else upb[u] = PP[o];
+}
+
+// rtmImageCondition (RTMCPUPlatform.cpp:513-540): z in [z0, z1), all x, y.
+__global__ void __launch_bounds__(kThreads)
+    k_image(float *__restrict__ img, const float *__restrict__ PS, const float *__restrict__ PR,
+            Grid g, int z0, int z1)
+{
+    const int lz = z1 - z0;
+    const long long n = (long long)g.nx * g.ny * lz;
+    const long long i = (long long)blockIdx.x * kThreads + threadIdx.x;
+    if (i >= n) return;
+    const int z = z0 + (int)(i % lz);
+    const long long r = i / lz;
+    const int y = (int)(r % g.ny);
+    const int x = (int)(r / g.ny);
+    const long long o = goff(g, x, y, z);
+    img[o] = img[o] + PS[o] * PR[o];
+}
+
+// Order-2 Laplacian filter (RTMGrid.cpp:226-300) evaluated only where
+// removeBorders (RTMGrid.cpp:129-166) keeps it; out is dense [nx][ny][nz].
+__global__ void __launch_bounds__(kThreads)
+    k_image_finish(const float *__restrict__ img, float *__restrict__ out, Grid g, int b,
+                   Coefs fc)
+{
+    const int nx = g.nx - 2 * b, ny = g.ny - 2 * b, nz = g.nz - 2 * b;
+    const long long n = (long long)nx * ny * nz;
+    const long long i = (long long)blockIdx.x * kThreads + threadIdx.x;
+    if (i >= n) return;
+    const int z = (int)(i % nz);
+    const long long r = i / nz;
+    const int y = (int)(r % ny);
+    const int x = (int)(r / ny);
+    const long long o = goff(g, x + b, y + b, z + b);
+    float ax = 0.f, ay = 0.f, az = 0.f;
+#pragma unroll
+    for (int io = 0; io < 3; ++io) {
+        ax += img[o + (io - 1) * g.sx] * fc.c[0][io];
+        ay += img[o + (long long)(io - 1) * g.sy] * fc.c[1][io];
+        az += img[o + (io - 1)] * fc.c[2][io];
+    }
+    out[i] = ax + ay + az;
+}
+
+__global__ void __launch_bounds__(kThreads) k_fill(float *__restrict__ p, size_t n, float v)
+{
+    for (size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * kThreads)
+        p[i] = v;
+}
+
+__global__ void __launch_bounds__(kThreads)
+    k_stack(float *__restrict__ d, const float *__restrict__ s, size_t n)
+{
+    for (size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * kThreads)
+        d[i] = d[i] + s[i];
+}
+
+inline unsigned nblocks(long long n)
+{
+    return (unsigned)((n + kThreads - 1) / kThreads);
+}
+
+inline unsigned stride_blocks(size_t n)
+{
+    size_t b = (n + kThreads - 1) / kThreads;
+    return (unsigned)(b > 8192 ? 8192 : (b ? b : 1));
+}
+
+}  // namespace
+
+hipError_t launch_taper(float *P, const float *taper, const Grid &g, int b, bool upper_only,
+                        hipStream_t s)
+{
+    if (b <= 0) return hipSuccess;
+    if (upper_only) {
+        const long long n = (long long)g.nx * g.ny * b;
+        hipLaunchKernelGGL(k_taper_upper, dim3(nblocks(n)), dim3(kThreads), 0, s, P, taper, g, b);
+        return hipGetLastError();
+    }
+    const long long lz = g.nz - 2 * b, ly = g.ny - 2 * b;
+    const long long nfr = (long long)b * b * lz;
+    if (nfr > 0 && g.nx > 2 * b) {
+        hipLaunchKernelGGL(k_taper_front_right, dim3(nblocks(nfr)), dim3(kThreads), 0, s, P,
+                           taper, g, b);
+    }
+    const long long n = (long long)g.nx * g.ny * 2 * b + (long long)g.nx * 2 * b * lz +
+                        (long long)2 * b * ly * lz;
+    hipLaunchKernelGGL(k_taper_all, dim3(nblocks(n)), dim3(kThreads), 0, s, P, taper, g, b);
+    return hipGetLastError();
+}
+
+hipError_t launch_source(float *PP, long long offset, const float *table, int it, float value,
+                         hipStream_t s)
+{
+    hipLaunchKernelGGL(k_source, dim3(1), dim3(64), 0, s, PP, offset, table, it, value);
+    return hipGetLastError();
+}
+
+hipError_t launch_receivers(float *PP, float *rcv, const Grid &g, int rx0, int ry0, int rz,
+                            int rdx, int rdy, int rnx, int rny, int ntstep, int lt, bool restore,
+                            hipStream_t s)
+{
+    const long long n = (long long)rnx * rny;
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_receivers, dim3(nblocks(n)), dim3(kThreads), 0, s, PP, rcv, g, rx0, ry0,
+                       rz, rdx, rdy, rnx, rny, ntstep, lt, restore ? 1 : 0);
+    return hipGetLastError();
+}
+
+hipError_t launch_receivers_image(float *PP, const float *rcv, const Grid &g, int rx0, int ry0,
+                                  int rz, int rdx, int rdy, int rnx, int rny, int ntstep, int lt,
+                                  float *img, const float *ps, int z0, int z1, int R,
+                                  hipStream_t s)
+{
+    const long long lz = z1 > z0 ? z1 - z0 : 0;
+    const long long n = (long long)rnx * rny + (long long)2 * R * g.ny * lz +
+                        (long long)(g.nx - 2 * R) * 2 * R * lz;
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_receivers_image, dim3(nblocks(n)), dim3(kThreads), 0, s, PP, rcv, g, rx0,
+                       ry0, rz, rdx < 1 ? 1 : rdx, rdy < 1 ? 1 : rdy, rnx, rny, ntstep, lt, img,
+                       ps, z0, z1, R);
+    return hipGetLastError();
+}
+
+hipError_t launch_upb(float *PP, float *upb, const Grid &g, int b, int hf, long long lt,
+                      bool restore, hipStream_t s)
+{
+    const long long n = (long long)g.nx * g.ny * hf;
+    hipLaunchKernelGGL(k_upb, dim3(nblocks(n)), dim3(kThreads), 0, s, PP, upb, g, b, hf, lt,
+                       restore ? 1 : 0);
+    return hipGetLastError();
+}
+
+hipError_t launch_image(float *img, const float *PS, const float *PR, const Grid &g, int z0,
+                        int z1, hipStream_t s)
+{
+    if (z1 <= z0) return hipSuccess;
+    const long long n = (long long)g.nx * g.ny * (z1 - z0);
+    hipLaunchKernelGGL(k_image, dim3(nblocks(n)), dim3(kThreads), 0, s, img, PS, PR, g, z0, z1);
+    return hipGetLastError();
+}
+
+hipError_t launch_image_finish(const float *img, float *out, const Grid &g, int b,
+                               const Coefs &fc, hipStream_t s)
+{
+    const long long n = (long long)(g.nx - 2 * b) * (g.ny - 2 * b) * (g.nz - 2 * b);
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_image_finish, dim3(nblocks(n)), dim3(kThreads), 0, s, img, out, g, b, fc);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill(float *p, size_t n, float v, hipStream_t s)
+{
+    if (!n) return hipSuccess;
+    hipLaunchKernelGGL(k_fill, dim3(stride_blocks(n)), dim3(kThreads), 0, s, p, n, v);
+    return hipGetLastError();
+}
+
+hipError_t launch_stack(float *dst, const float *src, size_t n, hipStream_t s)
+{
+    if (!n) return hipSuccess;
+    hipLaunchKernelGGL(k_stack, dim3(stride_blocks(n)), dim3(kThreads), 0, s, dst, src, n);
+    return hipGetLastError();
+}
+
+}  // namespace rtmhip

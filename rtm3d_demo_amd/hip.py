@@ -1,0 +1,261 @@
+"""ctypes binding of librtm3d_hip.so (the C ABI in include/rtm_hip.h).
+
+Host-side plumbing for tests, the benchmark and scripting.  There is no CPU
+fallback: if the library is missing or no GPU is present, every entry raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(PKG, "lib", "librtm3d_hip.so")
+
+OK, ERR_INVALID, ERR_HIP, ERR_DISPERSION, ERR_NOT_INIT, ERR_UNSUPPORTED = range(6)
+
+EXPORTED = [
+    "rtmhip_init", "rtmhip_finalize", "rtmhip_last_error", "rtmhip_device_count",
+    "rtmhip_synchronize", "rtmhip_version", "rtmhip_alloc", "rtmhip_free",
+    "rtmhip_copy_to_device", "rtmhip_copy_from_device", "rtmhip_copy_device", "rtmhip_fill",
+    "rtmhip_step", "rtmhip_taper", "rtmhip_apply_source", "rtmhip_receivers",
+    "rtmhip_upper_border", "rtmhip_image_condition", "rtmhip_image_finish", "rtmhip_stack",
+    "rtmhip_model_steps", "rtmhip_forward_steps", "rtmhip_backward_steps",
+    "rtmhip_timing_enable", "rtmhip_timing_read",
+]
+
+
+class RtmError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("rtmhip error %d: %s" % (code, msg))
+        self.code = code
+
+
+class RcvGeom(C.Structure):
+    _fields_ = [(n, C.c_int) for n in
+                ("start_x", "start_y", "depth_z", "dist_x", "dist_y", "count_x", "count_y")]
+
+
+class Shot(C.Structure):
+    _fields_ = [("nxe", C.c_int), ("nye", C.c_int), ("nze", C.c_int), ("blen", C.c_int),
+                ("order", C.c_int), ("nt", C.c_int), ("ntstep", C.c_int), ("sxe", C.c_int),
+                ("sye", C.c_int), ("sze", C.c_int), ("rcv", RcvGeom)]
+
+
+_lib = None
+
+
+def load():
+    """Load the library; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIBPATH):
+        raise ImportError("librtm3d_hip.so not built (%s); run __graft_entry__.build()" % LIBPATH)
+    L = C.CDLL(LIBPATH)
+    i, vp, fp = C.c_int, C.c_void_p, C.c_void_p
+    sz = C.c_size_t
+    pp = C.POINTER(C.c_void_p)
+    sig = {
+        "rtmhip_init": [i], "rtmhip_finalize": [], "rtmhip_device_count": [C.POINTER(C.c_int)],
+        "rtmhip_synchronize": [], "rtmhip_version": [],
+        "rtmhip_alloc": [pp, sz], "rtmhip_free": [vp],
+        "rtmhip_copy_to_device": [vp, vp, sz], "rtmhip_copy_from_device": [vp, vp, sz],
+        "rtmhip_copy_device": [vp, vp, sz], "rtmhip_fill": [fp, sz, C.c_float],
+        "rtmhip_step": [fp, fp, fp, fp, i, i, i, i],
+        "rtmhip_taper": [fp, fp, i, i, i, i, i],
+        "rtmhip_apply_source": [fp, i, i, i, i, i, i, C.c_float],
+        "rtmhip_receivers": [fp, fp, i, i, i, i, C.POINTER(RcvGeom), i, i, i],
+        "rtmhip_upper_border": [fp, fp, i, i, i, i, i, i, i],
+        "rtmhip_image_condition": [fp, fp, fp, i, i, i, i],
+        "rtmhip_image_finish": [fp, fp, i, i, i, i, fp],
+        "rtmhip_stack": [fp, fp, sz],
+        "rtmhip_model_steps": [C.POINTER(Shot), fp, fp, fp, fp, pp, pp, fp, i, i],
+        "rtmhip_forward_steps": [C.POINTER(Shot), i, fp, fp, fp, fp, pp, pp, fp, i, i],
+        "rtmhip_backward_steps": [C.POINTER(Shot), i, fp, fp, fp, pp, pp, pp, pp, fp, fp, fp,
+                                  i, i, i],
+        "rtmhip_timing_enable": [i],
+        "rtmhip_timing_read": [i, C.POINTER(C.c_longlong), C.POINTER(C.c_double)],
+    }
+    for name, args in sig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = C.c_int
+    L.rtmhip_last_error.argtypes = []
+    L.rtmhip_last_error.restype = C.c_char_p
+    _lib = L
+    return L
+
+
+def check(rc):
+    if rc != OK:
+        raise RtmError(rc, load().rtmhip_last_error().decode(errors="replace"))
+
+
+def _host_ptr(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+class DeviceArray:
+    """A device buffer owned by the caller (the C ABI only borrows it)."""
+
+    def __init__(self, shape, dtype=np.float32):
+        self.shape = tuple(int(s) for s in shape)
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        p = C.c_void_p()
+        check(load().rtmhip_alloc(C.byref(p), self.nbytes))
+        self.ptr = p
+
+    @property
+    def size(self):
+        return int(np.prod(self.shape))
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a, self.dtype)
+        assert a.nbytes == self.nbytes, (a.shape, self.shape)
+        check(load().rtmhip_copy_to_device(self.ptr, _host_ptr(a), self.nbytes))
+        return self
+
+    def download(self):
+        out = np.empty(self.shape, self.dtype)
+        check(load().rtmhip_copy_from_device(_host_ptr(out), self.ptr, self.nbytes))
+        return out
+
+    def zero(self):
+        check(load().rtmhip_fill(self.ptr, self.size, 0.0))
+        return self
+
+    def free(self):
+        if self.ptr:
+            check(load().rtmhip_free(self.ptr))
+            self.ptr = C.c_void_p()
+
+    @classmethod
+    def from_host(cls, a, dtype=np.float32):
+        d = cls(np.shape(a), dtype)
+        return d.upload(a)
+
+
+def init(device=0):
+    check(load().rtmhip_init(device))
+
+
+def finalize():
+    check(load().rtmhip_finalize())
+
+
+def synchronize():
+    check(load().rtmhip_synchronize())
+
+
+def device_count():
+    n = C.c_int(0)
+    check(load().rtmhip_device_count(C.byref(n)))
+    return n.value
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, np.float32)
+
+
+# ---------------- per-step operators ----------------
+def step(P, PP, V, coefs, order):
+    c = _f32(coefs)
+    check(load().rtmhip_step(P.ptr, PP.ptr, V.ptr, _host_ptr(c), *P.shape, order))
+
+
+def taper(P, g, blen, upper_only):
+    check(load().rtmhip_taper(P.ptr, g.ptr, *P.shape, blen, int(upper_only)))
+
+
+def apply_source(PP, sxe, sye, sze, value):
+    check(load().rtmhip_apply_source(PP.ptr, *PP.shape, sxe, sye, sze, float(value)))
+
+
+def receivers(PP, rcv, blen, geom, ntstep, lt, restore):
+    check(load().rtmhip_receivers(PP.ptr, rcv.ptr, *PP.shape, blen, C.byref(geom), ntstep, lt,
+                                  int(restore)))
+
+
+def upper_border(PP, upb, blen, order, lt, restore):
+    check(load().rtmhip_upper_border(PP.ptr, upb.ptr, *PP.shape, blen, order, lt, int(restore)))
+
+
+def image_condition(img, PS, PR, blen):
+    check(load().rtmhip_image_condition(img.ptr, PS.ptr, PR.ptr, *img.shape, blen))
+
+
+def image_finish(img, out, blen, fcoefs):
+    c = _f32(fcoefs)
+    check(load().rtmhip_image_finish(img.ptr, out.ptr, *img.shape, blen, _host_ptr(c)))
+
+
+def stack(dst, src):
+    check(load().rtmhip_stack(dst.ptr, src.ptr, dst.size))
+
+
+# ---------------- time loops ----------------
+def make_shot(nxe, nye, nze, blen, order, nt, ntstep, src_ext, geom):
+    return Shot(nxe, nye, nze, blen, order, nt, ntstep, src_ext[0], src_ext[1], src_ext[2], geom)
+
+
+class FieldPair:
+    """Two wavefield buffers swapped in place by the time loops."""
+
+    def __init__(self, shape):
+        self.a = DeviceArray(shape).zero()
+        self.b = DeviceArray(shape).zero()
+        self.p = C.c_void_p(self.a.ptr.value)
+        self.pp = C.c_void_p(self.b.ptr.value)
+
+    def by_ptr(self, v):
+        return self.a if v.value == self.a.ptr.value else self.b
+
+    @property
+    def P(self):
+        return self.by_ptr(self.p)
+
+    @property
+    def PP(self):
+        return self.by_ptr(self.pp)
+
+    def free(self):
+        self.a.free()
+        self.b.free()
+
+
+def model_steps(shot, V, coefs, g, ricker, fields, rcv_chunk, it0, nsteps):
+    c = _f32(coefs)
+    check(load().rtmhip_model_steps(C.byref(shot), V.ptr, _host_ptr(c), g.ptr, ricker.ptr,
+                                    C.byref(fields.p), C.byref(fields.pp), rcv_chunk.ptr, it0,
+                                    nsteps))
+
+
+def forward_steps(shot, hbc, V, coefs, g, ricker, fields, upb, it0, nsteps):
+    c = _f32(coefs)
+    check(load().rtmhip_forward_steps(C.byref(shot), int(hbc), V.ptr, _host_ptr(c),
+                                      g.ptr if g is not None else None, ricker.ptr,
+                                      C.byref(fields.p), C.byref(fields.pp),
+                                      upb.ptr if upb is not None else None, it0, nsteps))
+
+
+def backward_steps(shot, hbc, V, coefs, g, src_fields, rcv_fields, upb, rcv_chunk, img, it_hi,
+                   nsteps, fused=True):
+    c = _f32(coefs)
+    check(load().rtmhip_backward_steps(C.byref(shot), int(hbc), V.ptr, _host_ptr(c),
+                                       g.ptr if g is not None else None,
+                                       C.byref(src_fields.p), C.byref(src_fields.pp),
+                                       C.byref(rcv_fields.p), C.byref(rcv_fields.pp),
+                                       upb.ptr if upb is not None else None, rcv_chunk.ptr,
+                                       img.ptr, it_hi, nsteps, int(fused)))
+
+
+def timing_enable(on=True):
+    check(load().rtmhip_timing_enable(int(on)))
+
+
+def timing_read(kernel_class):
+    n = C.c_longlong(0)
+    ms = C.c_double(0.0)
+    check(load().rtmhip_timing_read(kernel_class, C.byref(n), C.byref(ms)))
+    return n.value, ms.value

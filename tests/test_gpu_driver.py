@@ -1,0 +1,86 @@
+"""The rtm3d driver binary end to end on the GPU: the reference's JSON in,
+the reference's files out, bit-identical to the reference program's files
+(seismogram chunks, extended velocity, migrated image)."""
+import json
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from rtm3d_demo_amd import DRIVER
+
+pytestmark = pytest.mark.gpu
+
+
+def _run(wd, params):
+    path = os.path.join(wd, "input.json")
+    with open(path, "w") as f:
+        json.dump(params, f)
+    r = subprocess.run([DRIVER, path], cwd=wd, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout[-3000:] + r.stderr[-3000:]
+    return r
+
+
+def _seism(p, s, itstep):
+    return "RTMSEISMOGRAM_S%dx%dx%d_NXNY_%dx%d_NTSTART%d_NTSTEP%d.seism" % (
+        s[0], s[1], s[2], p["receiver_count_x"], p["receiver_count_y"], itstep, p["ntstep"])
+
+
+def _shots(p):
+    return [(ix * p["source_distance_x"] + p["source_start_x"],
+             iy * p["source_distance_y"] + p["source_start_y"], p["source_depth_z"])
+            for ix in range(p["source_count_x"]) for iy in range(p["source_count_y"])]
+
+
+@pytest.fixture
+def workdir(tmp_path, golden):
+    for d in ("vel", "out", "tmp", "dat"):
+        (tmp_path / d).mkdir()
+    golden("small_mod")["vel"].tofile(str(tmp_path / "vel" / "model.bin"))
+    return str(tmp_path)
+
+
+def test_driver_modeling_then_migration(workdir, cases, golden):
+    print("driver small case", flush=True)
+    p = dict(cases["small_mod"])
+    p["verbose"] = True
+    _run(workdir, p)
+    recs = golden("small_mod")["records"]
+    for k, s in enumerate(_shots(p)):
+        for c, itstep in enumerate(range(0, p["nt"], p["ntstep"])):
+            got = np.fromfile(os.path.join(workdir, "dat", _seism(p, s, itstep)), np.float32)
+            assert np.array_equal(got.view(np.uint32), recs[k, c].ravel().view(np.uint32))
+    for bc in ("hbc", "rbc"):
+        q = dict(cases["small_" + bc])
+        _run(workdir, q)
+        vpe = np.fromfile(os.path.join(workdir, "vel", "model_ext.bin"), np.float32)
+        assert np.array_equal(vpe, golden("small_" + bc)["vpe"].ravel())
+        name = "RTMIMG_%s_NS2_NXNYNZ_%dx%dx%d_NT%d.bin.NP1" % (
+            q["mname"].upper(), q["nx"], q["ny"], q["nz"], q["nt"])
+        img = np.fromfile(os.path.join(workdir, "out", name), np.float32)
+        assert np.array_equal(img.view(np.uint32), golden("small_" + bc)["image"].ravel().view(np.uint32))
+
+
+def test_driver_snapshots(workdir, cases):
+    p = dict(cases["small_mod"])
+    p.update({"save_snapshots": True, "snapshot_step": 50, "source_count_x": 1})
+    _run(workdir, p)
+    nye, nze = p["ny"] + 2 * p["blen"], p["nz"] + 2 * p["blen"]
+    name = "RTMSNAPSHOT_S%dx%d_%dx%dx%d.snapshot.RTM_MOD" % (
+        p["source_start_x"] + p["blen"], p["source_start_y"] + p["blen"], p["nt"] // 50, nye, nze)
+    snap = np.fromfile(os.path.join(workdir, "tmp", name), np.float32)
+    assert snap.size == (p["nt"] // 50) * nye * nze
+    assert np.isfinite(snap).all() and np.abs(snap[-nye * nze:]).max() > 0
+
+
+def test_driver_reports_dispersion(workdir, cases):
+    """An unstable time step blows the field up to NaN: exit status 1 with
+    the reference's 'Numerical Dispersion' message."""
+    p = dict(cases["small_mod"])
+    p.update({"dt": 0.05, "nt": 300, "ntstep": 300, "source_count_x": 1})
+    path = os.path.join(workdir, "input.json")
+    with open(path, "w") as f:
+        json.dump(p, f)
+    r = subprocess.run([DRIVER, path], cwd=workdir, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 1 and "Numerical Dispersion" in r.stdout
