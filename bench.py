@@ -192,6 +192,7 @@ def main():
     ap.add_argument("--nt", type=int, default=3000)
     ap.add_argument("--cpu-steps", type=int, default=6)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-tune", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on one GPU only: the propagation path is measured on a single "
@@ -204,6 +205,18 @@ def main():
     sh = Shot(hip, host, workloads, args.nt)
     log("modeling pass: %.3f s (%.0f Mcell-updates/s)"
         % (sh.modeling_s, sh.U * args.nt / sh.modeling_s / 1e6))
+    tiling = None
+    if not args.no_tune:
+        from rtm3d_demo_amd import autotune
+        p = sh.p
+        tshot = hip.make_shot(*sh.shape, sh.b, sh.order, 8, 8,
+                              (p["source_start_x"] + sh.b, p["source_start_y"] + sh.b,
+                               p["source_depth_z"] + sh.b), sh.geom)
+        t = time.time()
+        best = autotune.tune(hip, tshot, sh.V, sh.coefs, sh.g, sh.src, sh.F, sh.R, sh.upb,
+                             sh.rcv, sh.img, steps=6, log=log)
+        tiling = autotune.describe(best)
+        log("autotune (%.1f s): %s" % (time.time() - t, tiling))
     for i in range(args.warmup):
         sh.migrate()
         hip.synchronize()
@@ -218,6 +231,7 @@ def main():
     t1 = time.perf_counter()
     n0, ms0 = hip.timing_read(0)
     n1, ms1 = hip.timing_read(1)
+    n2, ms2 = hip.timing_read(2)
     hip.timing_enable(False)
     img = sh.img.download()
     if not np.isfinite(img).all() or not np.abs(img).max() > 0:
@@ -252,6 +266,12 @@ def main():
         "fused_receiver_step": {"launches": n1, "avg_ms": round(avg1 * 1e3, 4),
                                 "achieved_GBs": round(achieved_fused, 1) if achieved_fused else None,
                                 "algorithmic_bytes_per_launch": 28.0 * sh.U},
+        "tapered_forward_step": {"launches": n2,
+                                 "avg_ms": round(ms2 / max(n2, 1), 4),
+                                 "achieved_GBs": round(bytes_per_launch / (ms2 / max(n2, 1) * 1e-3)
+                                                       / 1e9, 1) if n2 else None,
+                                 "algorithmic_bytes_per_launch": bytes_per_launch},
+        "tiling": tiling,
     }
     for x in (sh.V, sh.src, sh.g, sh.F, sh.R, sh.rcv, sh.img, sh.upb):
         x.free()

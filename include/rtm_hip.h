@@ -175,12 +175,24 @@ int rtmhip_backward_steps(const rtmhip_shot *shot, int hbc, const float *V2DT2,
                           float **Pr, float **PPr, const float *upb, const float *rcv_chunk,
                           float *img, int it_hi, int nsteps, int fused);
 
-/* ================= measurement =================
- * When enabled, every launch of the stencil kernels is bracketed by HIP
+/* ================= tuning and measurement =================
+ * Stencil kernel classes: 0 = plain step (rtmhip_step, backward source
+ * reconstruction, RBC), 1 = receiver step fused with receiver injection and
+ * imaging, 2 = HBC forward step with the upper taper fused.  No reference
+ * counterpart: the reference's launch shape is fixed (10x10x10 blocks,
+ * RTMGPUPlatform.cpp:176-185).
+ *
+ * rtmhip_set_tiling selects the tile shape of one class: tz lanes along z,
+ * by thread rows, ry rows per thread (compiled set: 64x4x4, 64x4x2, 64x8x2,
+ * 32x8x4, 32x8x2), xchunk planes marched per block (0 = sized from the
+ * device's resident-block capacity).  tz = 0 restores the default 64x4x4. */
+int rtmhip_set_tiling(int kernel_class, int tz, int by, int ry, int xchunk);
+int rtmhip_get_tiling(int kernel_class, int *tz, int *by, int *ry, int *xchunk);
+
+/* When enabled, every launch of the stencil kernels is bracketed by HIP
  * events on the library stream; rtmhip_timing_read() synchronises and returns
- * the number of launches and their summed duration per kernel class
- * (0 = plain stencil step, 1 = stencil step fused with receivers+imaging),
- * then clears the record. */
+ * the number of launches and their summed duration for one kernel class,
+ * then clears that class's record. */
 int rtmhip_timing_enable(int on);
 int rtmhip_timing_read(int kernel_class, long long *launches, double *total_ms);
 

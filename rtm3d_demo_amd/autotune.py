@@ -1,0 +1,55 @@
+"""Tile-shape autotuner for the stencil kernels.
+
+For a given grid, every compiled tile shape is timed for the three stencil
+kernel classes (0 plain step, 1 fused receiver step, 2 tapered HBC forward
+step) with HIP events around each launch, by running a few steps of the
+forward and fused backward loops on the caller's buffers; the fastest shape
+per class is installed with rtmhip_set_tiling.  Results are identical for
+every shape (tested), so tuning only changes speed.
+"""
+import numpy as np
+
+
+def tune(hip, shot, V, coefs, g, src, F, R, upb, rcv, img, steps=6, rounds=2, log=None):
+    """Returns {class: ((tz, by, ry), avg_ms)} and installs the winners.
+
+    The buffers are scratch for the duration of the call (their contents are
+    overwritten); `shot` must allow `steps` time steps."""
+    best = {}
+    nt = shot.nt
+    for tiling in hip.TILINGS:
+        for cls in range(3):
+            hip.set_tiling(cls, *tiling, 0)
+        # one untimed pass to settle clocks and caches
+        hip.forward_steps(shot, True, V, coefs, g, src, F, upb, 0, steps)
+        hip.synchronize()
+        hip.timing_enable(True)
+        for _ in range(rounds):
+            hip.forward_steps(shot, True, V, coefs, g, src, F, upb, 0, steps)
+            hip.backward_steps(shot, True, V, coefs, g, F, R, upb, rcv, img, nt - 1, steps, True)
+        res = [hip.timing_read(c) for c in range(3)]
+        hip.timing_enable(False)
+        for cls, (n, ms) in enumerate(res):
+            if n == 0:
+                continue
+            avg = ms / n
+            if log:
+                log("tile %s class %d: %.4f ms" % (tiling, cls, avg))
+            if cls not in best or avg < best[cls][1]:
+                best[cls] = (tiling, avg)
+    for cls in range(3):
+        if cls in best:
+            hip.set_tiling(cls, *best[cls][0], 0)
+        else:
+            hip.set_tiling(cls, 0)
+    for f in (F.a, F.b, R.a, R.b, img):
+        f.zero()
+    return best
+
+
+def describe(best):
+    return {str(c): {"tile": "%dx%dx%d" % t, "avg_ms": round(ms, 4)} for c, (t, ms) in
+            sorted(best.items())}
+
+
+__all__ = ["tune", "describe", "np"]

@@ -26,7 +26,7 @@ DRIVER = os.path.join(BINDIR, "rtm3d")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 HIP_SOURCES = ["rtm_stencil.hip", "rtm_ops.hip", "rtm_capi.hip"]
 HIP_FLAGS = [
-    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
+    "--offload-arch=gfx950", "-O3", "-std=c++20", "-fPIC", "-ffp-contract=off",
     "-fno-gpu-flush-denormals-to-zero", "-Xclang", "-target-feature", "-Xclang",
     "-packed-fp32-ops", "-Wall", "-Wno-unused-function", "-I" + os.path.join(REPO, "include"),
 ]

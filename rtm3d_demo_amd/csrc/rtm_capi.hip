@@ -29,7 +29,8 @@ struct State {
     hipStream_t stream = nullptr;
     int *flag = nullptr;  // NaN ("numerical dispersion") flag written by the stencil
     bool timing = false;
-    Timing tm[2];
+    Timing tm[3];
+    StencilTiling tiling[3] = {};  // per kernel class; tz == 0: built-in default
 };
 
 State g_state;
@@ -105,9 +106,11 @@ int check_rcv(const rtmhip_rcv_geom *g, int nze, int blen)
 }
 
 hipError_t stencil(int radius, const float *P, float *PP, const float *V, const Coefs &cf,
-                   const Grid &g, const ImageFuse *fuse)
+                   const Grid &g, const ImageFuse *fuse, const TaperFuse *taper = nullptr)
 {
-    Timing *tm = g_state.timing ? &g_state.tm[fuse ? 1 : 0] : nullptr;
+    // kernel class: 0 plain step, 1 fused receiver step, 2 tapered HBC step
+    const int cls = fuse ? 1 : (taper ? 2 : 0);
+    Timing *tm = g_state.timing ? &g_state.tm[cls] : nullptr;
     if (tm) {
         if (tm->used == tm->start.size()) {
             hipEvent_t a, b;
@@ -121,8 +124,9 @@ hipError_t stencil(int radius, const float *P, float *PP, const float *V, const 
         hipError_t e = hipEventRecord(tm->start[tm->used], g_state.stream);
         if (e != hipSuccess) return e;
     }
-    hipError_t e = launch_stencil(radius, P, PP, V, cf, g, fuse, g_state.flag, g_state.stream,
-                                  nullptr);
+    const StencilTiling *tl = g_state.tiling[cls].tz ? &g_state.tiling[cls] : nullptr;
+    hipError_t e = launch_stencil(radius, P, PP, V, cf, g, fuse, taper, g_state.flag,
+                                  g_state.stream, tl);
     if (e != hipSuccess) return e;
     if (tm) {
         e = hipEventRecord(tm->stop[tm->used], g_state.stream);
@@ -460,18 +464,27 @@ int rtmhip_forward_steps(const rtmhip_shot *shot, int hbc, const float *V2DT2,
     if (it0 < 0 || it0 + nsteps > shot->nt)
         return fail(RTMHIP_ERR_INVALID, "time range outside [0,%d)", shot->nt);
     hipStream_t s = g_state.stream;
+    // HBC: the two in-place upper tapers of each step are fused into the
+    // stencil (P tapered on load, the centre PP tapered twice -- once on the
+    // first step of a call, whose PP already carries one taper).  The taper
+    // still pending on P is applied in place after the last step, so the
+    // state between calls is the reference's.
     for (int it = it0; it < it0 + nsteps; ++it) {
         float *t = *P;
         *P = *PP;
         *PP = t;
         if (hbc) {
-            HIPCHK(launch_taper(*P, taper, c.g, shot->blen, true, s), "taper");
-            HIPCHK(launch_taper(*PP, taper, c.g, shot->blen, true, s), "taper");
+            const TaperFuse tf{taper, shot->blen, it == it0 ? 1 : 2};
+            HIPCHK(stencil(c.radius, *P, *PP, V2DT2, c.cf, c.g, nullptr, &tf), "stencil");
+            HIPCHK(launch_taper_frame(*PP, taper, c.g, shot->blen, c.radius, tf.pp_taps, s),
+                   "frame taper");
+        } else {
+            HIPCHK(stencil(c.radius, *P, *PP, V2DT2, c.cf, c.g, nullptr), "stencil");
         }
-        HIPCHK(stencil(c.radius, *P, *PP, V2DT2, c.cf, c.g, nullptr), "stencil");
         HIPCHK(launch_source(*PP, c.src_off, ricker, it, 0.f, s), "source");
         if (hbc) HIPCHK(launch_upb(*PP, upb, c.g, shot->blen, c.hf, it, false, s), "upb save");
     }
+    if (hbc) HIPCHK(launch_taper(*P, taper, c.g, shot->blen, true, s), "taper");
     return RTMHIP_OK;
 }
 
@@ -509,12 +522,14 @@ int rtmhip_backward_steps(const rtmhip_shot *shot, int hbc, const float *V2DT2,
             S = *Ps;
         }
         // Receiver wavefield and imaging (:311-345).
-        if (hbc) {
+        if (hbc && !fused) {
             HIPCHK(launch_taper(*Pr, taper, c.g, b, true, s), "taper");
             HIPCHK(launch_taper(*PPr, taper, c.g, b, true, s), "taper");
         }
         const int lt = it % shot->ntstep;
         if (fused) {
+            // tapers fused as in the forward loop; finishing taper below
+            const TaperFuse tf{taper, b, it == it_hi ? 1 : 2};
             ImageFuse fz;
             fz.img = img;
             fz.ps = S;
@@ -527,7 +542,11 @@ int rtmhip_backward_steps(const rtmhip_shot *shot, int hbc, const float *V2DT2,
             fz.ry0 = c.ry0;
             fz.rdy = shot->rcv.dist_y;
             fz.rny = shot->rcv.count_y;
-            HIPCHK(stencil(c.radius, *Pr, *PPr, V2DT2, c.cf, c.g, &fz), "fused stencil");
+            HIPCHK(stencil(c.radius, *Pr, *PPr, V2DT2, c.cf, c.g, &fz, hbc ? &tf : nullptr),
+                   "fused stencil");
+            if (hbc)
+                HIPCHK(launch_taper_frame(*PPr, taper, c.g, b, c.radius, tf.pp_taps, s),
+                       "frame taper");
             HIPCHK(launch_receivers_image(*PPr, rcv_chunk, c.g, c.rx0, c.ry0, c.rz,
                                           shot->rcv.dist_x, shot->rcv.dist_y, shot->rcv.count_x,
                                           shot->rcv.count_y, shot->ntstep, lt, img, S, z0, z1,
@@ -545,6 +564,36 @@ int rtmhip_backward_steps(const rtmhip_shot *shot, int hbc, const float *V2DT2,
         *Pr = *PPr;
         *PPr = t;
     }
+    // pending in-place taper of the older receiver field (fused HBC path)
+    if (hbc && fused) HIPCHK(launch_taper(*PPr, taper, c.g, b, true, s), "taper");
+    return RTMHIP_OK;
+}
+
+int rtmhip_set_tiling(int kernel_class, int tz, int by, int ry, int xchunk)
+{
+    NEED_INIT();
+    if (kernel_class < 0 || kernel_class > 2)
+        return fail(RTMHIP_ERR_INVALID, "kernel class %d not in 0..2", kernel_class);
+    if (tz == 0) {
+        g_state.tiling[kernel_class] = StencilTiling{};
+        return RTMHIP_OK;
+    }
+    if (!stencil_tiling_supported(tz, by, ry) || xchunk < 0)
+        return fail(RTMHIP_ERR_UNSUPPORTED, "tile shape %dx%dx%d not compiled", tz, by, ry);
+    g_state.tiling[kernel_class] = StencilTiling{tz, by, ry, xchunk};
+    return RTMHIP_OK;
+}
+
+int rtmhip_get_tiling(int kernel_class, int *tz, int *by, int *ry, int *xchunk)
+{
+    NEED_INIT();
+    if (kernel_class < 0 || kernel_class > 2 || !tz || !by || !ry || !xchunk)
+        return fail(RTMHIP_ERR_INVALID, "bad arguments");
+    const StencilTiling &t = g_state.tiling[kernel_class];
+    *tz = t.tz;
+    *by = t.by;
+    *ry = t.ry;
+    *xchunk = t.xchunk;
     return RTMHIP_OK;
 }
 
@@ -558,7 +607,7 @@ int rtmhip_timing_enable(int on)
 int rtmhip_timing_read(int kernel_class, long long *launches, double *total_ms)
 {
     NEED_INIT();
-    if (kernel_class < 0 || kernel_class > 1 || !launches || !total_ms)
+    if (kernel_class < 0 || kernel_class > 2 || !launches || !total_ms)
         return fail(RTMHIP_ERR_INVALID, "bad arguments");
     HIPCHK(hipStreamSynchronize(g_state.stream), "hipStreamSynchronize");
     Timing &t = g_state.tm[kernel_class];

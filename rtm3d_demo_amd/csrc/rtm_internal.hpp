@@ -37,18 +37,34 @@ struct ImageFuse {
     int ry0, rdy, rny;
 };
 
+// Upper-border taper fused into the stencil (HBC steps): P values with
+// z < b are tapered once on load, the centre PP value pp_taps times.
+struct TaperFuse {
+    const float *t;  // device table g[b]
+    int b;
+    int pp_taps;     // 1 or 2
+};
+
 // Tile configuration of the stencil kernel.
 struct StencilTiling {
     int tz, by, ry;  // lanes along z, thread rows, rows per thread
     int xchunk;      // planes marched per block (0 = automatic)
 };
 
+// Tile shapes compiled into the stencil kernel (tz lanes x by rows, ry rows
+// per thread); the default is 64x4x4.
+bool stencil_tiling_supported(int tz, int by, int ry);
+
 // Launchers: enqueue on `s`, return the launch status.
 hipError_t launch_stencil(int radius, const float *P, float *PP, const float *V, const Coefs &cf,
-                          const Grid &g, const ImageFuse *fuse, int *flag, hipStream_t s,
-                          const StencilTiling *tiling);
+                          const Grid &g, const ImageFuse *fuse, const TaperFuse *taper,
+                          int *flag, hipStream_t s, const StencilTiling *tiling);
 hipError_t launch_taper(float *P, const float *taper, const Grid &g, int blen, bool upper_only,
                         hipStream_t s);
+// Upper taper (applied `taps` times) of the slab cells the stencil never
+// updates (z < R or the x / y frame of width R), in place.
+hipError_t launch_taper_frame(float *P, const float *taper, const Grid &g, int b, int R, int taps,
+                              hipStream_t s);
 hipError_t launch_source(float *PP, long long offset, const float *table, int it, float value,
                          hipStream_t s);
 hipError_t launch_receivers(float *PP, float *rcv, const Grid &g, int rx0, int ry0, int rz,

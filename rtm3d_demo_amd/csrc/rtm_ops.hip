@@ -56,6 +56,56 @@ __global__ void __launch_bounds__(kThreads)
     P[o] = v;
 }
 
+// In-place upper taper, applied `taps` times, of the slab cells (z < b) that
+// the stencil does not update: z < R, or x / y within R of the grid edge.
+// The fused HBC step tapers every other slab value as it is loaded; these
+// cells of PP are not read during the step, so tapering them in place here
+// keeps PP's memory equal to the reference's after its in-place tapers.
+__global__ void __launch_bounds__(kThreads)
+    k_taper_frame(float *__restrict__ P, const float *__restrict__ t, Grid g, int b, int R,
+                  int taps)
+{
+    const long long nA = (long long)g.nx * g.ny * R;  // z < R, all x, y
+    const int lz = b - R;                             // z in [R, b)
+    const long long nB = (long long)2 * R * g.ny * lz;              // x frame
+    const long long nC = (long long)(g.nx - 2 * R) * 2 * R * lz;    // y frame
+    long long i = (long long)blockIdx.x * kThreads + threadIdx.x;
+    int x, y, z;
+    if (i < nA) {
+        z = (int)(i % R);
+        const long long r = i / R;
+        y = (int)(r % g.ny);
+        x = (int)(r / g.ny);
+    } else if ((i -= nA) < nB) {
+        z = R + (int)(i % lz);
+        const long long r = i / lz;
+        y = (int)(r % g.ny);
+        const int xx = (int)(r / g.ny);
+        x = xx < R ? xx : g.nx - 2 * R + xx;
+    } else if ((i -= nB) < nC) {
+        z = R + (int)(i % lz);
+        const long long r = i / lz;
+        const int yy = (int)(r % (2 * R));
+        x = R + (int)(r / (2 * R));
+        y = yy < R ? yy : g.ny - 2 * R + yy;
+    } else {
+        return;
+    }
+    const bool bx = x < b || x >= g.nx - b;
+    const bool by = y < b || y >= g.ny - b;
+    const float fy = by ? gfac(t, y, g.ny, b) : 1.0f;
+    const float fx = bx ? gfac(t, x, g.nx, b) : 1.0f;
+    const float fz = t[b - 1 - z];
+    const long long o = goff(g, x, y, z);
+    float v = P[o];
+    for (int k = 0; k < taps; ++k) {
+        v *= fy;
+        v *= fx;
+        v *= fz;
+    }
+    P[o] = v;
+}
+
 // The front-right edge of taper3D (RTMCPUPlatform.cpp:900-904) writes cells
 // x < blen, y >= ny-blen, z interior from the face-tapered cell
 // (blen, y, z).  This kernel runs before k_taper_all so it reads that cell
@@ -335,6 +385,18 @@ hipError_t launch_taper(float *P, const float *taper, const Grid &g, int b, bool
     const long long n = (long long)g.nx * g.ny * 2 * b + (long long)g.nx * 2 * b * lz +
                         (long long)2 * b * ly * lz;
     hipLaunchKernelGGL(k_taper_all, dim3(nblocks(n)), dim3(kThreads), 0, s, P, taper, g, b);
+    return hipGetLastError();
+}
+
+hipError_t launch_taper_frame(float *P, const float *taper, const Grid &g, int b, int R, int taps,
+                              hipStream_t s)
+{
+    if (b <= 0 || taps <= 0) return hipSuccess;
+    const long long lz = b - R;
+    const long long n = (long long)g.nx * g.ny * R + (long long)2 * R * g.ny * lz +
+                        (long long)(g.nx - 2 * R) * 2 * R * lz;
+    hipLaunchKernelGGL(k_taper_frame, dim3(nblocks(n)), dim3(kThreads), 0, s, P, taper, g, b, R,
+                       taps);
     return hipGetLastError();
 }
 

@@ -20,7 +20,7 @@ EXPORTED = [
     "rtmhip_step", "rtmhip_taper", "rtmhip_apply_source", "rtmhip_receivers",
     "rtmhip_upper_border", "rtmhip_image_condition", "rtmhip_image_finish", "rtmhip_stack",
     "rtmhip_model_steps", "rtmhip_forward_steps", "rtmhip_backward_steps",
-    "rtmhip_timing_enable", "rtmhip_timing_read",
+    "rtmhip_set_tiling", "rtmhip_get_tiling", "rtmhip_timing_enable", "rtmhip_timing_read",
 ]
 
 
@@ -73,6 +73,8 @@ def load():
         "rtmhip_forward_steps": [C.POINTER(Shot), i, fp, fp, fp, fp, pp, pp, fp, i, i],
         "rtmhip_backward_steps": [C.POINTER(Shot), i, fp, fp, fp, pp, pp, pp, pp, fp, fp, fp,
                                   i, i, i],
+        "rtmhip_set_tiling": [i, i, i, i, i],
+        "rtmhip_get_tiling": [i] + [C.POINTER(C.c_int)] * 4,
         "rtmhip_timing_enable": [i],
         "rtmhip_timing_read": [i, C.POINTER(C.c_longlong), C.POINTER(C.c_double)],
     }
@@ -248,6 +250,19 @@ def backward_steps(shot, hbc, V, coefs, g, src_fields, rcv_fields, upb, rcv_chun
                                        C.byref(rcv_fields.p), C.byref(rcv_fields.pp),
                                        upb.ptr if upb is not None else None, rcv_chunk.ptr,
                                        img.ptr, it_hi, nsteps, int(fused)))
+
+
+def set_tiling(kernel_class, tz=0, by=0, ry=0, xchunk=0):
+    check(load().rtmhip_set_tiling(kernel_class, tz, by, ry, xchunk))
+
+
+def get_tiling(kernel_class):
+    v = [C.c_int() for _ in range(4)]
+    check(load().rtmhip_get_tiling(kernel_class, *[C.byref(x) for x in v]))
+    return tuple(x.value for x in v)
+
+
+TILINGS = [(64, 4, 4), (64, 4, 2), (64, 8, 2), (32, 8, 4), (32, 8, 2)]
 
 
 def timing_enable(on=True):

@@ -167,3 +167,114 @@ def test_hbc_reconstruction_is_time_reversible(gpu):
     assert np.count_nonzero(img.download()) == 0  # zero receiver field images nothing
     for d in (V, g, src, F, upb, R, rcv, img):
         d.free()
+
+
+def _pair(gpu, A, B):
+    F = gpu.FieldPair(A.shape)
+    F.a.upload(A)
+    F.b.upload(B)
+    return F
+
+
+@pytest.fixture(params=[0, 1, 5])
+def xchunk(request, gpu):
+    """Planes marched per block: automatic, one, five (multi-plane marching
+    through the register ring is exercised even on small grids)."""
+    for cls in range(3):
+        gpu.set_tiling(cls, *(gpu.TILINGS[0] if request.param else (0, 0, 0)), request.param)
+    yield request.param
+    for cls in range(3):
+        gpu.set_tiling(cls, 0)
+
+
+@pytest.mark.parametrize("splits", [(5,), (2, 1, 2)])
+def test_fused_taper_forward_matches_operator_sequence(gpu, oracle, splits, xchunk):
+    """HBC forward steps with the upper taper fused into the stencil equal
+    the reference's operator sequence (taper, taper, step, source, save
+    UPB) on non-zero fields, whether run in one call or split into calls."""
+    shape, b, order, nt = (40, 44, 90), 12, 8, 5
+    rng = np.random.default_rng(21)
+    P0 = rng.uniform(-1, 1, shape).astype(np.float32)
+    PP0 = rng.uniform(-1, 1, shape).astype(np.float32)
+    v2 = rng.uniform(0.0, 0.15, shape).astype(np.float32)
+    g = oracle.taper_table(b, np.float32(0.3))
+    coefs = oracle.stencil_coefs(order, 10.0, 11.0, 9.0)
+    src = rng.uniform(-1, 1, nt).astype(np.float32)
+    # oracle: reference operator order
+    P, PP = P0.copy(), PP0.copy()
+    upb = np.zeros((nt, shape[0], shape[1], 4), np.float32)
+    for it in range(nt):
+        P, PP = PP, P
+        oracle.taper(P, g, b, True)
+        oracle.taper(PP, g, b, True)
+        oracle.step(P, PP, v2, coefs, order)
+        oracle.apply_source(PP, 20, 22, b + 2, src[it])
+        oracle.upb(PP, upb, b, order, it, False)
+    F = _pair(gpu, P0, PP0)
+    V, dg, ds = (gpu.DeviceArray.from_host(a) for a in (v2, g, src))
+    dupb = gpu.DeviceArray(upb.shape).zero()
+    shot = gpu.make_shot(*shape, b, order, nt, nt, (20, 22, b + 2), gpu.RcvGeom(0, 0, 0, 1, 1, 1, 1))
+    it0 = 0
+    for n in splits:
+        gpu.forward_steps(shot, True, V, coefs, dg, ds, F, dupb, it0, n)
+        it0 += n
+    assert _bits_equal(F.P.download(), P)
+    assert _bits_equal(F.PP.download(), PP)
+    assert _bits_equal(dupb.download(), upb)
+    for d in (F, V, dg, ds, dupb):
+        d.free()
+
+
+@pytest.mark.parametrize("splits", [(3,), (1, 2)])
+def test_fused_backward_matches_operator_sequence(gpu, oracle, splits, xchunk):
+    """Fused receiver step (tapers, stencil, receiver injection, imaging)
+    against the reference's operator sequence on non-zero fields."""
+    shape, b, order, nt = (40, 44, 90), 12, 8, 8
+    rng = np.random.default_rng(22)
+    Ps, PPs, Pr, PPr, img = (rng.uniform(-1, 1, shape).astype(np.float32) for _ in range(5))
+    v2 = rng.uniform(0.0, 0.15, shape).astype(np.float32)
+    g = oracle.taper_table(b, np.float32(0.3))
+    coefs = oracle.stencil_coefs(order, 10.0, 11.0, 9.0)
+    upb = rng.uniform(-1, 1, (nt, shape[0], shape[1], 4)).astype(np.float32)
+    rcv = rng.uniform(-1, 1, (7, 6, nt)).astype(np.float32)
+    og = oracle.RcvGeom(1, 2, 3, 3, 4, 7, 6)
+    geom = gpu.RcvGeom(1, 2, 3, 3, 4, 7, 6)
+    d = {k: gpu.DeviceArray.from_host(a) for k, a in
+         dict(V=v2, g=g, upb=upb, rcv=rcv, img=img).items()}
+    S, R = _pair(gpu, Ps, PPs), _pair(gpu, Pr, PPr)
+    shot = gpu.make_shot(*shape, b, order, nt, nt, (20, 22, b + 2), geom)
+    hi = nt - 3
+    for n in splits:
+        gpu.backward_steps(shot, True, d["V"], coefs, d["g"], S, R, d["upb"], d["rcv"],
+                           d["img"], hi, n, True)
+        hi -= n
+    for it in range(nt - 3, nt - 3 - sum(splits), -1):
+        oracle.step(Ps, PPs, v2, coefs, order)
+        oracle.upb(PPs, upb, b, order, it, True)
+        Ps, PPs = PPs, Ps
+        oracle.taper(Pr, g, b, True)
+        oracle.taper(PPr, g, b, True)
+        oracle.step(Pr, PPr, v2, coefs, order)
+        oracle.receivers(PPr, rcv, b, og, nt, it, True)
+        oracle.image(img, Ps, PPr, b)
+        Pr, PPr = PPr, Pr
+    assert _bits_equal(S.P.download(), Ps) and _bits_equal(S.PP.download(), PPs)
+    assert _bits_equal(R.P.download(), Pr) and _bits_equal(R.PP.download(), PPr)
+    assert _bits_equal(d["img"].download(), img)
+    for x in list(d.values()) + [S, R]:
+        x.free()
+
+
+@pytest.mark.parametrize("tiling", [(64, 4, 2), (64, 8, 2), (32, 8, 4), (32, 8, 2)])
+def test_tile_shapes_bit_exact(gpu, oracle, cases, golden, tiling):
+    """Every compiled tile shape, for all three stencil classes, reproduces
+    the reference HBC image bit for bit."""
+    for cls in range(3):
+        gpu.set_tiling(cls, *tiling, 0)
+    try:
+        p = cases["small_hbc"]
+        img = gpu_migration(gpu, p, golden("small_hbc")["vpe"], golden("small_mod")["records"])
+        assert _bits_equal(img, golden("small_hbc")["image"])
+    finally:
+        for cls in range(3):
+            gpu.set_tiling(cls, 0)
