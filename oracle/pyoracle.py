@@ -18,6 +18,11 @@ LIBPATH = os.path.join(HERE, "librtmoracle.so")
 
 _f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
 _f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+_u16p = np.ctypeslib.ndpointer(np.uint16, flags="C_CONTIGUOUS")
+
+# Wavefield storage formats: suffix -> (arithmetic pointer, wavefield pointer,
+# scalar type, wavefield dtype).  f16/bf16 wavefields are raw uint16 bits.
+FORMATS = ("f32", "f64", "f16", "bf16")
 
 
 class RcvGeom(C.Structure):
@@ -58,15 +63,20 @@ def lib():
         L.orc_ricker_f64.argtypes = [i, d, d, _f64p]
         L.orc_taper_table_f64.argtypes = [i, d, _f64p]
         L.orc_v2dt2_f64.argtypes = [_f32p, C.c_size_t, d, _f64p]
-        for s, p, r in (("f32", _f32p, f), ("f64", _f64p, d)):
-            getattr(L, "orc_step_" + s).argtypes = [p, p, p, p, i, i, i, i]
-            getattr(L, "orc_taper_all_" + s).argtypes = [p, p, i, i, i, i]
-            getattr(L, "orc_taper_upper_" + s).argtypes = [p, p, i, i, i, i]
-            getattr(L, "orc_apply_source_" + s).argtypes = [p, i, i, i, i, i, i, r]
-            getattr(L, "orc_receivers_" + s).argtypes = [p, p, i, i, i, i, C.POINTER(RcvGeom),
+        for fn in ("orc_to_f16", "orc_to_bf16"):
+            getattr(L, fn).argtypes = [_f32p, _u16p, C.c_size_t]
+        for fn in ("orc_from_f16", "orc_from_bf16"):
+            getattr(L, fn).argtypes = [_u16p, _f32p, C.c_size_t]
+        for s, p, w, r in (("f32", _f32p, _f32p, f), ("f64", _f64p, _f64p, d),
+                           ("f16", _f32p, _u16p, f), ("bf16", _f32p, _u16p, f)):
+            getattr(L, "orc_step_" + s).argtypes = [w, w, p, p, i, i, i, i]
+            getattr(L, "orc_taper_all_" + s).argtypes = [w, p, i, i, i, i]
+            getattr(L, "orc_taper_upper_" + s).argtypes = [w, p, i, i, i, i]
+            getattr(L, "orc_apply_source_" + s).argtypes = [w, i, i, i, i, i, i, r]
+            getattr(L, "orc_receivers_" + s).argtypes = [w, p, i, i, i, i, C.POINTER(RcvGeom),
                                                          i, i, i]
-            getattr(L, "orc_upb_" + s).argtypes = [p, p, i, i, i, i, i, i, i]
-            getattr(L, "orc_image_" + s).argtypes = [p, p, p, i, i, i, i]
+            getattr(L, "orc_upb_" + s).argtypes = [w, w, i, i, i, i, i, i, i]
+            getattr(L, "orc_image_" + s).argtypes = [p, w, w, i, i, i, i]
             getattr(L, "orc_image_finish_" + s).argtypes = [p, p, i, i, i, i, p]
             getattr(L, "orc_model_shot_" + s).argtypes = [p, p, p, p, i, i, i, i, i, i, i, i, i,
                                                           i, C.POINTER(RcvGeom), p]
@@ -77,8 +87,35 @@ def lib():
     return _lib
 
 
-def _sfx(dtype):
-    return "f32" if np.dtype(dtype) == np.float32 else "f64"
+def _sfx(dtype, fmt=None):
+    if fmt is not None:
+        assert fmt in FORMATS, fmt
+        return fmt
+    dtype = np.dtype(dtype)
+    if dtype == np.uint16:
+        raise ValueError("16-bit wavefields need fmt='f16' or 'bf16'")
+    return "f32" if dtype == np.float32 else "f64"
+
+
+def to_format(a, fmt):
+    """float32 array -> wavefield storage of format fmt (uint16 bits for
+    f16/bf16; nearest-even)."""
+    a = np.ascontiguousarray(a, np.float32)
+    if fmt == "f32":
+        return a.copy()
+    out = np.empty(a.shape, np.uint16)
+    getattr(lib(), "orc_to_" + fmt)(a.reshape(-1), out.reshape(-1), a.size)
+    return out
+
+
+def from_format(a, fmt):
+    """Wavefield storage of format fmt -> float32 (exact)."""
+    if fmt == "f32":
+        return np.array(a, np.float32)
+    a = np.ascontiguousarray(a, np.uint16)
+    out = np.empty(a.shape, np.float32)
+    getattr(lib(), "orc_from_" + fmt)(a.reshape(-1), out.reshape(-1), a.size)
+    return out
 
 
 # ---------------- set-up helpers ----------------
@@ -145,32 +182,35 @@ def v2dt2(ve, dt, dtype=np.float32):
 
 
 # ---------------- per-step operators ----------------
-def step(P, PP, V2, coefs, order):
+def step(P, PP, V2, coefs, order, fmt=None):
     """In place on PP; returns 0 or -1 on NaN."""
     nxe, nye, nze = P.shape
-    return getattr(lib(), "orc_step_" + _sfx(P.dtype))(P, PP, V2, coefs, nxe, nye, nze, order)
+    return getattr(lib(), "orc_step_" + _sfx(P.dtype, fmt))(P, PP, V2, coefs, nxe, nye, nze,
+                                                           order)
 
 
-def taper(P, g, blen, upper_only):
+def taper(P, g, blen, upper_only, fmt=None):
     fn = "orc_taper_upper_" if upper_only else "orc_taper_all_"
-    getattr(lib(), fn + _sfx(P.dtype))(P, g, *P.shape, blen)
+    getattr(lib(), fn + _sfx(P.dtype, fmt))(P, g, *P.shape, blen)
 
 
-def apply_source(PP, sxe, sye, sze, val):
-    getattr(lib(), "orc_apply_source_" + _sfx(PP.dtype))(PP, *PP.shape, sxe, sye, sze, val)
+def apply_source(PP, sxe, sye, sze, val, fmt=None):
+    getattr(lib(), "orc_apply_source_" + _sfx(PP.dtype, fmt))(PP, *PP.shape, sxe, sye, sze, val)
 
 
-def receivers(PP, rcv, blen, geom, ntstep, lt, restore):
-    getattr(lib(), "orc_receivers_" + _sfx(PP.dtype))(PP, rcv, *PP.shape, blen, C.byref(geom),
-                                                      ntstep, lt, int(restore))
+def receivers(PP, rcv, blen, geom, ntstep, lt, restore, fmt=None):
+    getattr(lib(), "orc_receivers_" + _sfx(PP.dtype, fmt))(PP, rcv, *PP.shape, blen,
+                                                           C.byref(geom), ntstep, lt,
+                                                           int(restore))
 
 
-def upb(PP, buf, blen, order, lt, restore):
-    getattr(lib(), "orc_upb_" + _sfx(PP.dtype))(PP, buf, *PP.shape, blen, order, lt, int(restore))
+def upb(PP, buf, blen, order, lt, restore, fmt=None):
+    getattr(lib(), "orc_upb_" + _sfx(PP.dtype, fmt))(PP, buf, *PP.shape, blen, order, lt,
+                                                     int(restore))
 
 
-def image(img, PS, PR, blen):
-    getattr(lib(), "orc_image_" + _sfx(img.dtype))(img, PS, PR, *img.shape, blen)
+def image(img, PS, PR, blen, fmt=None):
+    getattr(lib(), "orc_image_" + _sfx(img.dtype, fmt))(img, PS, PR, *img.shape, blen)
 
 
 def image_finish(img, blen, dx, dy, dz):
@@ -194,24 +234,27 @@ def shots_from_params(p):
             for ix in range(p["source_count_x"]) for iy in range(p["source_count_y"])]
 
 
-def model_shot(v2, coefs, g, src, blen, order, nt, ntstep, shot, geom):
+def model_shot(v2, coefs, g, src, blen, order, nt, ntstep, shot, geom, fmt=None):
     dt = v2.dtype
     nch = nt // ntstep
     rcv = np.zeros((nch, geom.count_x, geom.count_y, ntstep), dt)
-    rc = getattr(lib(), "orc_model_shot_" + _sfx(dt))(v2, coefs, g, src, *v2.shape, blen, order,
-                                                      nt, ntstep, *shot, C.byref(geom), rcv)
+    rc = getattr(lib(), "orc_model_shot_" + _sfx(dt, fmt))(v2, coefs, g, src, *v2.shape, blen,
+                                                           order, nt, ntstep, *shot,
+                                                           C.byref(geom), rcv)
     if rc != 0:
         raise RuntimeError("oracle modeling failed (%d)" % rc)
     return rcv
 
 
-def migrate_shot(v2, coefs, g, src, blen, order, nt, ntstep, shot, geom, rcv, hbc=True):
+def migrate_shot(v2, coefs, g, src, blen, order, nt, ntstep, shot, geom, rcv, hbc=True,
+                 fmt=None):
     dt = v2.dtype
     img = np.zeros(v2.shape, dt)
     rcv = np.ascontiguousarray(rcv, dt)
-    rc = getattr(lib(), "orc_migrate_shot_" + _sfx(dt))(v2, coefs, g, src, *v2.shape, blen,
-                                                        order, nt, ntstep, *shot, C.byref(geom),
-                                                        rcv, int(hbc), img, None)
+    rc = getattr(lib(), "orc_migrate_shot_" + _sfx(dt, fmt))(v2, coefs, g, src, *v2.shape, blen,
+                                                             order, nt, ntstep, *shot,
+                                                             C.byref(geom), rcv, int(hbc), img,
+                                                             None)
     if rc != 0:
         raise RuntimeError("oracle migration failed (%d)" % rc)
     return img
@@ -235,16 +278,17 @@ def _setup(p, ve, dtype):
     return coefs, v2, src, g
 
 
-def pipeline_modeling(p, vel, dtype=np.float32):
-    """Records per shot, [nshot][nchunk][nrx][nry][ntstep] (modeling uses ABC)."""
+def pipeline_modeling(p, vel, dtype=np.float32, fmt=None):
+    """Records per shot, [nshot][nchunk][nrx][nry][ntstep] (modeling uses ABC).
+    fmt = 'f16' / 'bf16' runs the float pipeline with 16-bit wavefields."""
     ve = extend_velocity(vel, p["blen"], "abc")
     coefs, v2, src, g = _setup(p, ve, dtype)
     geom = geom_from_params(p)
     return np.stack([model_shot(v2, coefs, g, src, p["blen"], p["stencil_order"], p["nt"],
-                                p["ntstep"], s, geom) for s in shots_from_params(p)])
+                                p["ntstep"], s, geom, fmt) for s in shots_from_params(p)])
 
 
-def pipeline_migration(p, vpe, records, dtype=np.float32):
+def pipeline_migration(p, vpe, records, dtype=np.float32, fmt=None):
     """Final [nx][ny][nz] image from an already-extended velocity model."""
     coefs, v2, src, g = _setup(p, vpe, dtype)
     geom = geom_from_params(p)
@@ -253,7 +297,7 @@ def pipeline_migration(p, vpe, records, dtype=np.float32):
     total = np.zeros(vpe.shape, dtype)
     for k, s in enumerate(shots_from_params(p)):
         shot_img = migrate_shot(v2, coefs, g, src, p["blen"], p["stencil_order"], p["nt"],
-                                p["ntstep"], s, geom, records[k], hbc)
+                                p["ntstep"], s, geom, records[k], hbc, fmt)
         total = total + shot_img
     return image_finish(total, p["blen"], float(np.float32(p["dx"])), float(np.float32(p["dy"])),
                         float(np.float32(p["dz"])))

@@ -12,6 +12,9 @@
  *     so that results are bit-identical to the reference program.
  *   - orc_*_f64: the same algorithm carried out in double precision from the
  *     same FP32 model, used as the accuracy yardstick for every dtype variant.
+ *   - orc_*_f16, orc_*_bf16: the FP32 algorithm with 16-bit wavefield
+ *     storage (each wavefield write rounded to the format), the bit-exact
+ *     yardstick of the 16-bit GPU path.
  *
  * Every function cites the reference file:line it restates (paths relative
  * to the reference tree; rtm-3d/ = source/rtm-3d/).
@@ -31,6 +34,12 @@
  *   orc_receivers/upb/image save/restore round trips; reference outputs.
  *   orc_*_shot / migrate    reference seismogram and image fixtures
  *                           (bit-exact), HBC reversibility.
+ *   orc_f32_to_f16 etc.     numpy float16 casts (all 65536 half values one
+ *                           way, 10^7 random and edge floats the other);
+ *                           torch bfloat16 casts.
+ *   orc_*_f16 / _bf16       equal to the f32 build when every wavefield value
+ *                           is exactly representable (integers, dyadic
+ *                           taps); error vs f64 within the format's bound.
  */
 #ifndef RTM_ORACLE_H
 #define RTM_ORACLE_H
@@ -87,19 +96,34 @@ void orc_ricker_f64(int nt, double dt, double fpeak, double *out);
 void orc_taper_table_f64(int blen, double factor, double *out);
 void orc_v2dt2_f64(const float *v, size_t n, double dt, double *out);
 
-/* ---------------- per-step operators (REAL = float / double) ----------- */
-#define ORC_DECLARE_OPS(R, S)                                                          \
-int  orc_step_##S(const R *P, R *PP, const R *V2, const R *coefs,                      \
+/* ---------------- 16-bit storage formats -------------------------------
+ * binary16 and bfloat16 <-> binary32, nearest-even, subnormals kept. */
+uint16_t orc_f32_to_f16(float f);
+float    orc_f16_to_f32(uint16_t h);
+uint16_t orc_f32_to_bf16(float f);
+float    orc_bf16_to_f32(uint16_t b);
+void orc_to_f16(const float *in, uint16_t *out, size_t n);
+void orc_from_f16(const uint16_t *in, float *out, size_t n);
+void orc_to_bf16(const float *in, uint16_t *out, size_t n);
+void orc_from_bf16(const uint16_t *in, float *out, size_t n);
+
+/* ---------------- per-step operators -----------------------------------
+ * R = arithmetic type, W = wavefield storage type:
+ *   f32 (float, float), f64 (double, double): the reference algorithm;
+ *   f16, bf16 (float, uint16_t): float arithmetic, 16-bit wavefields, every
+ *   wavefield write rounded to the format (nearest-even). */
+#define ORC_DECLARE_OPS(R, W, S)                                                       \
+int  orc_step_##S(const W *P, W *PP, const R *V2, const R *coefs,                      \
                   int nxe, int nye, int nze, int order);                               \
-void orc_taper_all_##S(R *P, const R *g, int nxe, int nye, int nze, int blen);         \
-void orc_taper_upper_##S(R *P, const R *g, int nxe, int nye, int nze, int blen);       \
-void orc_apply_source_##S(R *PP, int nxe, int nye, int nze, int sxe, int sye, int sze, \
+void orc_taper_all_##S(W *P, const R *g, int nxe, int nye, int nze, int blen);         \
+void orc_taper_upper_##S(W *P, const R *g, int nxe, int nye, int nze, int blen);       \
+void orc_apply_source_##S(W *PP, int nxe, int nye, int nze, int sxe, int sye, int sze, \
                           R val);                                                      \
-void orc_receivers_##S(R *PP, R *rcv, int nxe, int nye, int nze, int blen,             \
+void orc_receivers_##S(W *PP, R *rcv, int nxe, int nye, int nze, int blen,             \
                        const orc_rcv_geom *g, int ntstep, int lt, int restore);        \
-void orc_upb_##S(R *PP, R *upb, int nxe, int nye, int nze, int blen, int order,        \
+void orc_upb_##S(W *PP, W *upb, int nxe, int nye, int nze, int blen, int order,        \
                  int lt, int restore);                                                 \
-void orc_image_##S(R *img, const R *PS, const R *PR, int nxe, int nye, int nze,        \
+void orc_image_##S(R *img, const W *PS, const W *PR, int nxe, int nye, int nze,        \
                    int blen);                                                          \
 void orc_image_finish_##S(const R *img, R *out, int nxe, int nye, int nze, int blen,   \
                           const R *fcoefs);                                            \
@@ -110,10 +134,12 @@ int  orc_model_shot_##S(const R *v2, const R *coefs, const R *g, const R *src,  
 int  orc_migrate_shot_##S(const R *v2, const R *coefs, const R *g, const R *src,       \
                           int nxe, int nye, int nze, int blen, int order, int nt,      \
                           int ntstep, int sx, int sy, int sz, const orc_rcv_geom *rg,  \
-                          const R *rcv, int hbc, R *img, R *fwd_last);
+                          const R *rcv, int hbc, R *img, W *fwd_last);
 
-ORC_DECLARE_OPS(float, f32)
-ORC_DECLARE_OPS(double, f64)
+ORC_DECLARE_OPS(float, float, f32)
+ORC_DECLARE_OPS(double, double, f64)
+ORC_DECLARE_OPS(float, uint16_t, f16)
+ORC_DECLARE_OPS(float, uint16_t, bf16)
 
 #ifdef __cplusplus
 }

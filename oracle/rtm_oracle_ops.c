@@ -2,10 +2,18 @@
  * rtm_oracle_ops.c -- per-step operators and time loops of the RTM oracle.
  * TEST INFRASTRUCTURE ONLY (see rtm_oracle.h).
  *
- * Compiled twice: -DORC_REAL=float -DORC_SFX=f32 and -DORC_REAL=double
- * -DORC_SFX=f64.  The float build restates the reference CPU platform
+ * Compiled four times: -DORC_REAL=float -DORC_SFX=f32, -DORC_REAL=double
+ * -DORC_SFX=f64, and the 16-bit storage builds -DORC_REAL=float with
+ * -DORC_SFX=f16 -DORC_STORE_F16 or -DORC_SFX=bf16 -DORC_STORE_BF16.  The
+ * float build restates the reference CPU platform
  * (rtm-3d/src/platform/host/RTMCPUPlatform.cpp) operation for operation;
  * -ffp-contract=off keeps every multiply and add separately rounded.
+ *
+ * Wavefields (P, PP, the UPB layer) have storage type W.  In the 16-bit
+ * builds every operator computes in float exactly as the float build and
+ * every wavefield write rounds to the storage format (nearest, ties to
+ * even): the reference's operator sequence with 16-bit grids.  Velocity,
+ * taps, traces and the image stay float.
  */
 #include <stdlib.h>
 #include <string.h>
@@ -15,6 +23,19 @@
 #error "compile with -DORC_REAL=float|double -DORC_SFX=f32|f64"
 #endif
 typedef ORC_REAL R;
+#if defined(ORC_STORE_F16)
+typedef uint16_t W;
+#define LD(v) orc_f16_to_f32(v)
+#define ST(x) orc_f32_to_f16(x)
+#elif defined(ORC_STORE_BF16)
+typedef uint16_t W;
+#define LD(v) orc_bf16_to_f32(v)
+#define ST(x) orc_f32_to_bf16(x)
+#else
+typedef R W;
+#define LD(v) (v)
+#define ST(x) (x)
+#endif
 #define CAT2(a, b) a##_##b
 #define CAT(a, b) CAT2(a, b)
 #define FN(name) CAT(name, ORC_SFX)
@@ -25,7 +46,7 @@ typedef ORC_REAL R;
  * acm_d accumulated from 0 in ascending tap order; lapl = (acmx+acmy)+acmz;
  * PP <- ((2*p) - pp) + v2dt2*lapl, in place.  Returns -1 on NaN
  * ("Numerical Dispersion"), 0 otherwise. */
-int FN(orc_step)(const R *P, R *PP, const R *V2, const R *coefs, int nxe, int nye, int nze,
+int FN(orc_step)(const W *P, W *PP, const R *V2, const R *coefs, int nxe, int nye, int nze,
                  int order)
 {
     const int hf = order / 2;
@@ -40,14 +61,14 @@ int FN(orc_step)(const R *P, R *PP, const R *V2, const R *coefs, int nxe, int ny
                 R ax = 0.0, ay = 0.0, az = 0.0;
                 for (int io = 0; io <= order; io++) {
                     ptrdiff_t d = io - hf;
-                    ax += P[o + d * (ptrdiff_t)sx] * cx[io];
-                    ay += P[o + d * (ptrdiff_t)sy] * cy[io];
-                    az += P[o + d] * cz[io];
+                    ax += LD(P[o + d * (ptrdiff_t)sx]) * cx[io];
+                    ay += LD(P[o + d * (ptrdiff_t)sy]) * cy[io];
+                    az += LD(P[o + d]) * cz[io];
                 }
                 R lapl = ax + ay + az;
-                R p = P[o], pp = PP[o];
+                R p = LD(P[o]), pp = LD(PP[o]);
                 R npp = 2 * p - pp + V2[o] * lapl;
-                PP[o] = npp;
+                PP[o] = ST(npp);
                 if (npp != npp) bad = 1;
             }
         }
@@ -57,29 +78,29 @@ int FN(orc_step)(const R *P, R *PP, const R *V2, const R *coefs, int nxe, int ny
 /* rtm-3d/src/platform/host/RTMCPUPlatform.cpp:710-920 (taper3D).  The loop
  * nests run in the reference's order; the front-right z-edge reads the
  * already face-tapered cell (blen, ny-blen+iy, iz) instead of its own. */
-void FN(orc_taper_all)(R *P, const R *g, int nxe, int nye, int nze, int blen)
+void FN(orc_taper_all)(W *P, const R *g, int nxe, int nye, int nze, int blen)
 {
     const int nx = nxe, ny = nye, nz = nze, b = blen;
 #pragma omp parallel for collapse(2)
     for (int ix = b; ix < nx - b; ix++)
         for (int iy = b; iy < ny - b; iy++)
             for (int iz = 0; iz < b; iz++) {
-                P[IDX(ix, iy, iz)] *= g[b - 1 - iz];
-                P[IDX(ix, iy, nz - b + iz)] *= g[iz];
+                P[IDX(ix, iy, iz)] = ST(LD(P[IDX(ix, iy, iz)]) * g[b - 1 - iz]);
+                P[IDX(ix, iy, nz - b + iz)] = ST(LD(P[IDX(ix, iy, nz - b + iz)]) * g[iz]);
             }
 #pragma omp parallel for collapse(2)
     for (int ix = b; ix < nx - b; ix++)
         for (int iz = b; iz < nz - b; iz++)
             for (int iy = 0; iy < b; iy++) {
-                P[IDX(ix, iy, iz)] *= g[b - 1 - iy];
-                P[IDX(ix, ny - b + iy, iz)] *= g[iy];
+                P[IDX(ix, iy, iz)] = ST(LD(P[IDX(ix, iy, iz)]) * g[b - 1 - iy]);
+                P[IDX(ix, ny - b + iy, iz)] = ST(LD(P[IDX(ix, ny - b + iy, iz)]) * g[iy]);
             }
 #pragma omp parallel for collapse(2)
     for (int iy = b; iy < ny - b; iy++)
         for (int iz = b; iz < nz - b; iz++)
             for (int ix = 0; ix < b; ix++) {
-                P[IDX(ix, iy, iz)] *= g[b - 1 - ix];
-                P[IDX(nx - b + ix, iy, iz)] *= g[ix];
+                P[IDX(ix, iy, iz)] = ST(LD(P[IDX(ix, iy, iz)]) * g[b - 1 - ix]);
+                P[IDX(nx - b + ix, iy, iz)] = ST(LD(P[IDX(nx - b + ix, iy, iz)]) * g[ix]);
             }
 #pragma omp parallel for collapse(2)
     for (int ix = 0; ix < b; ix++)
@@ -88,61 +109,61 @@ void FN(orc_taper_all)(R *P, const R *g, int nxe, int nye, int nze, int blen)
                 const int has_y = (iy < b) || (iy >= ny - b);
                 const R gy = iy < b ? g[b - 1 - iy] : (iy >= ny - b ? g[iy - (ny - b)] : (R)1);
                 R v;
-                v = P[IDX(ix, iy, iz)];
+                v = LD(P[IDX(ix, iy, iz)]);
                 if (has_y) v *= gy;
                 v *= g[b - 1 - ix]; v *= g[b - 1 - iz];
-                P[IDX(ix, iy, iz)] = v;
-                v = P[IDX(nx - b + ix, iy, iz)];
+                P[IDX(ix, iy, iz)] = ST(v);
+                v = LD(P[IDX(nx - b + ix, iy, iz)]);
                 if (has_y) v *= gy;
                 v *= g[ix]; v *= g[b - 1 - iz];
-                P[IDX(nx - b + ix, iy, iz)] = v;
-                v = P[IDX(ix, iy, nz - b + iz)];
+                P[IDX(nx - b + ix, iy, iz)] = ST(v);
+                v = LD(P[IDX(ix, iy, nz - b + iz)]);
                 if (has_y) v *= gy;
                 v *= g[b - 1 - ix]; v *= g[iz];
-                P[IDX(ix, iy, nz - b + iz)] = v;
-                v = P[IDX(nx - b + ix, iy, nz - b + iz)];
+                P[IDX(ix, iy, nz - b + iz)] = ST(v);
+                v = LD(P[IDX(nx - b + ix, iy, nz - b + iz)]);
                 if (has_y) v *= gy;
                 v *= g[ix]; v *= g[iz];
-                P[IDX(nx - b + ix, iy, nz - b + iz)] = v;
+                P[IDX(nx - b + ix, iy, nz - b + iz)] = ST(v);
             }
 #pragma omp parallel for collapse(2)
     for (int iy = 0; iy < b; iy++)
         for (int iz = 0; iz < b; iz++)
             for (int ix = b; ix < nx - b; ix++) {
                 R v;
-                v = P[IDX(ix, iy, iz)]; v *= g[b - 1 - iy]; v *= g[b - 1 - iz];
-                P[IDX(ix, iy, iz)] = v;
-                v = P[IDX(ix, ny - b + iy, iz)]; v *= g[iy]; v *= g[b - 1 - iz];
-                P[IDX(ix, ny - b + iy, iz)] = v;
-                v = P[IDX(ix, iy, nz - b + iz)]; v *= g[b - 1 - iy]; v *= g[iz];
-                P[IDX(ix, iy, nz - b + iz)] = v;
-                v = P[IDX(ix, ny - b + iy, nz - b + iz)]; v *= g[iy]; v *= g[iz];
-                P[IDX(ix, ny - b + iy, nz - b + iz)] = v;
+                v = LD(P[IDX(ix, iy, iz)]); v *= g[b - 1 - iy]; v *= g[b - 1 - iz];
+                P[IDX(ix, iy, iz)] = ST(v);
+                v = LD(P[IDX(ix, ny - b + iy, iz)]); v *= g[iy]; v *= g[b - 1 - iz];
+                P[IDX(ix, ny - b + iy, iz)] = ST(v);
+                v = LD(P[IDX(ix, iy, nz - b + iz)]); v *= g[b - 1 - iy]; v *= g[iz];
+                P[IDX(ix, iy, nz - b + iz)] = ST(v);
+                v = LD(P[IDX(ix, ny - b + iy, nz - b + iz)]); v *= g[iy]; v *= g[iz];
+                P[IDX(ix, ny - b + iy, nz - b + iz)] = ST(v);
             }
 #pragma omp parallel for collapse(2)
     for (int ix = 0; ix < b; ix++)
         for (int iy = 0; iy < b; iy++)
             for (int iz = b; iz < nz - b; iz++) {
                 R v;
-                v = P[IDX(ix, iy, iz)]; v *= g[b - 1 - ix]; v *= g[b - 1 - iy];
-                P[IDX(ix, iy, iz)] = v;
-                v = P[IDX(b, ny - b + iy, iz)]; v *= g[b - 1 - ix]; v *= g[iy];
-                P[IDX(ix, ny - b + iy, iz)] = v;
-                v = P[IDX(nx - b + ix, iy, iz)]; v *= g[ix]; v *= g[b - 1 - iy];
-                P[IDX(nx - b + ix, iy, iz)] = v;
-                v = P[IDX(nx - b + ix, ny - b + iy, iz)]; v *= g[ix]; v *= g[iy];
-                P[IDX(nx - b + ix, ny - b + iy, iz)] = v;
+                v = LD(P[IDX(ix, iy, iz)]); v *= g[b - 1 - ix]; v *= g[b - 1 - iy];
+                P[IDX(ix, iy, iz)] = ST(v);
+                v = LD(P[IDX(b, ny - b + iy, iz)]); v *= g[b - 1 - ix]; v *= g[iy];
+                P[IDX(ix, ny - b + iy, iz)] = ST(v);
+                v = LD(P[IDX(nx - b + ix, iy, iz)]); v *= g[ix]; v *= g[b - 1 - iy];
+                P[IDX(nx - b + ix, iy, iz)] = ST(v);
+                v = LD(P[IDX(nx - b + ix, ny - b + iy, iz)]); v *= g[ix]; v *= g[iy];
+                P[IDX(nx - b + ix, ny - b + iy, iz)] = ST(v);
             }
 }
 
 /* rtm-3d/src/platform/host/RTMCPUPlatform.cpp:566-657 (taperUpper3D). */
-void FN(orc_taper_upper)(R *P, const R *g, int nxe, int nye, int nze, int blen)
+void FN(orc_taper_upper)(W *P, const R *g, int nxe, int nye, int nze, int blen)
 {
     const int nx = nxe, ny = nye, b = blen;
 #pragma omp parallel for collapse(2)
     for (int ix = b; ix < nx - b; ix++)
         for (int iy = b; iy < ny - b; iy++)
-            for (int iz = 0; iz < b; iz++) P[IDX(ix, iy, iz)] *= g[b - 1 - iz];
+            for (int iz = 0; iz < b; iz++) P[IDX(ix, iy, iz)] = ST(LD(P[IDX(ix, iy, iz)]) * g[b - 1 - iz]);
 #pragma omp parallel for collapse(2)
     for (int ix = 0; ix < b; ix++)
         for (int iz = 0; iz < b; iz++)
@@ -150,39 +171,39 @@ void FN(orc_taper_upper)(R *P, const R *g, int nxe, int nye, int nze, int blen)
                 const int has_y = (iy < b) || (iy >= ny - b);
                 const R gy = iy < b ? g[b - 1 - iy] : (iy >= ny - b ? g[iy - (ny - b)] : (R)1);
                 R v;
-                v = P[IDX(ix, iy, iz)];
+                v = LD(P[IDX(ix, iy, iz)]);
                 if (has_y) v *= gy;
                 v *= g[b - 1 - ix]; v *= g[b - 1 - iz];
-                P[IDX(ix, iy, iz)] = v;
-                v = P[IDX(nx - b + ix, iy, iz)];
+                P[IDX(ix, iy, iz)] = ST(v);
+                v = LD(P[IDX(nx - b + ix, iy, iz)]);
                 if (has_y) v *= gy;
                 v *= g[ix]; v *= g[b - 1 - iz];
-                P[IDX(nx - b + ix, iy, iz)] = v;
+                P[IDX(nx - b + ix, iy, iz)] = ST(v);
             }
 #pragma omp parallel for collapse(2)
     for (int iy = 0; iy < b; iy++)
         for (int iz = 0; iz < b; iz++)
             for (int ix = b; ix < nx - b; ix++) {
                 R v;
-                v = P[IDX(ix, iy, iz)]; v *= g[b - 1 - iy]; v *= g[b - 1 - iz];
-                P[IDX(ix, iy, iz)] = v;
-                v = P[IDX(ix, ny - b + iy, iz)]; v *= g[iy]; v *= g[b - 1 - iz];
-                P[IDX(ix, ny - b + iy, iz)] = v;
+                v = LD(P[IDX(ix, iy, iz)]); v *= g[b - 1 - iy]; v *= g[b - 1 - iz];
+                P[IDX(ix, iy, iz)] = ST(v);
+                v = LD(P[IDX(ix, ny - b + iy, iz)]); v *= g[iy]; v *= g[b - 1 - iz];
+                P[IDX(ix, ny - b + iy, iz)] = ST(v);
             }
 }
 
 /* rtm-3d/src/platform/host/RTMCPUPlatform.cpp:103-113 and
  * rtm-3d/src/grid/RTMGrid.cpp:36-40 (coordinates are extended ones). */
-void FN(orc_apply_source)(R *PP, int nxe, int nye, int nze, int sxe, int sye, int sze, R val)
+void FN(orc_apply_source)(W *PP, int nxe, int nye, int nze, int sxe, int sye, int sze, R val)
 {
     if (sxe < 0 || sxe >= nxe || sye < 0 || sye >= nye || sze < 0 || sze >= nze) return;
-    PP[IDX(sxe, sye, sze)] = PP[IDX(sxe, sye, sze)] + val;
+    PP[IDX(sxe, sye, sze)] = ST(LD(PP[IDX(sxe, sye, sze)]) + val);
 }
 
 /* rtm-3d/src/platform/host/RTMCPUPlatform.cpp:114-151.  rcv is one ntstep
  * chunk laid out [nrx][nry][ntstep]; restore SETs the wavefield value (the
  * CPU semantics, not the CUDA accumulate). */
-void FN(orc_receivers)(R *PP, R *rcv, int nxe, int nye, int nze, int blen, const orc_rcv_geom *g,
+void FN(orc_receivers)(W *PP, R *rcv, int nxe, int nye, int nze, int blen, const orc_rcv_geom *g,
                        int ntstep, int lt, int restore)
 {
     for (int ix = 0; ix < g->count_x; ix++)
@@ -192,14 +213,14 @@ void FN(orc_receivers)(R *PP, R *rcv, int nxe, int nye, int nze, int blen, const
             int rz = g->depth_z + blen;
             if (rx < 0 || rx >= nxe || ry < 0 || ry >= nye || rz < 0 || rz >= nze) continue;
             size_t ro = ((size_t)ix * g->count_y + iy) * (size_t)ntstep + lt;
-            if (restore) PP[IDX(rx, ry, rz)] = rcv[ro];
-            else rcv[ro] = PP[IDX(rx, ry, rz)];
+            if (restore) PP[IDX(rx, ry, rz)] = ST(rcv[ro]);
+            else rcv[ro] = LD(PP[IDX(rx, ry, rz)]);
         }
 }
 
 /* rtm-3d/src/platform/host/RTMCPUPlatform.cpp:153-189.  upb is
  * [ntstep][nxe][nye][order/2]; the layer is z in [blen-order/2, blen). */
-void FN(orc_upb)(R *PP, R *upb, int nxe, int nye, int nze, int blen, int order, int lt,
+void FN(orc_upb)(W *PP, W *upb, int nxe, int nye, int nze, int blen, int order, int lt,
                  int restore)
 {
     const int hf = order / 2;
@@ -216,14 +237,14 @@ void FN(orc_upb)(R *PP, R *upb, int nxe, int nye, int nze, int blen, int order, 
 
 /* rtm-3d/src/platform/host/RTMCPUPlatform.cpp:513-540: all x, y and
  * z in [blen, nze-blen); img <- img + PS*PR. */
-void FN(orc_image)(R *img, const R *PS, const R *PR, int nxe, int nye, int nze, int blen)
+void FN(orc_image)(R *img, const W *PS, const W *PR, int nxe, int nye, int nze, int blen)
 {
 #pragma omp parallel for collapse(2)
     for (int ix = 0; ix < nxe; ix++)
         for (int iy = 0; iy < nye; iy++)
             for (int iz = blen; iz < nze - blen; iz++) {
                 size_t o = IDX(ix, iy, iz);
-                img[o] = img[o] + PS[o] * PR[o];
+                img[o] = img[o] + LD(PS[o]) * LD(PR[o]);
             }
 }
 
@@ -251,9 +272,9 @@ void FN(orc_image_finish)(const R *img, R *out, int nxe, int nye, int nze, int b
             }
 }
 
-static R *zalloc(size_t n)
+static W *zalloc(size_t n)
 {
-    return (R *)calloc(n, sizeof(R));
+    return (W *)calloc(n, sizeof(W));
 }
 
 /* rtm-3d/src/kernel/RTMFiniteDifferencesKernel.cpp:129-246 (modeling loop):
@@ -266,12 +287,12 @@ int FN(orc_model_shot)(const R *v2, const R *coefs, const R *g, const R *src, in
 {
     const size_t E = (size_t)nxe * nye * nze;
     const size_t chunk = (size_t)rg->count_x * rg->count_y * ntstep;
-    R *A = zalloc(E), *B = zalloc(E);
-    R *P = A, *PP = B;
+    W *A = zalloc(E), *B = zalloc(E);
+    W *P = A, *PP = B;
     int rc = 0;
     if (!A || !B) { free(A); free(B); return -2; }
     for (int it = 0; it < nt && rc == 0; it++) {
-        R *t = P; P = PP; PP = t;
+        W *t = P; P = PP; PP = t;
         FN(orc_taper_all)(P, g, nxe, nye, nze, blen);
         FN(orc_taper_all)(PP, g, nxe, nye, nze, blen);
         rc = FN(orc_step)(P, PP, v2, coefs, nxe, nye, nze, order);
@@ -294,19 +315,19 @@ int FN(orc_model_shot)(const R *v2, const R *coefs, const R *g, const R *src, in
 int FN(orc_migrate_shot)(const R *v2, const R *coefs, const R *g, const R *src, int nxe,
                          int nye, int nze, int blen, int order, int nt, int ntstep, int sx,
                          int sy, int sz, const orc_rcv_geom *rg, const R *rcv, int hbc, R *img,
-                         R *fwd_last)
+                         W *fwd_last)
 {
     const size_t E = (size_t)nxe * nye * nze;
     const size_t chunk = (size_t)rg->count_x * rg->count_y * ntstep;
     const int hf = order / 2;
-    R *A = zalloc(E), *B = zalloc(E), *C = zalloc(E), *D = zalloc(E);
-    R *upb = hbc ? zalloc((size_t)nt * nxe * nye * hf) : NULL;
-    R *snap0, *snap1;
+    W *A = zalloc(E), *B = zalloc(E), *C = zalloc(E), *D = zalloc(E);
+    W *upb = hbc ? zalloc((size_t)nt * nxe * nye * hf) : NULL;
+    W *snap0, *snap1;
     int rc = 0;
     if (!A || !B || !C || !D || (hbc && !upb)) rc = -2;
-    R *P = A, *PP = B;
+    W *P = A, *PP = B;
     for (int it = 0; it < nt && rc == 0; it++) {
-        R *t = P; P = PP; PP = t;
+        W *t = P; P = PP; PP = t;
         if (hbc) {
             FN(orc_taper_upper)(P, g, nxe, nye, nze, blen);
             FN(orc_taper_upper)(PP, g, nxe, nye, nze, blen);
@@ -317,12 +338,12 @@ int FN(orc_migrate_shot)(const R *v2, const R *coefs, const R *g, const R *src, 
     }
     snap0 = P;  /* p(nt-2) after its forward taper */
     snap1 = PP; /* p(nt-1) */
-    if (fwd_last && rc == 0) memcpy(fwd_last, snap1, E * sizeof(R));
+    if (fwd_last && rc == 0) memcpy(fwd_last, snap1, E * sizeof(W));
     /* Backward: C/D hold the receiver field; the source field reuses A/B via
      * the snapshot seeding (no copies needed). */
     memset(img, 0, E * sizeof(R));
-    R *Ps = snap1, *PPs = snap0;
-    R *Pr = C, *PPr = D;
+    W *Ps = snap1, *PPs = snap0;
+    W *Pr = C, *PPr = D;
     for (int it = nt - 1; it >= 0 && rc == 0; it--) {
         if (it == nt - 1) {
             /* ppSrc <- snap1, swap: Ps holds p(nt-1); PPs holds zeros that
@@ -335,7 +356,7 @@ int FN(orc_migrate_shot)(const R *v2, const R *coefs, const R *g, const R *src, 
         } else {
             rc = FN(orc_step)(Ps, PPs, v2, coefs, nxe, nye, nze, order);
             if (hbc) FN(orc_upb)(PPs, upb, nxe, nye, nze, blen, order, it, 1);
-            { R *t = Ps; Ps = PPs; PPs = t; }
+            { W *t = Ps; Ps = PPs; PPs = t; }
         }
         if (hbc) {
             FN(orc_taper_upper)(Pr, g, nxe, nye, nze, blen);
@@ -345,7 +366,7 @@ int FN(orc_migrate_shot)(const R *v2, const R *coefs, const R *g, const R *src, 
         FN(orc_receivers)(PPr, (R *)rcv + (size_t)(it / ntstep) * chunk, nxe, nye, nze, blen, rg,
                           ntstep, it % ntstep, 1);
         FN(orc_image)(img, Ps, PPr, nxe, nye, nze, blen);
-        { R *t = Pr; Pr = PPr; PPr = t; }
+        { W *t = Pr; Pr = PPr; PPr = t; }
     }
     free(A); free(B); free(C); free(D); free(upb);
     return rc;

@@ -296,3 +296,83 @@ void orc_v2dt2_f64(const float *v, size_t n, double dt, double *out)
 {
     for (size_t i = 0; i < n; i++) out[i] = (double)v[i] * (double)v[i] * dt * dt;
 }
+
+/* ---------------- 16-bit storage conversions ----------------------------
+ * IEEE 754 binary16 and bfloat16 (the top half of binary32), both rounded
+ * to nearest, ties to even, with subnormals kept and overflow to infinity.
+ * Written from the format definitions; pinned in the tests against numpy's
+ * float16 casts (exhaustively for the widening direction) and torch's
+ * bfloat16 casts. */
+uint16_t orc_f32_to_f16(float f)
+{
+    uint32_t u;
+    memcpy(&u, &f, 4);
+    const uint32_t sign = (u >> 16) & 0x8000u, a = u & 0x7fffffffu;
+    if (a > 0x7f800000u) return (uint16_t)(sign | 0x7e00u | ((a >> 13) & 0x3ffu)); /* NaN */
+    if (a >= 0x477ff000u) return (uint16_t)(sign | 0x7c00u); /* >= 65520 rounds to inf */
+    if (a >= 0x38800000u) {                                  /* normal: >= 2^-14 */
+        const uint32_t m = a - 0x38000000u;                  /* exponent rebias 127 -> 15 */
+        return (uint16_t)(sign | ((m + 0xfffu + ((m >> 13) & 1u)) >> 13));
+    }
+    if (a < 0x33000000u) return (uint16_t)sign; /* below half the smallest subnormal */
+    const int e = (int)(a >> 23);               /* 102..112 */
+    const uint32_t mant = (a & 0x7fffffu) | 0x800000u;
+    const int shift = 126 - e;                  /* value / 2^-24 = mant >> shift */
+    uint32_t q = mant >> shift;
+    const uint32_t rem = mant & ((1u << shift) - 1u), half = 1u << (shift - 1);
+    if (rem > half || (rem == half && (q & 1u))) q++;
+    return (uint16_t)(sign | q);
+}
+
+float orc_f16_to_f32(uint16_t h)
+{
+    const uint32_t sign = (uint32_t)(h & 0x8000u) << 16;
+    uint32_t e = (h >> 10) & 0x1fu, m = h & 0x3ffu, u;
+    if (e == 0x1fu) {
+        u = sign | 0x7f800000u | (m << 13);
+    } else if (e) {
+        u = sign | ((e + 112u) << 23) | (m << 13);
+    } else if (!m) {
+        u = sign;
+    } else {
+        int s = 0;
+        while (!(m & 0x400u)) { m <<= 1; s++; }
+        u = sign | ((uint32_t)(113 - s) << 23) | ((m & 0x3ffu) << 13);
+    }
+    float f;
+    memcpy(&f, &u, 4);
+    return f;
+}
+
+uint16_t orc_f32_to_bf16(float f)
+{
+    uint32_t u;
+    memcpy(&u, &f, 4);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u); /* quiet NaN */
+    return (uint16_t)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+}
+
+float orc_bf16_to_f32(uint16_t b)
+{
+    const uint32_t u = (uint32_t)b << 16;
+    float f;
+    memcpy(&f, &u, 4);
+    return f;
+}
+
+void orc_to_f16(const float *in, uint16_t *out, size_t n)
+{
+    for (size_t i = 0; i < n; i++) out[i] = orc_f32_to_f16(in[i]);
+}
+void orc_from_f16(const uint16_t *in, float *out, size_t n)
+{
+    for (size_t i = 0; i < n; i++) out[i] = orc_f16_to_f32(in[i]);
+}
+void orc_to_bf16(const float *in, uint16_t *out, size_t n)
+{
+    for (size_t i = 0; i < n; i++) out[i] = orc_f32_to_bf16(in[i]);
+}
+void orc_from_bf16(const uint16_t *in, float *out, size_t n)
+{
+    for (size_t i = 0; i < n; i++) out[i] = orc_bf16_to_f32(in[i]);
+}
