@@ -14,7 +14,9 @@ x (3 nt - 2) wavefield steps / shot time), RTM shots/hour, the roofline of the
 dominant kernel (the stencil step: 16 algorithmic bytes per updated cell,
 durations from HIP events around every launch in the timed region), and the
 reference CPU program timed on a bounded sample of the same workload on this
-host's cores.
+host's cores.  Beside the FP32 line: the same shot with FP16 and with BF16
+wavefield storage (FP32 arithmetic; 10 algorithmic bytes per stencil cell),
+and every format's trace / image error against the FP64 oracle on a crop.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -71,6 +73,7 @@ class Shot:
                                    p["source_depth_z"] + b), self.geom)
         self.F = hip.FieldPair(self.shape)
         self.R = hip.FieldPair(self.shape)
+        self.extra = {}  # 16-bit wavefield buffers per format
         self.rcv = hip.DeviceArray((p["receiver_count_x"], p["receiver_count_y"], nt)).zero()
         self.img = hip.DeviceArray(self.shape).zero()
         self.upb = hip.DeviceArray((nt, self.shape[0], self.shape[1], order // 2))
@@ -83,14 +86,34 @@ class Shot:
         del vel
         self.U = float((self.shape[0] - order) * (self.shape[1] - order) * (self.shape[2] - order))
 
-    def migrate(self):
+    def buffers(self, fmt):
+        """(shot, F, R, upb) for a wavefield storage format."""
+        if fmt == "f32":
+            return self.shot, self.F, self.R, self.upb
+        if fmt not in self.extra:
+            hip, p, b = self.hip, self.p, self.b
+            dt = hip.WAVE_FORMATS[fmt][1]
+            shot = hip.make_shot(*self.shape, b, self.order, self.nt, self.nt,
+                                 (p["source_start_x"] + b, p["source_start_y"] + b,
+                                  p["source_depth_z"] + b), self.geom, fmt)
+            self.extra[fmt] = (shot, hip.FieldPair(self.shape, dt), hip.FieldPair(self.shape, dt),
+                               hip.DeviceArray((self.nt, self.shape[0], self.shape[1],
+                                                self.order // 2), dt))
+        return self.extra[fmt]
+
+    def release(self, fmt):
+        if fmt in self.extra:
+            for x in self.extra.pop(fmt)[1:]:
+                x.free()
+
+    def migrate(self, fmt="f32"):
         hip = self.hip
-        for f in (self.F.a, self.F.b, self.R.a, self.R.b, self.img):
+        shot, F, R, upb = self.buffers(fmt)
+        for f in (F.a, F.b, R.a, R.b, self.img):
             f.zero()
-        hip.forward_steps(self.shot, True, self.V, self.coefs, self.g, self.src, self.F,
-                          self.upb, 0, self.nt)
-        hip.backward_steps(self.shot, True, self.V, self.coefs, self.g, self.F, self.R, self.upb,
-                           self.rcv, self.img, self.nt - 1, self.nt, True)
+        hip.forward_steps(shot, True, self.V, self.coefs, self.g, self.src, F, upb, 0, self.nt)
+        hip.backward_steps(shot, True, self.V, self.coefs, self.g, F, R, upb, self.rcv, self.img,
+                           self.nt - 1, self.nt, True)
 
 
 def read_traffic():
@@ -137,10 +160,11 @@ def cpu_reference_sample(workloads, steps, cores):
         shutil.rmtree(wd, ignore_errors=True)
 
 
-def accuracy_crop(hip, host, workloads):
-    """FP32 device result vs the FP64 oracle on a crop of the salt model
+def accuracy_crop(hip, host, workloads, formats=("f32",)):
+    """Device results vs the FP64 oracle on a crop of the salt model
     (80x80x64 + border 16, 400 steps): relative L2 of the receiver traces and
-    of the migrated HBC image."""
+    of the migrated HBC image, for each wavefield storage format (each
+    format models its own record and migrates it)."""
     sys.path.insert(0, os.path.join(REPO, "oracle"))
     import pyoracle as po
 
@@ -158,30 +182,36 @@ def accuracy_crop(hip, host, workloads):
     vpe = host.extend_velocity(vel, b, "hbc")
     shape = ve.shape
     V = hip.DeviceArray.from_host(host.v2dt2(ve, p["dt"]))
+    Vh = hip.DeviceArray.from_host(host.v2dt2(vpe, p["dt"]))
     dsrc, dg = hip.DeviceArray.from_host(src), hip.DeviceArray.from_host(g)
-    shot = hip.make_shot(*shape, b, order, nt, nt, (40 + b, 40 + b, 2 + b), geom)
-    F = hip.FieldPair(shape)
-    rcv = hip.DeviceArray((20, 20, nt)).zero()
-    hip.model_steps(shot, V, coefs, dg, dsrc, F, rcv, 0, nt)
-    rec32 = rcv.download()
-    V.upload(host.v2dt2(vpe, p["dt"]))
-    S, R = hip.FieldPair(shape), hip.FieldPair(shape)
-    upb = hip.DeviceArray((nt, shape[0], shape[1], 4))
-    img = hip.DeviceArray(shape).zero()
-    hip.forward_steps(shot, True, V, coefs, dg, dsrc, S, upb, 0, nt)
-    hip.backward_steps(shot, True, V, coefs, dg, S, R, upb, rcv, img, nt - 1, nt, True)
-    out = hip.DeviceArray((80, 80, 64))
-    hip.image_finish(img, out, b, host.stencil_coefs(2, d, d, d))
-    img32 = out.download()
-    for x in (V, dsrc, dg, F, rcv, S, R, upb, img, out):
+    results = {}
+    for fmt in formats:
+        wdt = hip.WAVE_FORMATS[fmt][1]
+        shot = hip.make_shot(*shape, b, order, nt, nt, (40 + b, 40 + b, 2 + b), geom, fmt)
+        F = hip.FieldPair(shape, wdt)
+        rcv = hip.DeviceArray((20, 20, nt)).zero()
+        hip.model_steps(shot, V, coefs, dg, dsrc, F, rcv, 0, nt)
+        rec = rcv.download()
+        S, R = hip.FieldPair(shape, wdt), hip.FieldPair(shape, wdt)
+        upb = hip.DeviceArray((nt, shape[0], shape[1], 4), wdt)
+        img = hip.DeviceArray(shape).zero()
+        hip.forward_steps(shot, True, Vh, coefs, dg, dsrc, S, upb, 0, nt)
+        hip.backward_steps(shot, True, Vh, coefs, dg, S, R, upb, rcv, img, nt - 1, nt, True)
+        out = hip.DeviceArray((80, 80, 64))
+        hip.image_finish(img, out, b, host.stencil_coefs(2, d, d, d))
+        results[fmt] = (rec, out.download())
+        for x in (F, rcv, S, R, upb, img, out):
+            x.free()
+    for x in (V, Vh, dsrc, dg):
         x.free()
     rec64 = po.pipeline_modeling(p, vel, np.float64)[0, 0]
     q = dict(p, modeling=False, migration=True)
     img64 = po.pipeline_migration(q, vpe, rec64[None, None], np.float64)
     rel = lambda a, r: float(np.linalg.norm(a - r) / np.linalg.norm(r))
-    return {"trace_rel_l2": rel(rec32.astype(np.float64), rec64),
-            "image_rel_l2": rel(img32.astype(np.float64), img64),
-            "case": "80x80x64 crop of the salt model, border 16, 400 steps, 20x20 receivers"}
+    return {fmt: {"trace_rel_l2": rel(rec.astype(np.float64), rec64),
+                  "image_rel_l2": rel(im.astype(np.float64), img64)}
+            for fmt, (rec, im) in results.items()}, \
+        "80x80x64 crop of the salt model, border 16, 400 steps, 20x20 receivers"
 
 
 def main():
@@ -193,6 +223,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=6)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--no-precision", action="store_true",
+                    help="skip the FP16 / BF16 wavefield-storage shots")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on one GPU only: the propagation path is measured on a single "
@@ -273,6 +305,35 @@ def main():
                                  "algorithmic_bytes_per_launch": bytes_per_launch},
         "tiling": tiling,
     }
+    # 16-bit wavefield storage: the same shot, one warm-up and one timed shot
+    # per format (arithmetic stays FP32; 10 algorithmic bytes per stencil cell)
+    formats = ["f32"]
+    if not args.no_precision:
+        prec = {}
+        for fmt in ("f16", "bf16"):
+            sh.migrate(fmt)
+            hip.synchronize()
+            hip.timing_enable(True)
+            t = time.perf_counter()
+            sh.migrate(fmt)
+            hip.synchronize()
+            s16 = time.perf_counter() - t
+            k0, m0 = hip.timing_read(0)
+            hip.timing_read(1)
+            hip.timing_read(2)
+            hip.timing_enable(False)
+            sh.release(fmt)
+            a0 = m0 / max(k0, 1) * 1e-3
+            prec[fmt] = {"ms_per_shot": round(s16 * 1e3, 2),
+                         "value": round(updates / s16 / 1e6, 1),
+                         "shots_per_hour": round(3600.0 / s16, 1),
+                         "speedup_vs_f32": round(shot_s / s16, 3),
+                         "stencil_avg_ms": round(a0 * 1e3, 4),
+                         "stencil_achieved_GBs": round(10.0 * sh.U / a0 / 1e9, 1),
+                         "algorithmic_bytes_per_launch": 10.0 * sh.U}
+            log("%s shot: %.3f s (%.2fx FP32)" % (fmt, s16, shot_s / s16))
+            formats.append(fmt)
+        line["wave_formats"] = prec
     for x in (sh.V, sh.src, sh.g, sh.F, sh.R, sh.rcv, sh.img, sh.upb):
         x.free()
 
@@ -280,8 +341,10 @@ def main():
         cores = min(16, os.cpu_count() or 1)
         cpu = None
         try:
-            acc = accuracy_crop(hip, host, workloads)
-            line["accuracy_vs_fp64"] = acc
+            acc, case = accuracy_crop(hip, host, workloads, tuple(formats))
+            line["accuracy_vs_fp64"] = dict(acc["f32"], case=case)
+            for fmt in formats[1:]:
+                line["wave_formats"][fmt].update(acc[fmt])
         except Exception as e:  # reported, never hidden
             line["accuracy_vs_fp64"] = {"error": str(e)}
         if os.path.exists(REFBIN):

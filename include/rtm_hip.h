@@ -59,7 +59,19 @@ typedef struct {
     int ntstep;          /* receiver-chunk length (divides nt) */
     int sxe, sye, sze;   /* source cell, extended coordinates */
     rtmhip_rcv_geom rcv; /* receiver geometry */
+    int wave_format;     /* RTMHIP_WAVE_*: storage of the wavefields and UPB */
 } rtmhip_shot;
+
+/* Wavefield storage formats of the time loops (no reference counterpart:
+ * the reference stores FP32 only).  Arithmetic is FP32 in every format; with
+ * F16 or BF16 every wavefield pointer of the loop entry points (P, PP, Ps,
+ * PPs, Pr, PPr and upb) refers to 2-byte elements and each wavefield write
+ * is rounded to the format (nearest-even), exactly the reference's operator
+ * sequence run on 16-bit grids.  Velocity, taper, Ricker, traces and the
+ * image stay FP32.  16-bit formats support stencil order 8 only. */
+#define RTMHIP_WAVE_F32  0
+#define RTMHIP_WAVE_F16  1
+#define RTMHIP_WAVE_BF16 2
 
 /* ================= lifecycle =================
  * Replaces RTMGPUPlatform::initRTMPlatform / destroyRTMPlatform
@@ -85,6 +97,9 @@ int rtmhip_copy_to_device(void *dst, const void *src, size_t bytes);
 int rtmhip_copy_from_device(void *dst, const void *src, size_t bytes);
 int rtmhip_copy_device(void *dst, const void *src, size_t bytes);
 int rtmhip_fill(float *dptr, size_t count, float value);
+/* count 2-byte elements set to the bit pattern `bits` (16-bit wavefields;
+ * 0 is +0.0 in both F16 and BF16).  No reference counterpart. */
+int rtmhip_fill16(void *dptr, size_t count, unsigned short bits);
 
 /* ================= per-step operators =================
  * One entry per RTMPlatform virtual (rtm-3d/include/RTMPlatform.hpp:28-102).
@@ -184,8 +199,11 @@ int rtmhip_backward_steps(const rtmhip_shot *shot, int hbc, const float *V2DT2,
  *
  * rtmhip_set_tiling selects the tile shape of one class: tz lanes along z,
  * by thread rows, ry rows per thread (compiled set: 64x4x4, 64x4x2, 64x8x2,
- * 32x8x4, 32x8x2), xchunk planes marched per block (0 = sized from the
- * device's resident-block capacity).  tz = 0 restores the default 64x4x4. */
+ * 32x8x4, 32x8x2; tz = 1 selects full-z-row blocks for order 8, by x ry =
+ * 4x4 for nze <= 256, 2x6 or 2x7 for nze <= 384), xchunk planes marched per
+ * block (0 = sized from the device's resident-block capacity).  tz = 0
+ * restores the default: full-z-row blocks where they fit, else 64x4x4.
+ * 16-bit wavefields use the default shapes. */
 int rtmhip_set_tiling(int kernel_class, int tz, int by, int ry, int xchunk);
 int rtmhip_get_tiling(int kernel_class, int *tz, int *by, int *ry, int *xchunk);
 

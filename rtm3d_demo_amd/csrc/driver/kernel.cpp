@@ -77,7 +77,8 @@ rtmhip_rcv_geom RTMKernel::geom() const
 
 rtmhip_shot RTMKernel::make_shot(const ShotPos &s) const
 {
-    rtmhip_shot sh;
+    rtmhip_shot sh{};
+    sh.wave_format = RTMHIP_WAVE_F32;
     sh.nxe = d_.nxe;
     sh.nye = d_.nye;
     sh.nze = d_.nze;

@@ -105,8 +105,9 @@ int check_rcv(const rtmhip_rcv_geom *g, int nze, int blen)
     return RTMHIP_OK;
 }
 
-hipError_t stencil(int radius, const float *P, float *PP, const float *V, const Coefs &cf,
-                   const Grid &g, const ImageFuse *fuse, const TaperFuse *taper = nullptr)
+template <typename T>
+hipError_t stencil(int radius, const T *P, T *PP, const float *V, const Coefs &cf, const Grid &g,
+                   const ImageFuse *fuse, const TaperFuse *taper = nullptr)
 {
     // kernel class: 0 plain step, 1 fused receiver step, 2 tapered HBC step
     const int cls = fuse ? 1 : (taper ? 2 : 0);
@@ -185,6 +186,34 @@ int prepare_shot(const rtmhip_shot *s, const float *coefs, ShotCtx *c)
     c->ry0 = s->rcv.start_y + s->blen;
     c->rz = s->rcv.depth_z + s->blen;
     return RTMHIP_OK;
+}
+
+int check_format(const rtmhip_shot *s)
+{
+    if (s->wave_format != RTMHIP_WAVE_F32 && s->wave_format != RTMHIP_WAVE_F16 &&
+        s->wave_format != RTMHIP_WAVE_BF16)
+        return fail(RTMHIP_ERR_INVALID, "unknown wave_format %d", s->wave_format);
+    if (s->wave_format != RTMHIP_WAVE_F32 && s->order != 8)
+        return fail(RTMHIP_ERR_UNSUPPORTED, "16-bit wavefields need stencil order 8 (got %d)",
+                    s->order);
+    return RTMHIP_OK;
+}
+
+// Calls f(T{}) with T the storage type of the shot's wavefields.
+template <typename F>
+int with_format(const rtmhip_shot *s, F &&f)
+{
+    switch (s->wave_format) {
+    case RTMHIP_WAVE_F16: return f(_Float16{});
+    case RTMHIP_WAVE_BF16: return f(__bf16{});
+    default: return f(float{});
+    }
+}
+
+template <typename T>
+T **as(float **p)
+{
+    return reinterpret_cast<T **>(p);
 }
 
 int check_chunk(const rtmhip_shot *s, int lo, int hi)
@@ -308,6 +337,14 @@ int rtmhip_fill(float *dptr, size_t count, float value)
     return RTMHIP_OK;
 }
 
+int rtmhip_fill16(void *dptr, size_t count, unsigned short bits)
+{
+    NEED_INIT();
+    if (!dptr && count) return fail(RTMHIP_ERR_INVALID, "NULL pointer");
+    HIPCHK(launch_fill_u16(static_cast<uint16_t *>(dptr), count, bits, g_state.stream), "fill16");
+    return RTMHIP_OK;
+}
+
 int rtmhip_step(const float *P, float *PP, const float *V2DT2, const float *coefs, int nxe,
                 int nye, int nze, int order)
 {
@@ -421,21 +458,17 @@ int rtmhip_stack(float *dst, const float *src, size_t count)
     return RTMHIP_OK;
 }
 
-int rtmhip_model_steps(const rtmhip_shot *shot, const float *V2DT2, const float *coefs,
-                       const float *taper, const float *ricker, float **P, float **PP,
-                       float *rcv_chunk, int it0, int nsteps)
+}  // extern "C"
+
+namespace {
+
+template <typename T>
+int model_loop(const rtmhip_shot *shot, const ShotCtx &c, const float *V2DT2, const float *taper,
+               const float *ricker, T **P, T **PP, float *rcv_chunk, int it0, int nsteps)
 {
-    NEED_INIT();
-    ShotCtx c;
-    int rc;
-    if ((rc = prepare_shot(shot, coefs, &c))) return rc;
-    if (!V2DT2 || !taper || !ricker || !P || !PP || !*P || !*PP || !rcv_chunk)
-        return fail(RTMHIP_ERR_INVALID, "NULL pointer");
-    if (nsteps <= 0) return RTMHIP_OK;
-    if ((rc = check_chunk(shot, it0, it0 + nsteps - 1))) return rc;
     hipStream_t s = g_state.stream;
     for (int it = it0; it < it0 + nsteps; ++it) {
-        float *t = *P;
+        T *t = *P;
         *P = *PP;
         *PP = t;
         HIPCHK(launch_taper(*P, taper, c.g, shot->blen, false, s), "taper");
@@ -450,19 +483,11 @@ int rtmhip_model_steps(const rtmhip_shot *shot, const float *V2DT2, const float 
     return RTMHIP_OK;
 }
 
-int rtmhip_forward_steps(const rtmhip_shot *shot, int hbc, const float *V2DT2,
-                         const float *coefs, const float *taper, const float *ricker, float **P,
-                         float **PP, float *upb, int it0, int nsteps)
+template <typename T>
+int forward_loop(const rtmhip_shot *shot, const ShotCtx &c, int hbc, const float *V2DT2,
+                 const float *taper, const float *ricker, T **P, T **PP, T *upb, int it0,
+                 int nsteps)
 {
-    NEED_INIT();
-    ShotCtx c;
-    int rc;
-    if ((rc = prepare_shot(shot, coefs, &c))) return rc;
-    if (!V2DT2 || !ricker || !P || !PP || !*P || !*PP || (hbc && (!taper || !upb)))
-        return fail(RTMHIP_ERR_INVALID, "NULL pointer");
-    if (nsteps <= 0) return RTMHIP_OK;
-    if (it0 < 0 || it0 + nsteps > shot->nt)
-        return fail(RTMHIP_ERR_INVALID, "time range outside [0,%d)", shot->nt);
     hipStream_t s = g_state.stream;
     // HBC: the two in-place upper tapers of each step are fused into the
     // stencil (P tapered on load, the centre PP tapered twice -- once on the
@@ -470,7 +495,7 @@ int rtmhip_forward_steps(const rtmhip_shot *shot, int hbc, const float *V2DT2,
     // still pending on P is applied in place after the last step, so the
     // state between calls is the reference's.
     for (int it = it0; it < it0 + nsteps; ++it) {
-        float *t = *P;
+        T *t = *P;
         *P = *PP;
         *PP = t;
         if (hbc) {
@@ -488,26 +513,17 @@ int rtmhip_forward_steps(const rtmhip_shot *shot, int hbc, const float *V2DT2,
     return RTMHIP_OK;
 }
 
-int rtmhip_backward_steps(const rtmhip_shot *shot, int hbc, const float *V2DT2,
-                          const float *coefs, const float *taper, float **Ps, float **PPs,
-                          float **Pr, float **PPr, const float *upb, const float *rcv_chunk,
-                          float *img, int it_hi, int nsteps, int fused)
+template <typename T>
+int backward_loop(const rtmhip_shot *shot, const ShotCtx &c, int hbc, const float *V2DT2,
+                  const float *taper, T **Ps, T **PPs, T **Pr, T **PPr, const T *upb,
+                  const float *rcv_chunk, float *img, int it_hi, int nsteps, int fused)
 {
-    NEED_INIT();
-    ShotCtx c;
-    int rc;
-    if ((rc = prepare_shot(shot, coefs, &c))) return rc;
-    if (!V2DT2 || !Ps || !PPs || !Pr || !PPr || !*Ps || !*PPs || !*Pr || !*PPr || !rcv_chunk ||
-        !img || (hbc && (!taper || !upb)))
-        return fail(RTMHIP_ERR_INVALID, "NULL pointer");
-    if (nsteps <= 0) return RTMHIP_OK;
-    if ((rc = check_chunk(shot, it_hi - nsteps + 1, it_hi))) return rc;
     hipStream_t s = g_state.stream;
     const int nt = shot->nt, b = shot->blen;
     const int z0 = b, z1 = shot->nze - b;
     for (int it = it_hi; it > it_hi - nsteps; --it) {
         // Source wavefield at time it (RTMFiniteDifferencesKernel_HBC.cpp:283-309).
-        const float *S;
+        const T *S;
         if (it == nt - 1) {
             S = *PPs;  // snap1
         } else if (it == nt - 2) {
@@ -515,8 +531,9 @@ int rtmhip_backward_steps(const rtmhip_shot *shot, int hbc, const float *V2DT2,
         } else {
             HIPCHK(stencil(c.radius, *Ps, *PPs, V2DT2, c.cf, c.g, nullptr), "stencil");
             if (hbc)
-                HIPCHK(launch_upb(*PPs, (float *)upb, c.g, b, c.hf, it, true, s), "upb restore");
-            float *t = *Ps;
+                HIPCHK(launch_upb(*PPs, const_cast<T *>(upb), c.g, b, c.hf, it, true, s),
+                       "upb restore");
+            T *t = *Ps;
             *Ps = *PPs;
             *PPs = t;
             S = *Ps;
@@ -554,19 +571,87 @@ int rtmhip_backward_steps(const rtmhip_shot *shot, int hbc, const float *V2DT2,
                    "receivers+image");
         } else {
             HIPCHK(stencil(c.radius, *Pr, *PPr, V2DT2, c.cf, c.g, nullptr), "stencil");
-            HIPCHK(launch_receivers(*PPr, (float *)rcv_chunk, c.g, c.rx0, c.ry0, c.rz,
-                                    shot->rcv.dist_x, shot->rcv.dist_y, shot->rcv.count_x,
+            HIPCHK(launch_receivers(*PPr, const_cast<float *>(rcv_chunk), c.g, c.rx0, c.ry0,
+                                    c.rz, shot->rcv.dist_x, shot->rcv.dist_y, shot->rcv.count_x,
                                     shot->rcv.count_y, shot->ntstep, lt, true, s),
                    "receivers");
-            HIPCHK(launch_image(img, S, *PPr, c.g, z0, z1, s), "image");
+            HIPCHK(launch_image(img, S, (const T *)*PPr, c.g, z0, z1, s), "image");
         }
-        float *t = *Pr;
+        T *t = *Pr;
         *Pr = *PPr;
         *PPr = t;
     }
     // pending in-place taper of the older receiver field (fused HBC path)
     if (hbc && fused) HIPCHK(launch_taper(*PPr, taper, c.g, b, true, s), "taper");
     return RTMHIP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int rtmhip_model_steps(const rtmhip_shot *shot, const float *V2DT2, const float *coefs,
+                       const float *taper, const float *ricker, float **P, float **PP,
+                       float *rcv_chunk, int it0, int nsteps)
+{
+    NEED_INIT();
+    ShotCtx c;
+    int rc;
+    if ((rc = prepare_shot(shot, coefs, &c))) return rc;
+    if ((rc = check_format(shot))) return rc;
+    if (!V2DT2 || !taper || !ricker || !P || !PP || !*P || !*PP || !rcv_chunk)
+        return fail(RTMHIP_ERR_INVALID, "NULL pointer");
+    if (nsteps <= 0) return RTMHIP_OK;
+    if ((rc = check_chunk(shot, it0, it0 + nsteps - 1))) return rc;
+    return with_format(shot, [&](auto t) {
+        using T = decltype(t);
+        return model_loop<T>(shot, c, V2DT2, taper, ricker, as<T>(P), as<T>(PP), rcv_chunk, it0,
+                             nsteps);
+    });
+}
+
+int rtmhip_forward_steps(const rtmhip_shot *shot, int hbc, const float *V2DT2,
+                         const float *coefs, const float *taper, const float *ricker, float **P,
+                         float **PP, float *upb, int it0, int nsteps)
+{
+    NEED_INIT();
+    ShotCtx c;
+    int rc;
+    if ((rc = prepare_shot(shot, coefs, &c))) return rc;
+    if ((rc = check_format(shot))) return rc;
+    if (!V2DT2 || !ricker || !P || !PP || !*P || !*PP || (hbc && (!taper || !upb)))
+        return fail(RTMHIP_ERR_INVALID, "NULL pointer");
+    if (nsteps <= 0) return RTMHIP_OK;
+    if (it0 < 0 || it0 + nsteps > shot->nt)
+        return fail(RTMHIP_ERR_INVALID, "time range outside [0,%d)", shot->nt);
+    return with_format(shot, [&](auto t) {
+        using T = decltype(t);
+        return forward_loop<T>(shot, c, hbc, V2DT2, taper, ricker, as<T>(P), as<T>(PP),
+                               reinterpret_cast<T *>(upb), it0, nsteps);
+    });
+}
+
+int rtmhip_backward_steps(const rtmhip_shot *shot, int hbc, const float *V2DT2,
+                          const float *coefs, const float *taper, float **Ps, float **PPs,
+                          float **Pr, float **PPr, const float *upb, const float *rcv_chunk,
+                          float *img, int it_hi, int nsteps, int fused)
+{
+    NEED_INIT();
+    ShotCtx c;
+    int rc;
+    if ((rc = prepare_shot(shot, coefs, &c))) return rc;
+    if ((rc = check_format(shot))) return rc;
+    if (!V2DT2 || !Ps || !PPs || !Pr || !PPr || !*Ps || !*PPs || !*Pr || !*PPr || !rcv_chunk ||
+        !img || (hbc && (!taper || !upb)))
+        return fail(RTMHIP_ERR_INVALID, "NULL pointer");
+    if (nsteps <= 0) return RTMHIP_OK;
+    if ((rc = check_chunk(shot, it_hi - nsteps + 1, it_hi))) return rc;
+    return with_format(shot, [&](auto t) {
+        using T = decltype(t);
+        return backward_loop<T>(shot, c, hbc, V2DT2, taper, as<T>(Ps), as<T>(PPs), as<T>(Pr),
+                                as<T>(PPr), reinterpret_cast<const T *>(upb), rcv_chunk, img,
+                                it_hi, nsteps, fused);
+    });
 }
 
 int rtmhip_set_tiling(int kernel_class, int tz, int by, int ry, int xchunk)

@@ -5,8 +5,35 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <type_traits>
 
 namespace rtmhip {
+
+// Wavefield storage formats.  Arithmetic is always FP32: a wavefield value
+// is widened on load and rounded (nearest-even) on store; "rnd" is the
+// round trip, used where the reference's operator sequence would have
+// stored the value before reading it back.
+enum WaveFormat { WAVE_F32 = 0, WAVE_F16 = 1, WAVE_BF16 = 2 };
+
+__device__ __forceinline__ float wld(float v) { return v; }
+__device__ __forceinline__ float wld(_Float16 v) { return (float)v; }
+__device__ __forceinline__ float wld(__bf16 v) { return (float)v; }
+template <typename T>
+__device__ __forceinline__ T wst(float v)
+{
+    // The FP32 value is made opaque first: otherwise the compiler fuses the
+    // multiply or add that produced it with the narrowing into one
+    // v_fma_mixlo_f16, which rounds the exact result once to FP16 instead of
+    // rounding to FP32 and then to FP16 as the operator sequence does.
+    if constexpr (!std::is_same<T, float>::value) asm("" : "+v"(v));
+    return (T)v;
+}
+template <typename T>
+__device__ __forceinline__ float wrnd(float v)
+{
+    if constexpr (std::is_same<T, float>::value) return v;
+    else return wld(wst<T>(v));
+}
 
 // Stencil plane [dim][tap] (rtm-3d/include/RTMGrid.hpp:400-446), passed by
 // value so every tap lives in a scalar register.
@@ -30,7 +57,7 @@ inline Grid dense_grid(int nx, int ny, int nz)
 // at receiver cells, which the receiver kernel handles after injection.
 struct ImageFuse {
     float *img;
-    const float *ps;
+    const void *ps;  // source wavefield, of the step's storage type
     int z0, z1;
     int rz;             // receiver plane (extended z)
     int rx0, rdx, rnx;  // receiver x = rx0 + i*rdx, i < rnx (extended)
@@ -55,32 +82,42 @@ struct StencilTiling {
 // per thread); the default is 64x4x4.
 bool stencil_tiling_supported(int tz, int by, int ry);
 
-// Launchers: enqueue on `s`, return the launch status.
-hipError_t launch_stencil(int radius, const float *P, float *PP, const float *V, const Coefs &cf,
+// Launchers: enqueue on `s`, return the launch status.  T is the wavefield
+// storage type (float, _Float16 or __bf16); V, tapers, traces and the image
+// are always float.  Instantiated for the three storage types.
+template <typename T>
+hipError_t launch_stencil(int radius, const T *P, T *PP, const float *V, const Coefs &cf,
                           const Grid &g, const ImageFuse *fuse, const TaperFuse *taper,
                           int *flag, hipStream_t s, const StencilTiling *tiling);
-hipError_t launch_taper(float *P, const float *taper, const Grid &g, int blen, bool upper_only,
+template <typename T>
+hipError_t launch_taper(T *P, const float *taper, const Grid &g, int blen, bool upper_only,
                         hipStream_t s);
 // Upper taper (applied `taps` times) of the slab cells the stencil never
 // updates (z < R or the x / y frame of width R), in place.
-hipError_t launch_taper_frame(float *P, const float *taper, const Grid &g, int b, int R, int taps,
+template <typename T>
+hipError_t launch_taper_frame(T *P, const float *taper, const Grid &g, int b, int R, int taps,
                               hipStream_t s);
-hipError_t launch_source(float *PP, long long offset, const float *table, int it, float value,
+template <typename T>
+hipError_t launch_source(T *PP, long long offset, const float *table, int it, float value,
                          hipStream_t s);
-hipError_t launch_receivers(float *PP, float *rcv, const Grid &g, int rx0, int ry0, int rz,
-                            int rdx, int rdy, int rnx, int rny, int ntstep, int lt, bool restore,
+template <typename T>
+hipError_t launch_receivers(T *PP, float *rcv, const Grid &g, int rx0, int ry0, int rz, int rdx,
+                            int rdy, int rnx, int rny, int ntstep, int lt, bool restore,
                             hipStream_t s);
-hipError_t launch_receivers_image(float *PP, const float *rcv, const Grid &g, int rx0, int ry0,
+template <typename T>
+hipError_t launch_receivers_image(T *PP, const float *rcv, const Grid &g, int rx0, int ry0,
                                   int rz, int rdx, int rdy, int rnx, int rny, int ntstep, int lt,
-                                  float *img, const float *ps, int z0, int z1, int R,
-                                  hipStream_t s);
-hipError_t launch_upb(float *PP, float *upb, const Grid &g, int blen, int hf, long long lt,
-                      bool restore, hipStream_t s);
-hipError_t launch_image(float *img, const float *PS, const float *PR, const Grid &g, int z0,
-                        int z1, hipStream_t s);
+                                  float *img, const T *ps, int z0, int z1, int R, hipStream_t s);
+template <typename T>
+hipError_t launch_upb(T *PP, T *upb, const Grid &g, int blen, int hf, long long lt, bool restore,
+                      hipStream_t s);
+template <typename T>
+hipError_t launch_image(float *img, const T *PS, const T *PR, const Grid &g, int z0, int z1,
+                        hipStream_t s);
 hipError_t launch_image_finish(const float *img, float *out, const Grid &g, int blen,
                                const Coefs &fc, hipStream_t s);
 hipError_t launch_fill(float *p, size_t n, float v, hipStream_t s);
+hipError_t launch_fill_u16(uint16_t *p, size_t n, uint16_t bits, hipStream_t s);
 hipError_t launch_stack(float *dst, const float *src, size_t n, hipStream_t s);
 
 }  // namespace rtmhip

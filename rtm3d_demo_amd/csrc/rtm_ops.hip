@@ -2,7 +2,10 @@
 // for gfx950.  Each restates one operator of the reference CPU platform
 // (rtm-3d/src/platform/host/RTMCPUPlatform.cpp) with the same per-cell
 // arithmetic; they are bandwidth-trivial next to the stencil and are kept
-// simple and coalesced (z on consecutive lanes).
+// simple and coalesced (z on consecutive lanes).  Wavefields have storage
+// type T (float, _Float16, __bf16): values are widened on load, computed in
+// float and rounded once per stored value, as the reference's in-place
+// operators would with grids of that type.
 #include "rtm_internal.hpp"
 
 #pragma clang fp contract(off)
@@ -28,8 +31,9 @@ __device__ __forceinline__ float gfac(const float *__restrict__ t, int i, int n,
 //   x, y interior           : v*gz
 //   x in a border (any y)   : ((v*gy if y in a border)*gx)*gz
 //   y in a border, x inner  : (v*gy)*gz
+template <typename T>
 __global__ void __launch_bounds__(kThreads)
-    k_taper_upper(float *__restrict__ P, const float *__restrict__ t, Grid g, int b)
+    k_taper_upper(T *__restrict__ P, const float *__restrict__ t, Grid g, int b)
 {
     const long long n = (long long)g.nx * g.ny * b;
     const long long i = (long long)blockIdx.x * kThreads + threadIdx.x;
@@ -41,7 +45,7 @@ __global__ void __launch_bounds__(kThreads)
     const bool bx = x < b || x >= g.nx - b;
     const bool by = y < b || y >= g.ny - b;
     const long long o = goff(g, x, y, z);
-    float v = P[o];
+    float v = wld(P[o]);
     const float gz = t[b - 1 - z];
     if (!bx && !by) {
         v *= gz;
@@ -53,17 +57,18 @@ __global__ void __launch_bounds__(kThreads)
         v *= gfac(t, y, g.ny, b);
         v *= gz;
     }
-    P[o] = v;
+    P[o] = wst<T>(v);
 }
 
 // In-place upper taper, applied `taps` times, of the slab cells (z < b) that
 // the stencil does not update: z < R, or x / y within R of the grid edge.
 // The fused HBC step tapers every other slab value as it is loaded; these
 // cells of PP are not read during the step, so tapering them in place here
-// keeps PP's memory equal to the reference's after its in-place tapers.
+// keeps PP's memory equal to the reference's after its in-place tapers
+// (each application is one in-place pass: rounded to T before the next).
+template <typename T>
 __global__ void __launch_bounds__(kThreads)
-    k_taper_frame(float *__restrict__ P, const float *__restrict__ t, Grid g, int b, int R,
-                  int taps)
+    k_taper_frame(T *__restrict__ P, const float *__restrict__ t, Grid g, int b, int R, int taps)
 {
     const long long nA = (long long)g.nx * g.ny * R;  // z < R, all x, y
     const int lz = b - R;                             // z in [R, b)
@@ -97,21 +102,24 @@ __global__ void __launch_bounds__(kThreads)
     const float fx = bx ? gfac(t, x, g.nx, b) : 1.0f;
     const float fz = t[b - 1 - z];
     const long long o = goff(g, x, y, z);
-    float v = P[o];
+    float v = wld(P[o]);
     for (int k = 0; k < taps; ++k) {
+        if (k) v = wrnd<T>(v);
         v *= fy;
         v *= fx;
         v *= fz;
     }
-    P[o] = v;
+    P[o] = wst<T>(v);
 }
 
 // The front-right edge of taper3D (RTMCPUPlatform.cpp:900-904) writes cells
 // x < blen, y >= ny-blen, z interior from the face-tapered cell
 // (blen, y, z).  This kernel runs before k_taper_all so it reads that cell
-// before the face pass rescales it:  ((P(blen,y,z)*gy)*gx)*gy.
+// before the face pass rescales it:  ((P(blen,y,z)*gy)*gx)*gy, where the
+// face pass's product P*gy is the value it stored (rounded to T).
+template <typename T>
 __global__ void __launch_bounds__(kThreads)
-    k_taper_front_right(float *__restrict__ P, const float *__restrict__ t, Grid g, int b)
+    k_taper_front_right(T *__restrict__ P, const float *__restrict__ t, Grid g, int b)
 {
     const int lz = g.nz - 2 * b;
     const long long n = (long long)b * b * lz;
@@ -123,11 +131,12 @@ __global__ void __launch_bounds__(kThreads)
     const int x = (int)(r / b);
     const int y = g.ny - b + iy;
     const float gy = t[iy];
-    float v = P[goff(g, b, y, z)];
+    float v = wld(P[goff(g, b, y, z)]);
     v *= gy;
+    v = wrnd<T>(v);
     v *= t[b - 1 - x];
     v *= gy;
-    P[goff(g, x, y, z)] = v;
+    P[goff(g, x, y, z)] = wst<T>(v);
 }
 
 // taper3D (RTMCPUPlatform.cpp:710-920) for every border cell except the
@@ -136,8 +145,9 @@ __global__ void __launch_bounds__(kThreads)
 //   B: y border, z inner        nx*2b*lz
 //   C: x border, y and z inner  2b*ly*lz
 // Per-cell factor order follows the reference loop nests.
+template <typename T>
 __global__ void __launch_bounds__(kThreads)
-    k_taper_all(float *__restrict__ P, const float *__restrict__ t, Grid g, int b)
+    k_taper_all(T *__restrict__ P, const float *__restrict__ t, Grid g, int b)
 {
     const int lz = g.nz - 2 * b, ly = g.ny - 2 * b;
     const long long nA = (long long)g.nx * g.ny * 2 * b;
@@ -174,7 +184,7 @@ __global__ void __launch_bounds__(kThreads)
     const bool bz = z < b || z >= g.nz - b;
     if (bx && by && !bz && x < b && y >= g.ny - b) return;  // front-right edge
     const long long o = goff(g, x, y, z);
-    float v = P[o];
+    float v = wld(P[o]);
     if (bx && bz) {
         if (by) v *= gfac(t, y, g.ny, b);
         v *= gfac(t, x, g.nx, b);
@@ -192,22 +202,24 @@ __global__ void __launch_bounds__(kThreads)
     } else {
         v *= gfac(t, x, g.nx, b);
     }
-    P[o] = v;
+    P[o] = wst<T>(v);
 }
 
 // rtmApplySource (RTMCPUPlatform.cpp:103-113): PP[s] += value (or table[it]).
-__global__ void k_source(float *__restrict__ PP, long long o, const float *__restrict__ table,
-                         int it, float value)
+template <typename T>
+__global__ void k_source(T *__restrict__ PP, long long o, const float *__restrict__ table, int it,
+                         float value)
 {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
         const float v = table ? table[it] : value;
-        PP[o] = PP[o] + v;
+        PP[o] = wst<T>(wld(PP[o]) + v);
     }
 }
 
 // rtmSaveReceiverData / rtmRestoreReceiverData (RTMCPUPlatform.cpp:114-151).
+template <typename T>
 __global__ void __launch_bounds__(kThreads)
-    k_receivers(float *__restrict__ PP, float *__restrict__ rcv, Grid g, int rx0, int ry0, int rz,
+    k_receivers(T *__restrict__ PP, float *__restrict__ rcv, Grid g, int rx0, int ry0, int rz,
                 int rdx, int rdy, int rnx, int rny, int ntstep, int lt, int restore)
 {
     const int i = blockIdx.x * kThreads + threadIdx.x;
@@ -217,8 +229,8 @@ __global__ void __launch_bounds__(kThreads)
     if (rx < 0 || rx >= g.nx || ry < 0 || ry >= g.ny) return;
     const long long o = goff(g, rx, ry, rz);
     const long long ro = (long long)i * ntstep + lt;
-    if (restore) PP[o] = rcv[ro];
-    else rcv[ro] = PP[o];
+    if (restore) PP[o] = wst<T>(rcv[ro]);
+    else rcv[ro] = wld(PP[o]);
 }
 
 // Epilogue of the fused backward receiver step.  Threads [0, rnx*rny):
@@ -234,11 +246,11 @@ __device__ __forceinline__ bool is_receiver(int x, int y, int z, int rx0, int ry
            (dx / rdx) < rnx && (dy / rdy) < rny;
 }
 
+template <typename T>
 __global__ void __launch_bounds__(kThreads)
-    k_receivers_image(float *__restrict__ PP, const float *__restrict__ rcv, Grid g, int rx0,
-                      int ry0, int rz, int rdx, int rdy, int rnx, int rny, int ntstep, int lt,
-                      float *__restrict__ img, const float *__restrict__ ps, int z0, int z1,
-                      int R)
+    k_receivers_image(T *__restrict__ PP, const float *__restrict__ rcv, Grid g, int rx0, int ry0,
+                      int rz, int rdx, int rdy, int rnx, int rny, int ntstep, int lt,
+                      float *__restrict__ img, const T *__restrict__ ps, int z0, int z1, int R)
 {
     long long i = (long long)blockIdx.x * kThreads + threadIdx.x;
     const long long nrcv = (long long)rnx * rny;
@@ -248,8 +260,8 @@ __global__ void __launch_bounds__(kThreads)
         if (rx < 0 || rx >= g.nx || ry < 0 || ry >= g.ny) return;
         const long long o = goff(g, rx, ry, rz);
         const float v = rcv[i * ntstep + lt];
-        PP[o] = v;
-        if (rz >= z0 && rz < z1) img[o] = img[o] + ps[o] * v;
+        PP[o] = wst<T>(v);
+        if (rz >= z0 && rz < z1) img[o] = img[o] + wld(ps[o]) * wrnd<T>(v);
         return;
     }
     i -= nrcv;
@@ -275,14 +287,14 @@ __global__ void __launch_bounds__(kThreads)
     }
     if (is_receiver(x, y, z, rx0, ry0, rz, rdx, rdy, rnx, rny)) return;
     const long long o = goff(g, x, y, z);
-    img[o] = img[o] + ps[o] * PP[o];
+    img[o] = img[o] + wld(ps[o]) * wld(PP[o]);
 }
 
 // rtmSaveUpperBorder / rtmRestoreUpperBorder (RTMCPUPlatform.cpp:153-189):
 // upb[lt][x][y][k] <-> PP[x][y][blen-hf+k], 64-bit offsets.
+template <typename T>
 __global__ void __launch_bounds__(kThreads)
-    k_upb(float *__restrict__ PP, float *__restrict__ upb, Grid g, int b, int hf, long long lt,
-          int restore)
+    k_upb(T *__restrict__ PP, T *__restrict__ upb, Grid g, int b, int hf, long long lt, int restore)
 {
     const long long n = (long long)g.nx * g.ny * hf;
     const long long i = (long long)blockIdx.x * kThreads + threadIdx.x;
@@ -298,9 +310,10 @@ __global__ void __launch_bounds__(kThreads)
 }
 
 // rtmImageCondition (RTMCPUPlatform.cpp:513-540): z in [z0, z1), all x, y.
+template <typename T>
 __global__ void __launch_bounds__(kThreads)
-    k_image(float *__restrict__ img, const float *__restrict__ PS, const float *__restrict__ PR,
-            Grid g, int z0, int z1)
+    k_image(float *__restrict__ img, const T *__restrict__ PS, const T *__restrict__ PR, Grid g,
+            int z0, int z1)
 {
     const int lz = z1 - z0;
     const long long n = (long long)g.nx * g.ny * lz;
@@ -311,7 +324,7 @@ __global__ void __launch_bounds__(kThreads)
     const int y = (int)(r % g.ny);
     const int x = (int)(r / g.ny);
     const long long o = goff(g, x, y, z);
-    img[o] = img[o] + PS[o] * PR[o];
+    img[o] = img[o] + wld(PS[o]) * wld(PR[o]);
 }
 
 // Order-2 Laplacian filter (RTMGrid.cpp:226-300) evaluated only where
@@ -347,6 +360,14 @@ __global__ void __launch_bounds__(kThreads) k_fill(float *__restrict__ p, size_t
 }
 
 __global__ void __launch_bounds__(kThreads)
+    k_fill_u16(uint16_t *__restrict__ p, size_t n, uint16_t v)
+{
+    for (size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * kThreads)
+        p[i] = v;
+}
+
+__global__ void __launch_bounds__(kThreads)
     k_stack(float *__restrict__ d, const float *__restrict__ s, size_t n)
 {
     for (size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x; i < n;
@@ -367,87 +388,94 @@ inline unsigned stride_blocks(size_t n)
 
 }  // namespace
 
-hipError_t launch_taper(float *P, const float *taper, const Grid &g, int b, bool upper_only,
+template <typename T>
+hipError_t launch_taper(T *P, const float *taper, const Grid &g, int b, bool upper_only,
                         hipStream_t s)
 {
     if (b <= 0) return hipSuccess;
     if (upper_only) {
         const long long n = (long long)g.nx * g.ny * b;
-        hipLaunchKernelGGL(k_taper_upper, dim3(nblocks(n)), dim3(kThreads), 0, s, P, taper, g, b);
+        hipLaunchKernelGGL(k_taper_upper<T>, dim3(nblocks(n)), dim3(kThreads), 0, s, P, taper, g, b);
         return hipGetLastError();
     }
     const long long lz = g.nz - 2 * b, ly = g.ny - 2 * b;
     const long long nfr = (long long)b * b * lz;
     if (nfr > 0 && g.nx > 2 * b) {
-        hipLaunchKernelGGL(k_taper_front_right, dim3(nblocks(nfr)), dim3(kThreads), 0, s, P,
+        hipLaunchKernelGGL(k_taper_front_right<T>, dim3(nblocks(nfr)), dim3(kThreads), 0, s, P,
                            taper, g, b);
     }
     const long long n = (long long)g.nx * g.ny * 2 * b + (long long)g.nx * 2 * b * lz +
                         (long long)2 * b * ly * lz;
-    hipLaunchKernelGGL(k_taper_all, dim3(nblocks(n)), dim3(kThreads), 0, s, P, taper, g, b);
+    hipLaunchKernelGGL(k_taper_all<T>, dim3(nblocks(n)), dim3(kThreads), 0, s, P, taper, g, b);
     return hipGetLastError();
 }
 
-hipError_t launch_taper_frame(float *P, const float *taper, const Grid &g, int b, int R, int taps,
+template <typename T>
+hipError_t launch_taper_frame(T *P, const float *taper, const Grid &g, int b, int R, int taps,
                               hipStream_t s)
 {
     if (b <= 0 || taps <= 0) return hipSuccess;
     const long long lz = b - R;
     const long long n = (long long)g.nx * g.ny * R + (long long)2 * R * g.ny * lz +
                         (long long)(g.nx - 2 * R) * 2 * R * lz;
-    hipLaunchKernelGGL(k_taper_frame, dim3(nblocks(n)), dim3(kThreads), 0, s, P, taper, g, b, R,
+    hipLaunchKernelGGL(k_taper_frame<T>, dim3(nblocks(n)), dim3(kThreads), 0, s, P, taper, g, b, R,
                        taps);
     return hipGetLastError();
 }
 
-hipError_t launch_source(float *PP, long long offset, const float *table, int it, float value,
+template <typename T>
+hipError_t launch_source(T *PP, long long offset, const float *table, int it, float value,
                          hipStream_t s)
 {
-    hipLaunchKernelGGL(k_source, dim3(1), dim3(64), 0, s, PP, offset, table, it, value);
+    hipLaunchKernelGGL(k_source<T>, dim3(1), dim3(64), 0, s, PP, offset, table, it, value);
     return hipGetLastError();
 }
 
-hipError_t launch_receivers(float *PP, float *rcv, const Grid &g, int rx0, int ry0, int rz,
+template <typename T>
+hipError_t launch_receivers(T *PP, float *rcv, const Grid &g, int rx0, int ry0, int rz,
                             int rdx, int rdy, int rnx, int rny, int ntstep, int lt, bool restore,
                             hipStream_t s)
 {
     const long long n = (long long)rnx * rny;
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_receivers, dim3(nblocks(n)), dim3(kThreads), 0, s, PP, rcv, g, rx0, ry0,
+    hipLaunchKernelGGL(k_receivers<T>, dim3(nblocks(n)), dim3(kThreads), 0, s, PP, rcv, g, rx0, ry0,
                        rz, rdx, rdy, rnx, rny, ntstep, lt, restore ? 1 : 0);
     return hipGetLastError();
 }
 
-hipError_t launch_receivers_image(float *PP, const float *rcv, const Grid &g, int rx0, int ry0,
+template <typename T>
+hipError_t launch_receivers_image(T *PP, const float *rcv, const Grid &g, int rx0, int ry0,
                                   int rz, int rdx, int rdy, int rnx, int rny, int ntstep, int lt,
-                                  float *img, const float *ps, int z0, int z1, int R,
+                                  float *img, const T *ps, int z0, int z1, int R,
                                   hipStream_t s)
 {
     const long long lz = z1 > z0 ? z1 - z0 : 0;
     const long long n = (long long)rnx * rny + (long long)2 * R * g.ny * lz +
                         (long long)(g.nx - 2 * R) * 2 * R * lz;
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_receivers_image, dim3(nblocks(n)), dim3(kThreads), 0, s, PP, rcv, g, rx0,
+    hipLaunchKernelGGL(k_receivers_image<T>, dim3(nblocks(n)), dim3(kThreads), 0, s, PP, rcv, g, rx0,
                        ry0, rz, rdx < 1 ? 1 : rdx, rdy < 1 ? 1 : rdy, rnx, rny, ntstep, lt, img,
                        ps, z0, z1, R);
     return hipGetLastError();
 }
 
-hipError_t launch_upb(float *PP, float *upb, const Grid &g, int b, int hf, long long lt,
+template <typename T>
+hipError_t launch_upb(T *PP, T *upb, const Grid &g, int b, int hf, long long lt,
                       bool restore, hipStream_t s)
 {
     const long long n = (long long)g.nx * g.ny * hf;
-    hipLaunchKernelGGL(k_upb, dim3(nblocks(n)), dim3(kThreads), 0, s, PP, upb, g, b, hf, lt,
+    hipLaunchKernelGGL(k_upb<T>, dim3(nblocks(n)), dim3(kThreads), 0, s, PP, upb, g, b, hf, lt,
                        restore ? 1 : 0);
     return hipGetLastError();
 }
 
-hipError_t launch_image(float *img, const float *PS, const float *PR, const Grid &g, int z0,
+template <typename T>
+hipError_t launch_image(float *img, const T *PS, const T *PR, const Grid &g, int z0,
                         int z1, hipStream_t s)
 {
     if (z1 <= z0) return hipSuccess;
     const long long n = (long long)g.nx * g.ny * (z1 - z0);
-    hipLaunchKernelGGL(k_image, dim3(nblocks(n)), dim3(kThreads), 0, s, img, PS, PR, g, z0, z1);
+    hipLaunchKernelGGL(k_image<T>, dim3(nblocks(n)), dim3(kThreads), 0, s, img, PS, PR, g, z0, z1);
     return hipGetLastError();
 }
 
@@ -467,11 +495,37 @@ hipError_t launch_fill(float *p, size_t n, float v, hipStream_t s)
     return hipGetLastError();
 }
 
+hipError_t launch_fill_u16(uint16_t *p, size_t n, uint16_t bits, hipStream_t s)
+{
+    if (!n) return hipSuccess;
+    hipLaunchKernelGGL(k_fill_u16, dim3(stride_blocks(n)), dim3(kThreads), 0, s, p, n, bits);
+    return hipGetLastError();
+}
+
 hipError_t launch_stack(float *dst, const float *src, size_t n, hipStream_t s)
 {
     if (!n) return hipSuccess;
     hipLaunchKernelGGL(k_stack, dim3(stride_blocks(n)), dim3(kThreads), 0, s, dst, src, n);
     return hipGetLastError();
 }
+
+#define RTM_INSTANTIATE_OPS(T)                                                                \
+    template hipError_t launch_taper<T>(T *, const float *, const Grid &, int, bool,            \
+                                        hipStream_t);                                           \
+    template hipError_t launch_taper_frame<T>(T *, const float *, const Grid &, int, int, int,  \
+                                              hipStream_t);                                     \
+    template hipError_t launch_source<T>(T *, long long, const float *, int, float, hipStream_t); \
+    template hipError_t launch_receivers<T>(T *, float *, const Grid &, int, int, int, int, int, \
+                                            int, int, int, int, bool, hipStream_t);             \
+    template hipError_t launch_receivers_image<T>(T *, const float *, const Grid &, int, int,   \
+                                                  int, int, int, int, int, int, int, float *,   \
+                                                  const T *, int, int, int, hipStream_t);       \
+    template hipError_t launch_upb<T>(T *, T *, const Grid &, int, int, long long, bool,        \
+                                      hipStream_t);                                             \
+    template hipError_t launch_image<T>(float *, const T *, const T *, const Grid &, int, int,  \
+                                        hipStream_t);
+RTM_INSTANTIATE_OPS(float)
+RTM_INSTANTIATE_OPS(_Float16)
+RTM_INSTANTIATE_OPS(__bf16)
 
 }  // namespace rtmhip

@@ -74,17 +74,46 @@ __device__ __forceinline__ float taper1(float v, float fx, float fy, float fz)
     return v;
 }
 
+// PP, V and the imaging operands are touched once per step; marking them
+// non-temporal keeps L2 capacity for P, whose rows are re-read as halos by
+// neighbouring tiles a few planes after their owner loaded them.
+// (Measured on MI355X: non-temporal hints made the step slower, so they are
+// off unless RTM_STREAM_NT=1 is defined at build time.)
+#ifndef RTM_STREAM_NT
+#define RTM_STREAM_NT 0
+#endif
+template <typename T>
+__device__ __forceinline__ float stream_load(const T *p)
+{
+#if RTM_STREAM_NT
+    return wld(__builtin_nontemporal_load(p));
+#else
+    return wld(*p);
+#endif
+}
+
+template <typename T>
+__device__ __forceinline__ void stream_store(T *p, float v)
+{
+#if RTM_STREAM_NT
+    __builtin_nontemporal_store(wst<T>(v), p);
+#else
+    *p = wst<T>(v);
+#endif
+}
+
 __device__ __forceinline__ float lat_factor(const float *__restrict__ t, int i, int n, int b)
 {
     return i < b ? t[b - 1 - i] : (i >= n - b ? t[i - (n - b)] : 1.0f);
 }
 
-template <int R, int TZ, int BY, int RY, bool IMG, bool TAPER>
+template <typename T, int R, int TZ, int BY, int RY, bool IMG, bool TAPER>
 __global__ void __launch_bounds__(TZ *BY)
-    k_stencil(const float *__restrict__ P, float *__restrict__ PP, const float *__restrict__ V,
+    k_stencil(const T *__restrict__ P, T *__restrict__ PP, const float *__restrict__ V,
               const Coefs cf, const Grid g, const TileGrid tg, const ImageFuse fz,
               const TaperFuse tf, int *__restrict__ flag)
 {
+    const T *__restrict__ PS = static_cast<const T *>(fz.ps);
     constexpr int TY = BY * RY;
     constexpr int LW = TZ + 2 * R;
     constexpr int LH = TY + 2 * R;
@@ -176,39 +205,41 @@ __global__ void __launch_bounds__(TZ *BY)
         return (TAPER && btap) ? lat_factor(tf.t, x, g.nx, tf.b) : 1.0f;
     };
     // Loads of P at plane x for the ring slot `slot` (own rows).
+    // A taper applied in registers is one in-place pass of the reference:
+    // its result is rounded to the storage type (exact for float).
     auto load_ring = [&](int x, float(&dst)[RY]) {
-        const float *__restrict__ Px = P + (size_t)x * (size_t)g.sx;
+        const T *__restrict__ Px = P + (size_t)x * (size_t)g.sx;
         const float fx = plane_fx(x);
 #pragma unroll
         for (int r = 0; r < RY; ++r) {
-            float v = Px[roff[r]];
-            if (TAPER && btap) v = taper1(v, z < tf.b ? fx : 1.0f, fyr[r], fzl);
+            float v = wld(Px[roff[r]]);
+            if (TAPER && btap) v = wrnd<T>(taper1(v, z < tf.b ? fx : 1.0f, fyr[r], fzl));
             dst[r] = v;
         }
     };
     auto fetch = [&](int x, int set) {
         const size_t pl = (size_t)x * (size_t)g.sx;
-        const float *__restrict__ Px = P + pl;
+        const T *__restrict__ Px = P + pl;
         const float fx = plane_fx(x);
 #pragma unroll
         for (int r = 0; r < RY; ++r) {
-            n_pp[set][r] = PP[pl + roff[r]];
-            n_v[set][r] = V[pl + roff[r]];
+            n_pp[set][r] = stream_load(&PP[pl + roff[r]]);
+            n_v[set][r] = stream_load(&V[pl + roff[r]]);
             if (IMG) {
-                n_ps[set][r] = fz.ps[pl + roff[r]];
-                n_img[set][r] = fz.img[pl + roff[r]];
+                n_ps[set][r] = stream_load(&PS[pl + roff[r]]);
+                n_img[set][r] = stream_load(&fz.img[pl + roff[r]]);
             }
         }
 #pragma unroll
         for (int i = 0; i < NYH; ++i) {
-            float v = Px[yh_off[i]];
-            if (TAPER && btap) v = taper1(v, z < tf.b ? fx : 1.0f, yh_fy[i], fzl);
+            float v = wld(Px[yh_off[i]]);
+            if (TAPER && btap) v = wrnd<T>(taper1(v, z < tf.b ? fx : 1.0f, yh_fy[i], fzl));
             n_yh[set][i] = v;
         }
 #pragma unroll
         for (int k = 0; k < NZH; ++k) {
-            float v = Px[zh_off[k]];
-            if (TAPER && btap) v = taper1(v, zh_tap[k] ? fx : 1.0f, zh_fy[k], zh_fz[k]);
+            float v = wld(Px[zh_off[k]]);
+            if (TAPER && btap) v = wrnd<T>(taper1(v, zh_tap[k] ? fx : 1.0f, zh_fy[k], zh_fz[k]));
             n_zh[set][k] = v;
         }
     };
@@ -267,15 +298,309 @@ __global__ void __launch_bounds__(TZ *BY)
             const float lapl = ax + ay + az;
             float pp = n_pp[cur][r];
             if (TAPER && btap && z < tf.b) {
-                pp = taper1(pp, fxc, fyr[r], fzl);
-                if (tf.pp_taps > 1) pp = taper1(pp, fxc, fyr[r], fzl);
+                pp = wrnd<T>(taper1(pp, fxc, fyr[r], fzl));
+                if (tf.pp_taps > 1) pp = wrnd<T>(taper1(pp, fxc, fyr[r], fzl));
             }
             const float npp = 2.f * p - pp + n_v[cur][r] * lapl;
             if (ycomp[r]) {
-                PP[pl + roff[r]] = npp;
+                stream_store(&PP[pl + roff[r]], npp);
                 bad |= (npp != npp);
+                // the image reads the stored (rounded) receiver value
                 if (IMG && imgz && !(rcvx && rcvrow[r]))
-                    fz.img[pl + roff[r]] = n_img[cur][r] + n_ps[cur][r] * npp;
+                    stream_store(&fz.img[pl + roff[r]],
+                                 n_img[cur][r] + n_ps[cur][r] * wrnd<T>(npp));
+            }
+        }
+        ++x;
+        return true;
+    };
+    while (x < xe) {
+        [&]<int... Ks>(std::integer_sequence<int, Ks...>) {
+            (step(std::integral_constant<int, Ks>{}) && ...);
+        }(std::make_integer_sequence<int, NR>{});
+    }
+    if (bad) atomicOr(flag, 1);
+}
+
+// ---------------------------------------------------------------------------
+// Full-row variant.  A block covers every z of its rows: 64*NW lanes per row
+// (NW = ceil(nz/64) <= NWM waves), so no z halo is loaded and each row is
+// read as one contiguous segment (a 64-wide z tile wastes most of the two
+// cache lines its 4-cell z halos touch).  BY thread rows x RY rows per
+// thread; the y halo rows are shared through LDS as in the tiled kernel.
+// Every wave lies in one thread row, so y and everything derived from it
+// (row offsets, halo rows, lateral taper factors, receiver rows) is
+// wave-uniform and lives in scalar registers: a global access is a scalar
+// row base plus the lane's 32-bit z byte offset, and an LDS access is one
+// per-lane base plus a compile-time offset (the LDS row pitch is fixed by
+// NWM).  The imaging operands (IMG) are loaded at the start of their own
+// plane step rather than a plane ahead, to bound the register count.
+struct RowGrid {
+    int nty, nxc, xchunk;
+};
+
+__device__ __forceinline__ void row_block_coords(const RowGrid &rg, int &by, int &bx)
+{
+    const int n = rg.nty * rg.nxc;
+    const int h = blockIdx.x;
+    const int q = n >> 3, r = n & 7, k = h & 7, j = h >> 3;
+    const int t = k * q + min(k, r) + j;
+    by = t % rg.nty;
+    bx = t / rg.nty;
+}
+
+// Raw buffer view of one plane: the 128-bit descriptor (base address, no
+// stride, range check off, 32-bit data format) lives in scalar registers, so
+// an access is descriptor + scalar row byte offset + the lane's 32-bit z byte
+// offset -- one VGPR of address per lane for every row and every array.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t plane_rsrc(const void *p)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, -1, 0x00020000);
+}
+// Loads and stores of one element of type T, widened to / rounded from
+// float.  The builtins move raw bits: reinterpret, never convert.
+template <typename T>
+__device__ __forceinline__ float ld_buf(__amdgpu_buffer_rsrc_t rs, unsigned soff, unsigned voff)
+{
+    if constexpr (sizeof(T) == 4)
+        return __builtin_bit_cast(
+            float, __builtin_amdgcn_raw_buffer_load_b32(rs, (int)voff, (int)soff, 0));
+    else
+        return wld(__builtin_bit_cast(
+            T, __builtin_amdgcn_raw_buffer_load_b16(rs, (int)voff, (int)soff, 0)));
+}
+template <typename T>
+__device__ __forceinline__ void st_buf(__amdgpu_buffer_rsrc_t rs, unsigned soff, unsigned voff,
+                                       float v)
+{
+    if constexpr (sizeof(T) == 4)
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), rs, (int)voff,
+                                              (int)soff, 0);
+    else
+        __builtin_amdgcn_raw_buffer_store_b16(__builtin_bit_cast(unsigned short, wst<T>(v)), rs,
+                                              (int)voff, (int)soff, 0);
+}
+
+template <int R, int BY, int RY, int NWM>
+constexpr size_t rows_lds_bytes()
+{
+    return 2u * (size_t)(BY * RY + 2 * R) * (size_t)(64 * NWM + 2 * R) * sizeof(float);
+}
+
+template <typename T, int R, int BY, int RY, int NWM, bool IMG, bool TAPER>
+__global__ void __launch_bounds__(64 * NWM * BY)
+    k_stencil_rows(const T *__restrict__ P, T *__restrict__ PP, const float *__restrict__ V,
+                   const Coefs cf, const Grid g, const RowGrid rg, const ImageFuse fz,
+                   const TaperFuse tf, const float *__restrict__ gt, int *__restrict__ flag)
+{
+    constexpr int TY = BY * RY;
+    constexpr int LW = 64 * NWM + 2 * R;  // LDS row: z in [-R, 64*NWM + R)
+    constexpr int LH = TY + 2 * R;
+    constexpr int NH = 2 * R;
+    constexpr int NYH = (NH + BY - 1) / BY;
+    constexpr int NR = 2 * R + 2;
+    constexpr unsigned ES = sizeof(T);          // wavefield element bytes
+    constexpr unsigned VS = 4u / sizeof(T);     // float / wavefield offset ratio
+    extern __shared__ float smem[];
+    const T *__restrict__ PS = static_cast<const T *>(fz.ps);
+
+    int by, bx;
+    row_block_coords(rg, by, bx);
+    const int z = threadIdx.x;
+    const int ty = __builtin_amdgcn_readfirstlane(threadIdx.y);
+    const int wz0 = __builtin_amdgcn_readfirstlane(threadIdx.x) & ~63;  // wave's first z
+    const int y0 = R + by * TY;
+    const int xb = R + bx * rg.xchunk;
+    const int xe = min(xb + rg.xchunk, g.nx - R);
+    const unsigned zc = (unsigned)min(z, g.nz - 1);
+    const unsigned zoff = zc * ES;   // wavefield
+    const unsigned zoffv = zc * 4u;  // V and the image
+    const bool zcomp = (z >= R) && (z < g.nz - R);
+
+    // Taper: only waves holding some z < b, only lanes with z < b.  The
+    // table g[b] is copied to LDS behind the two plane buffers, so per-plane
+    // factor reads wait on the LDS counter, not behind the prefetches.
+    float *tab = smem + 2 * LH * LW;
+    if (TAPER) {
+        const int tid = threadIdx.y * blockDim.x + threadIdx.x;
+        for (int i = tid; i < tf.b; i += blockDim.x * blockDim.y) tab[i] = gt[i];
+        __syncthreads();
+    }
+    // uniform factor lat(i) of a lateral border of n cells
+    auto lat = [&](int i, int n) -> float {
+        const int j = i < tf.b ? tf.b - 1 - i : (i >= n - tf.b ? i - (n - tf.b) : -1);
+        const float v = j >= 0 ? tab[max(j, 0)] : 1.0f;
+        return __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, v)));
+    };
+    const bool wtap = TAPER && (wz0 < tf.b);
+    const bool ztap = wtap && (z < tf.b);
+    float fzl = 1.0f;
+    if (ztap) fzl = tab[tf.b - 1 - z];
+
+    unsigned rowb[RY];  // row byte offsets of the wavefield in a plane, uniform
+    unsigned ymask = 0, rmask = 0;  // bit r: row r is updated / holds receivers
+    float fyr[RY];
+#pragma unroll
+    for (int r = 0; r < RY; ++r) {
+        const int y = y0 + ty + BY * r;
+        const int yc = min(y, g.ny - 1);
+        rowb[r] = (unsigned)yc * (unsigned)g.sy * ES;
+        if (y < g.ny - R) ymask |= 1u << r;
+        if (IMG) {
+            const int dy = y - fz.ry0;
+            if (dy >= 0 && (dy % fz.rdy) == 0 && (dy / fz.rdy) < fz.rny) rmask |= 1u << r;
+        }
+        fyr[r] = 1.0f;
+        if (wtap) fyr[r] = lat(yc, g.ny);
+    }
+    const bool rcvz = IMG && (z == fz.rz);
+    const bool imgz = IMG && (z >= fz.z0) && (z < fz.z1);
+
+    unsigned yh_b[NYH];
+    int yh_lrow[NYH];
+    bool yh_valid[NYH];
+    float yh_fy[NYH];
+#pragma unroll
+    for (int i = 0; i < NYH; ++i) {
+        const int h = ty + BY * i;
+        yh_valid[i] = h < NH;
+        const int gy = min(h < R ? y0 - R + h : y0 + TY + (h - R), g.ny - 1);
+        yh_lrow[i] = h < R ? h : R + TY + (h - R);
+        yh_b[i] = (unsigned)gy * (unsigned)g.sy * ES;
+        yh_fy[i] = 1.0f;
+        if (wtap) yh_fy[i] = lat(gy, g.ny);
+    }
+
+    float q[NR][RY];
+    float n_yh[2][NYH];
+
+    auto plane_fx = [&](int x) -> float { return wtap ? lat(x, g.nx) : 1.0f; };
+    // Branch-free taper: the lane factors are exactly 1 where no taper applies
+    // (z >= b), so every lane multiplies and the result is unchanged there.
+    // One application is one in-place pass of the reference: rounded to T.
+    auto tap = [&](float v, float fx, float fy) -> float {
+        return wrnd<T>(taper1(v, ztap ? fx : 1.0f, ztap ? fy : 1.0f, fzl));
+    };
+    // Loads are issued raw; the taper of a prefetched value is applied at the
+    // start of the step that first uses it, when the load has long landed
+    // (tapering at issue would make each wave wait for its newest load).
+    auto load_ring = [&](int x, float(&dst)[RY]) {
+        const auto rs = plane_rsrc(P + (size_t)x * (size_t)g.sx);
+#pragma unroll
+        for (int r = 0; r < RY; ++r) dst[r] = ld_buf<T>(rs, rowb[r], zoff);
+    };
+    auto taper_ring = [&](int x, float(&dst)[RY]) {
+        if (TAPER && wtap) {
+            const float fx = plane_fx(x);
+#pragma unroll
+            for (int r = 0; r < RY; ++r) dst[r] = tap(dst[r], fx, fyr[r]);
+        }
+    };
+    auto fetch_halo = [&](int x, int set) {
+        const auto rs = plane_rsrc(P + (size_t)x * (size_t)g.sx);
+#pragma unroll
+        for (int i = 0; i < NYH; ++i)
+            n_yh[set][i] = yh_valid[i] ? ld_buf<T>(rs, yh_b[i], zoff) : 0.f;
+    };
+    auto taper_halo = [&](int x, int set) {
+        if (TAPER && wtap) {
+            const float fx = plane_fx(x);
+#pragma unroll
+            for (int i = 0; i < NYH; ++i) n_yh[set][i] = tap(n_yh[set][i], fx, yh_fy[i]);
+        }
+    };
+
+#pragma unroll
+    for (int s = 0; s <= 2 * R; ++s) load_ring(xb - R + s, q[s]);
+    fetch_halo(xb, 0);
+#pragma unroll
+    for (int s = 0; s < 2 * R; ++s) taper_ring(xb - R + s, q[s]);  // lead: at the first step
+
+    // per-lane LDS element index of this thread's first row, centre column
+    const int lbase = (R + ty) * LW + R + z;
+    bool bad = false;
+    int x = xb;
+    auto step = [&]<int K>(std::integral_constant<int, K>) -> bool {
+        if (x >= xe) return false;
+        constexpr int cur = K & 1;
+        constexpr int ctr = (K + R) % NR;
+        // prefetched last step (or in the prologue): lead plane x + R, halo of x
+        taper_ring(x + R, q[(K + 2 * R) % NR]);
+        taper_halo(x, cur);
+        float *tl = smem + cur * (LH * LW);
+        const size_t pl = (size_t)x * (size_t)g.sx;
+        const auto rpp = plane_rsrc(PP + pl);
+        const auto rv = plane_rsrc(V + pl);
+        const auto rps = plane_rsrc(IMG ? (const void *)(PS + pl) : (const void *)PP);
+        const auto rimg = plane_rsrc(IMG ? (const void *)(fz.img + pl) : (const void *)PP);
+        // this plane's PP, V (and imaging operands), consumed after the
+        // whole plane's Laplacians, which hide their latency
+        float c_pp[RY], c_v[RY], c_ps[RY], c_img[RY];
+#pragma unroll
+        for (int r = 0; r < RY; ++r) {
+            c_pp[r] = ld_buf<T>(rpp, rowb[r], zoff);
+            c_v[r] = ld_buf<float>(rv, rowb[r] * VS, zoffv);
+        }
+        if (IMG) {
+#pragma unroll
+            for (int r = 0; r < RY; ++r) {
+                c_ps[r] = ld_buf<T>(rps, rowb[r], zoff);
+                c_img[r] = ld_buf<float>(rimg, rowb[r] * VS, zoffv);
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < RY; ++r) tl[lbase + BY * r * LW] = q[ctr][r];
+#pragma unroll
+        for (int i = 0; i < NYH; ++i)
+            if (yh_valid[i]) tl[yh_lrow[i] * LW + R + z] = n_yh[cur][i];
+        if (x + 1 < xe) {
+            load_ring(x + 1 + R, q[(K + 2 * R + 1) % NR]);
+            fetch_halo(x + 1, cur ^ 1);
+        }
+        __syncthreads();
+
+        bool rcvx = false;
+        if (IMG) {
+            const int dx = x - fz.rx0;
+            rcvx = dx >= 0 && (dx % fz.rdx) == 0 && (dx / fz.rdx) < fz.rnx;
+        }
+        float fxc = 1.0f;
+        if (TAPER) fxc = plane_fx(x);
+        float lapl[RY];
+#pragma unroll
+        for (int r = 0; r < RY; ++r) {
+            const float *t = tl + lbase + BY * r * LW;
+            const float p = q[ctr][r];
+            float ax = 0.f, ay = 0.f, az = 0.f;
+#pragma unroll
+            for (int io = 0; io <= 2 * R; ++io) {
+                const float vy = (io == R) ? p : t[(io - R) * LW];
+                const float vz = (io == R) ? p : t[io - R];
+                ax += q[(K + io) % NR][r] * cf.c[0][io];
+                ay += vy * cf.c[1][io];
+                az += vz * cf.c[2][io];
+            }
+            lapl[r] = ax + ay + az;
+            // Pin the Laplacian here: left alone, the compiler sinks it into
+            // the divergent store branch below and hoists every row's LDS
+            // operands above all branches, doubling the register count.
+            asm volatile("" : "+v"(lapl[r]));
+        }
+#pragma unroll
+        for (int r = 0; r < RY; ++r) {
+            const float p = q[ctr][r];
+            float pp = c_pp[r];
+            if (TAPER && wtap) {
+                pp = tap(pp, fxc, fyr[r]);
+                if (tf.pp_taps > 1) pp = tap(pp, fxc, fyr[r]);
+            }
+            const float npp = 2.f * p - pp + c_v[r] * lapl[r];
+            if (((ymask >> r) & 1u) && zcomp) {
+                st_buf<T>(rpp, rowb[r], zoff, npp);
+                bad |= (npp != npp);
+                // the image reads the stored (rounded) receiver value
+                if (IMG && imgz && !(rcvx && ((rmask >> r) & 1u) && rcvz))
+                    st_buf<float>(rimg, rowb[r] * VS, zoffv, c_img[r] + c_ps[r] * wrnd<T>(npp));
             }
         }
         ++x;
@@ -292,28 +617,35 @@ __global__ void __launch_bounds__(TZ *BY)
 struct LaunchKey {
     const void *fn;
     int threads;
+    size_t lds;
     bool operator<(const LaunchKey &o) const
     {
-        return fn < o.fn || (fn == o.fn && threads < o.threads);
+        if (fn != o.fn) return fn < o.fn;
+        if (threads != o.threads) return threads < o.threads;
+        return lds < o.lds;
     }
 };
 
 // Resident blocks on the whole device for one kernel (occupancy x CUs).
-int device_capacity(const void *fn, int threads)
+// Kernels using more than 64 KiB of dynamic LDS get the attribute raised
+// first (gfx950 has 160 KiB per CU).
+int device_capacity(const void *fn, int threads, size_t lds = 0)
 {
     static std::mutex mu;
     static std::map<LaunchKey, int> cache;
     std::lock_guard<std::mutex> lock(mu);
-    const LaunchKey key{fn, threads};
+    const LaunchKey key{fn, threads, lds};
     auto it = cache.find(key);
     if (it != cache.end()) return it->second;
+    if (lds > 65536)
+        (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     int dev = 0, ncu = 0, per_cu = 0;
     if (hipGetDevice(&dev) != hipSuccess) dev = 0;
     if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
         ncu = 256;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, threads, 0) != hipSuccess ||
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, threads, lds) != hipSuccess ||
         per_cu <= 0)
-        per_cu = 4;
+        per_cu = 1;
     const int cap = std::max(1, ncu * per_cu);
     cache[key] = cap;
     return cap;
@@ -341,13 +673,13 @@ int choose_chunks(long long tiles, int nxi, int R, int cap)
     return best_n;
 }
 
-template <int R, int TZ, int BY, int RY, bool IMG, bool TAPER>
-hipError_t launch_cfg(const float *P, float *PP, const float *V, const Coefs &cf, const Grid &g,
+template <typename T, int R, int TZ, int BY, int RY, bool IMG, bool TAPER>
+hipError_t launch_cfg(const T *P, T *PP, const float *V, const Coefs &cf, const Grid &g,
                       const ImageFuse &fz, const TaperFuse &tf, int *flag, hipStream_t s,
                       int xchunk)
 {
     constexpr int TY = BY * RY;
-    auto kern = k_stencil<R, TZ, BY, RY, IMG, TAPER>;
+    auto kern = k_stencil<T, R, TZ, BY, RY, IMG, TAPER>;
     TileGrid tg;
     tg.ntz = (g.nz + TZ - 1) / TZ;
     tg.nty = (g.ny - 2 * R + TY - 1) / TY;
@@ -366,46 +698,149 @@ hipError_t launch_cfg(const float *P, float *PP, const float *V, const Coefs &cf
     return hipGetLastError();
 }
 
-template <int R, int TZ, int BY, int RY>
-hipError_t launch_variant(const float *P, float *PP, const float *V, const Coefs &cf,
-                          const Grid &g, const ImageFuse *fuse, const TaperFuse *taper,
-                          int *flag, hipStream_t s, int xchunk)
+// The four fusion variants of one compiled shape.  L is a generic lambda
+// taking the two flags as integral constants.
+template <typename L>
+hipError_t by_variant(const ImageFuse *fuse, const TaperFuse *taper, L &&launch)
 {
-    ImageFuse fz{};
-    TaperFuse tf{};
+    const bool tap = taper && taper->b > 0;
+    if (fuse && tap) return launch(std::true_type{}, std::true_type{});
+    if (fuse) return launch(std::true_type{}, std::false_type{});
+    if (tap) return launch(std::false_type{}, std::true_type{});
+    return launch(std::false_type{}, std::false_type{});
+}
+
+void fuse_args(const ImageFuse *fuse, const TaperFuse *taper, ImageFuse &fz, TaperFuse &tf)
+{
+    fz = ImageFuse{};
+    tf = TaperFuse{};
     if (fuse) {
         fz = *fuse;
         if (fz.rdx <= 0) fz.rdx = 1;
         if (fz.rdy <= 0) fz.rdy = 1;
     }
     if (taper) tf = *taper;
-    const bool tap = taper && taper->b > 0;
-    if (fuse && tap)
-        return launch_cfg<R, TZ, BY, RY, true, true>(P, PP, V, cf, g, fz, tf, flag, s, xchunk);
-    if (fuse)
-        return launch_cfg<R, TZ, BY, RY, true, false>(P, PP, V, cf, g, fz, tf, flag, s, xchunk);
-    if (tap)
-        return launch_cfg<R, TZ, BY, RY, false, true>(P, PP, V, cf, g, fz, tf, flag, s, xchunk);
-    return launch_cfg<R, TZ, BY, RY, false, false>(P, PP, V, cf, g, fz, tf, flag, s, xchunk);
 }
 
-template <int R>
-hipError_t launch_radius(const float *P, float *PP, const float *V, const Coefs &cf,
-                         const Grid &g, const ImageFuse *fuse, const TaperFuse *taper, int *flag,
-                         hipStream_t s, const StencilTiling *t)
+template <typename T, int R, int TZ, int BY, int RY>
+hipError_t launch_variant(const T *P, T *PP, const float *V, const Coefs &cf, const Grid &g,
+                          const ImageFuse *fuse, const TaperFuse *taper, int *flag,
+                          hipStream_t s, int xchunk)
 {
+    ImageFuse fz;
+    TaperFuse tf;
+    fuse_args(fuse, taper, fz, tf);
+    return by_variant(fuse, taper, [&](auto img, auto tap) {
+        return launch_cfg<T, R, TZ, BY, RY, decltype(img)::value, decltype(tap)::value>(
+            P, PP, V, cf, g, fz, tf, flag, s, xchunk);
+    });
+}
+
+template <typename T, int R, int BY, int RY, int NWM, bool IMG, bool TAPER>
+hipError_t launch_rows_cfg(const T *P, T *PP, const float *V, const Coefs &cf, const Grid &g,
+                           const ImageFuse &fz, const TaperFuse &tf, int *flag, hipStream_t s,
+                           int xchunk)
+{
+    constexpr int TY = BY * RY;
+    auto kern = k_stencil_rows<T, R, BY, RY, NWM, IMG, TAPER>;
+    const int nw = (g.nz + 63) / 64;
+    if (nw > NWM) return hipErrorInvalidConfiguration;
+    RowGrid rg;
+    rg.nty = (g.ny - 2 * R + TY - 1) / TY;
+    const int nxi = g.nx - 2 * R;
+    if (rg.nty <= 0 || nxi <= 0 || g.nz - 2 * R <= 0) return hipSuccess;
+    const size_t lds = rows_lds_bytes<R, BY, RY, NWM>() + (TAPER ? (size_t)tf.b * sizeof(float) : 0);
+    const int threads = 64 * nw * BY;
+    const int cap = device_capacity((const void *)kern, threads, lds);
+    if (xchunk <= 0) {
+        const int n = choose_chunks(rg.nty, nxi, R, cap);
+        xchunk = (nxi + n - 1) / n;
+    }
+    rg.xchunk = xchunk;
+    rg.nxc = (nxi + xchunk - 1) / xchunk;
+    hipLaunchKernelGGL(kern, dim3((unsigned)(rg.nty * rg.nxc)), dim3(64 * nw, BY), lds, s, P, PP,
+                       V, cf, g, rg, fz, tf, tf.t, flag);
+    return hipGetLastError();
+}
+
+template <typename T, int R, int BY, int RY, int NWM>
+hipError_t launch_rows(const T *P, T *PP, const float *V, const Coefs &cf, const Grid &g,
+                       const ImageFuse *fuse, const TaperFuse *taper, int *flag, hipStream_t s,
+                       int xchunk)
+{
+    ImageFuse fz;
+    TaperFuse tf;
+    fuse_args(fuse, taper, fz, tf);
+    return by_variant(fuse, taper, [&](auto img, auto tap) {
+        return launch_rows_cfg<T, R, BY, RY, NWM, decltype(img)::value, decltype(tap)::value>(
+            P, PP, V, cf, g, fz, tf, flag, s, xchunk);
+    });
+}
+
+// Full-row shapes compiled for radius 4 (by thread rows x ry rows each):
+// (4, 4) up to 4 waves per row (nz <= 256); (2, 6) and (2, 7) up to 6 waves
+// per row (nz <= 384).
+inline bool rows_fit(int nz, int by, int ry)
+{
+    const int nw = (nz + 63) / 64;
+    if (by == 4 && ry == 4) return nw <= 4;
+    if (by == 2 && (ry == 6 || ry == 7)) return nw <= 6;
+    return false;
+}
+
+// FP32 wavefields: every compiled shape.  16-bit wavefields: radius 4 with
+// the default shapes (full rows 4x4 and 2x6, tiles 64x4x4); any other
+// requested shape runs the default.
+template <typename T, int R>
+hipError_t launch_radius(const T *P, T *PP, const float *V, const Coefs &cf, const Grid &g,
+                         const ImageFuse *fuse, const TaperFuse *taper, int *flag, hipStream_t s,
+                         const StencilTiling *t)
+{
+    constexpr bool full = std::is_same<T, float>::value;
     const int xchunk = t ? t->xchunk : 0;
-    if (!t || (t->tz == 64 && t->by == 4 && t->ry == 4))
-        return launch_variant<R, 64, 4, 4>(P, PP, V, cf, g, fuse, taper, flag, s, xchunk);
-    if (t->tz == 64 && t->by == 4 && t->ry == 2)
-        return launch_variant<R, 64, 4, 2>(P, PP, V, cf, g, fuse, taper, flag, s, xchunk);
-    if (t->tz == 64 && t->by == 8 && t->ry == 2)
-        return launch_variant<R, 64, 8, 2>(P, PP, V, cf, g, fuse, taper, flag, s, xchunk);
-    if (t->tz == 32 && t->by == 8 && t->ry == 4)
-        return launch_variant<R, 32, 8, 4>(P, PP, V, cf, g, fuse, taper, flag, s, xchunk);
-    if (t->tz == 32 && t->by == 8 && t->ry == 2)
-        return launch_variant<R, 32, 8, 2>(P, PP, V, cf, g, fuse, taper, flag, s, xchunk);
-    return hipErrorInvalidValue;
+    if constexpr (R == 4) {
+        // full-row blocks by default whenever a compiled shape covers nz (and
+        // the taper table fits in LDS beside the plane buffers)
+        const bool want_rows =
+            (!t || t->tz == 1 || !full) && (!taper || taper->b <= 4096);
+        if (want_rows) {
+            int by = t ? t->by : 0, ry = t ? t->ry : 0;
+            if (!t || !full) {
+                const bool keep = t && t->tz == 1 && (by == 4 && ry == 4 ? rows_fit(g.nz, 4, 4)
+                                                      : by == 2 && ry == 6 && rows_fit(g.nz, 2, 6));
+                if (!keep) {
+                    by = ry = 0;
+                    if (rows_fit(g.nz, 4, 4)) by = 4, ry = 4;
+                    else if (rows_fit(g.nz, 2, 6)) by = 2, ry = 6;
+                }
+            }
+            if (by == 4 && ry == 4 && rows_fit(g.nz, 4, 4))
+                return launch_rows<T, R, 4, 4, 4>(P, PP, V, cf, g, fuse, taper, flag, s, xchunk);
+            if (by == 2 && ry == 6 && rows_fit(g.nz, 2, 6))
+                return launch_rows<T, R, 2, 6, 6>(P, PP, V, cf, g, fuse, taper, flag, s, xchunk);
+            if constexpr (full) {
+                if (by == 2 && ry == 7 && rows_fit(g.nz, 2, 7))
+                    return launch_rows<T, R, 2, 7, 6>(P, PP, V, cf, g, fuse, taper, flag, s,
+                                                      xchunk);
+            }
+            // a full-row shape that does not cover nz falls back to tiles
+        }
+    }
+    if constexpr (full) {
+        if (!t || t->tz == 1 || (t->tz == 64 && t->by == 4 && t->ry == 4))
+            return launch_variant<T, R, 64, 4, 4>(P, PP, V, cf, g, fuse, taper, flag, s, xchunk);
+        if (t->tz == 64 && t->by == 4 && t->ry == 2)
+            return launch_variant<T, R, 64, 4, 2>(P, PP, V, cf, g, fuse, taper, flag, s, xchunk);
+        if (t->tz == 64 && t->by == 8 && t->ry == 2)
+            return launch_variant<T, R, 64, 8, 2>(P, PP, V, cf, g, fuse, taper, flag, s, xchunk);
+        if (t->tz == 32 && t->by == 8 && t->ry == 4)
+            return launch_variant<T, R, 32, 8, 4>(P, PP, V, cf, g, fuse, taper, flag, s, xchunk);
+        if (t->tz == 32 && t->by == 8 && t->ry == 2)
+            return launch_variant<T, R, 32, 8, 2>(P, PP, V, cf, g, fuse, taper, flag, s, xchunk);
+        return hipErrorInvalidValue;
+    } else {
+        return launch_variant<T, R, 64, 4, 4>(P, PP, V, cf, g, fuse, taper, flag, s, xchunk);
+    }
 }
 
 }  // namespace
@@ -413,20 +848,40 @@ hipError_t launch_radius(const float *P, float *PP, const float *V, const Coefs 
 bool stencil_tiling_supported(int tz, int by, int ry)
 {
     return (tz == 64 && by == 4 && (ry == 4 || ry == 2)) || (tz == 64 && by == 8 && ry == 2) ||
-           (tz == 32 && by == 8 && (ry == 4 || ry == 2));
+           (tz == 32 && by == 8 && (ry == 4 || ry == 2)) ||
+           (tz == 1 && ((by == 4 && ry == 4) || (by == 2 && (ry == 6 || ry == 7))));
 }
 
-hipError_t launch_stencil(int radius, const float *P, float *PP, const float *V, const Coefs &cf,
+template <typename T>
+hipError_t launch_stencil(int radius, const T *P, T *PP, const float *V, const Coefs &cf,
                           const Grid &g, const ImageFuse *fuse, const TaperFuse *taper,
                           int *flag, hipStream_t s, const StencilTiling *tiling)
 {
-    switch (radius) {
-    case 1: return launch_radius<1>(P, PP, V, cf, g, fuse, taper, flag, s, tiling);
-    case 2: return launch_radius<2>(P, PP, V, cf, g, fuse, taper, flag, s, tiling);
-    case 3: return launch_radius<3>(P, PP, V, cf, g, fuse, taper, flag, s, tiling);
-    case 4: return launch_radius<4>(P, PP, V, cf, g, fuse, taper, flag, s, tiling);
-    default: return hipErrorInvalidValue;
+    if constexpr (std::is_same<T, float>::value) {
+        switch (radius) {
+        case 1: return launch_radius<T, 1>(P, PP, V, cf, g, fuse, taper, flag, s, tiling);
+        case 2: return launch_radius<T, 2>(P, PP, V, cf, g, fuse, taper, flag, s, tiling);
+        case 3: return launch_radius<T, 3>(P, PP, V, cf, g, fuse, taper, flag, s, tiling);
+        case 4: return launch_radius<T, 4>(P, PP, V, cf, g, fuse, taper, flag, s, tiling);
+        default: return hipErrorInvalidValue;
+        }
+    } else {
+        if (radius != 4) return hipErrorNotSupported;
+        return launch_radius<T, 4>(P, PP, V, cf, g, fuse, taper, flag, s, tiling);
     }
 }
+
+template hipError_t launch_stencil<float>(int, const float *, float *, const float *,
+                                          const Coefs &, const Grid &, const ImageFuse *,
+                                          const TaperFuse *, int *, hipStream_t,
+                                          const StencilTiling *);
+template hipError_t launch_stencil<_Float16>(int, const _Float16 *, _Float16 *, const float *,
+                                             const Coefs &, const Grid &, const ImageFuse *,
+                                             const TaperFuse *, int *, hipStream_t,
+                                             const StencilTiling *);
+template hipError_t launch_stencil<__bf16>(int, const __bf16 *, __bf16 *, const float *,
+                                           const Coefs &, const Grid &, const ImageFuse *,
+                                           const TaperFuse *, int *, hipStream_t,
+                                           const StencilTiling *);
 
 }  // namespace rtmhip

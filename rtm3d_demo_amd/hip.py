@@ -17,6 +17,7 @@ EXPORTED = [
     "rtmhip_init", "rtmhip_finalize", "rtmhip_last_error", "rtmhip_device_count",
     "rtmhip_synchronize", "rtmhip_version", "rtmhip_alloc", "rtmhip_free",
     "rtmhip_copy_to_device", "rtmhip_copy_from_device", "rtmhip_copy_device", "rtmhip_fill",
+    "rtmhip_fill16",
     "rtmhip_step", "rtmhip_taper", "rtmhip_apply_source", "rtmhip_receivers",
     "rtmhip_upper_border", "rtmhip_image_condition", "rtmhip_image_finish", "rtmhip_stack",
     "rtmhip_model_steps", "rtmhip_forward_steps", "rtmhip_backward_steps",
@@ -38,7 +39,12 @@ class RcvGeom(C.Structure):
 class Shot(C.Structure):
     _fields_ = [("nxe", C.c_int), ("nye", C.c_int), ("nze", C.c_int), ("blen", C.c_int),
                 ("order", C.c_int), ("nt", C.c_int), ("ntstep", C.c_int), ("sxe", C.c_int),
-                ("sye", C.c_int), ("sze", C.c_int), ("rcv", RcvGeom)]
+                ("sye", C.c_int), ("sze", C.c_int), ("rcv", RcvGeom), ("wave_format", C.c_int)]
+
+
+# Wavefield storage formats (RTMHIP_WAVE_*): name -> (code, numpy storage dtype).
+# 16-bit wavefields are held as raw uint16 bits on the host.
+WAVE_FORMATS = {"f32": (0, np.float32), "f16": (1, np.uint16), "bf16": (2, np.uint16)}
 
 
 _lib = None
@@ -61,6 +67,7 @@ def load():
         "rtmhip_alloc": [pp, sz], "rtmhip_free": [vp],
         "rtmhip_copy_to_device": [vp, vp, sz], "rtmhip_copy_from_device": [vp, vp, sz],
         "rtmhip_copy_device": [vp, vp, sz], "rtmhip_fill": [fp, sz, C.c_float],
+        "rtmhip_fill16": [vp, sz, C.c_ushort],
         "rtmhip_step": [fp, fp, fp, fp, i, i, i, i],
         "rtmhip_taper": [fp, fp, i, i, i, i, i],
         "rtmhip_apply_source": [fp, i, i, i, i, i, i, C.c_float],
@@ -124,7 +131,10 @@ class DeviceArray:
         return out
 
     def zero(self):
-        check(load().rtmhip_fill(self.ptr, self.size, 0.0))
+        if self.dtype.itemsize == 2:
+            check(load().rtmhip_fill16(self.ptr, self.size, 0))
+        else:
+            check(load().rtmhip_fill(self.ptr, self.size, 0.0))
         return self
 
     def free(self):
@@ -197,16 +207,18 @@ def stack(dst, src):
 
 
 # ---------------- time loops ----------------
-def make_shot(nxe, nye, nze, blen, order, nt, ntstep, src_ext, geom):
-    return Shot(nxe, nye, nze, blen, order, nt, ntstep, src_ext[0], src_ext[1], src_ext[2], geom)
+def make_shot(nxe, nye, nze, blen, order, nt, ntstep, src_ext, geom, fmt="f32"):
+    return Shot(nxe, nye, nze, blen, order, nt, ntstep, src_ext[0], src_ext[1], src_ext[2], geom,
+                WAVE_FORMATS[fmt][0])
 
 
 class FieldPair:
-    """Two wavefield buffers swapped in place by the time loops."""
+    """Two wavefield buffers swapped in place by the time loops (dtype:
+    float32, or uint16 for the 16-bit formats)."""
 
-    def __init__(self, shape):
-        self.a = DeviceArray(shape).zero()
-        self.b = DeviceArray(shape).zero()
+    def __init__(self, shape, dtype=np.float32):
+        self.a = DeviceArray(shape, dtype).zero()
+        self.b = DeviceArray(shape, dtype).zero()
         self.p = C.c_void_p(self.a.ptr.value)
         self.pp = C.c_void_p(self.b.ptr.value)
 
@@ -262,7 +274,8 @@ def get_tiling(kernel_class):
     return tuple(x.value for x in v)
 
 
-TILINGS = [(64, 4, 4), (64, 4, 2), (64, 8, 2), (32, 8, 4), (32, 8, 2)]
+TILINGS = [(1, 4, 4), (1, 2, 6), (1, 2, 7), (64, 4, 4), (64, 4, 2), (64, 8, 2), (32, 8, 4),
+           (32, 8, 2)]
 
 
 def timing_enable(on=True):

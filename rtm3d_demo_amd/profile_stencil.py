@@ -1,0 +1,137 @@
+"""Fixed launch sequence for hardware-counter profiling of the stencil.
+
+Run under rocprofv3 (one counter pass per run), e.g.
+    rocprofv3 --pmc FETCH_SIZE -d OUT -o run --output-format csv -- \
+        python3 rtm3d_demo_amd/profile_stencil.py
+
+On the 592x592x336 grid of the 512x512x256 + 40 salt workload it launches:
+  * k_stack over the grid 3 times: a calibration kernel that reads 8 B and
+    writes 4 B per cell, so its counters give the scale between the counter
+    and true bytes;
+  * the plain stencil step 5 times (rtmhip_step);
+  * 5 HBC forward steps (tapered stencil class) and 5 fused backward steps.
+Fields hold random values; counters do not depend on them.
+
+    python3 rtm3d_demo_amd/profile_stencil.py --summarize FETCH_DIR WRITE_DIR
+
+turns the counter_collection.csv files of a FETCH_SIZE pass and a WRITE_SIZE
+pass into per-kernel bytes per dispatch.  Each counter is scaled by what it
+reads for k_stack, whose true traffic is known (8 B read + 4 B written per
+cell), and the plain stencil's total is printed as
+"traffic_bytes_per_launch: N", the line bench.py reads.
+"""
+import csv
+import glob
+import os
+import sys
+
+import numpy as np
+
+sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+
+
+SHAPE = (592, 592, 336)
+
+
+def _per_dispatch(d, counter):
+    """Mean counter value (KiB) per dispatch, keyed by full kernel name."""
+    files = glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True)
+    if not files:
+        raise SystemExit("no counter_collection.csv under %s" % d)
+    per = {}
+    for f in files:
+        with open(f, newline="") as fh:
+            for row in csv.DictReader(fh):
+                if row["Counter_Name"] != counter:
+                    continue
+                key = (row["Kernel_Name"], row["Dispatch_Id"])
+                per[key] = per.get(key, 0.0) + float(row["Counter_Value"])
+    out = {}
+    for (name, _), v in per.items():
+        out.setdefault(name, []).append(v)
+    return {k: (sum(v) / len(v), len(v)) for k, v in out.items()}
+
+
+def _short(name):
+    name = name.replace("rtmhip::(anonymous namespace)::", "").replace("void ", "")
+    return name.split("(")[0]
+
+
+def summarize(fetch_dir, write_dir):
+    fetch = _per_dispatch(fetch_dir, "FETCH_SIZE")
+    write = _per_dispatch(write_dir, "WRITE_SIZE")
+    cells = SHAPE[0] * SHAPE[1] * SHAPE[2]
+    stack = [k for k in fetch if _short(k) == "k_stack"]
+    if not stack:
+        raise SystemExit("k_stack not found: calibration impossible")
+    kf, kw = fetch[stack[0]][0] * 1024.0, write[stack[0]][0] * 1024.0
+    f_scale, w_scale = 8.0 * cells / kf, 4.0 * cells / kw
+    interior = (SHAPE[0] - 8) * (SHAPE[1] - 8) * (SHAPE[2] - 8)
+    print("calibration: k_stack reads %.1f MB and writes %.1f MB per dispatch;"
+          % (8e-6 * cells, 4e-6 * cells))
+    print("FETCH_SIZE reads %.1f MB (scale %.3f), WRITE_SIZE %.1f MB (scale %.3f)"
+          % (kf * 1e-6, f_scale, kw * 1e-6, w_scale))
+    print()
+    print("| kernel | dispatches | read MB | write MB | total MB | B per updated cell |")
+    print("|---|---|---|---|---|---|")
+    plain = None
+    for name in sorted(fetch, key=lambda k: -fetch[k][0]):
+        if name not in write:
+            continue
+        r = fetch[name][0] * 1024.0 * f_scale
+        w = write[name][0] * 1024.0 * w_scale
+        short = _short(name)
+        print("| %s | %d | %.1f | %.1f | %.1f | %.2f |" % (
+            short, fetch[name][1], r * 1e-6, w * 1e-6, (r + w) * 1e-6, (r + w) / interior))
+        if short.startswith("k_stencil") and "false, false" in name and plain is None:
+            plain = r + w
+    if plain is not None:
+        print()
+        print("algorithmic bytes per plain launch: %.0f (16 B x %d updated cells)"
+              % (16.0 * interior, interior))
+        print("traffic_bytes_per_launch: %.0f" % plain)
+
+
+def main():
+    if len(sys.argv) == 4 and sys.argv[1] == "--summarize":
+        summarize(sys.argv[2], sys.argv[3])
+        return 0
+    from rtm3d_demo_amd import hip, host
+
+    shape = SHAPE
+    b, order, nt = 40, 8, 16
+    hip.init(0)
+    rng = np.random.default_rng(0)
+
+    def rand(lo, hi):
+        out = np.empty(shape, np.float32)
+        for x in range(shape[0]):
+            out[x] = rng.uniform(lo, hi, shape[1:])
+        return out
+
+    V = hip.DeviceArray.from_host(rand(0.0, 0.2))
+    coefs = host.stencil_coefs(order, 12.5, 12.5, 12.5)
+    g = hip.DeviceArray.from_host(host.taper_table(b, 0.9))
+    src = hip.DeviceArray.from_host(host.ricker(nt, 1.1e-3, 25.0))
+    F, R = hip.FieldPair(shape), hip.FieldPair(shape)
+    F.a.upload(rand(-1, 1))
+    F.b.upload(rand(-1, 1))
+    img = hip.DeviceArray(shape).zero()
+    upb = hip.DeviceArray((nt, shape[0], shape[1], 4)).zero()
+    rcv = hip.DeviceArray((128, 128, nt)).zero()
+    geom = hip.RcvGeom(0, 0, 0, 4, 4, 128, 128)
+    shot = hip.make_shot(*shape, b, order, nt, nt, (296, 296, 42), geom)
+    for _ in range(3):
+        hip.stack(img, V)
+    for _ in range(5):
+        hip.step(F.a, F.b, V, coefs, order)
+    hip.forward_steps(shot, True, V, coefs, g, src, F, upb, 0, 5)
+    hip.backward_steps(shot, True, V, coefs, g, F, R, upb, rcv, img, nt - 1, 7, True)
+    hip.synchronize()
+    print("profile sequence done", flush=True)
+    hip.finalize()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

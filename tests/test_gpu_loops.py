@@ -27,10 +27,12 @@ def _shots(p):
 
 
 class Setup:
-    """Product host set-up of one case, uploaded to the device."""
+    """Product host set-up of one case, uploaded to the device; fmt is the
+    wavefield storage format ('f32', 'f16', 'bf16')."""
 
-    def __init__(self, gpu, p, ve):
-        self.gpu, self.p = gpu, p
+    def __init__(self, gpu, p, ve, fmt="f32"):
+        self.gpu, self.p, self.fmt = gpu, p, fmt
+        self.wdtype = gpu.WAVE_FORMATS[fmt][1]
         self.blen, self.order = p["blen"], p["stencil_order"]
         self.nt, self.ntstep = p["nt"], p["ntstep"]
         self.shape = ve.shape
@@ -43,18 +45,18 @@ class Setup:
     def shot(self, s):
         b = self.blen
         return self.gpu.make_shot(*self.shape, b, self.order, self.nt, self.ntstep,
-                                  (s[0] + b, s[1] + b, s[2] + b), self.geom)
+                                  (s[0] + b, s[1] + b, s[2] + b), self.geom, self.fmt)
 
     def free(self):
         for d in (self.V, self.src, self.g):
             d.free()
 
 
-def gpu_records(gpu, p, vel):
-    S = Setup(gpu, p, host.extend_velocity(vel, p["blen"], "abc"))
+def gpu_records(gpu, p, vel, fmt="f32"):
+    S = Setup(gpu, p, host.extend_velocity(vel, p["blen"], "abc"), fmt)
     out = []
     for s in _shots(p):
-        F = gpu.FieldPair(S.shape)
+        F = gpu.FieldPair(S.shape, S.wdtype)
         rcv = gpu.DeviceArray((p["receiver_count_x"], p["receiver_count_y"], S.ntstep)).zero()
         chunks = []
         for itstep in range(0, S.nt, S.ntstep):
@@ -69,8 +71,9 @@ def gpu_records(gpu, p, vel):
 
 def gpu_shot_image(gpu, S, s, records, hbc, fused):
     b = S.blen
-    F, R = gpu.FieldPair(S.shape), gpu.FieldPair(S.shape)
-    upb = gpu.DeviceArray((S.nt, S.shape[0], S.shape[1], S.order // 2)) if hbc else None
+    F, R = gpu.FieldPair(S.shape, S.wdtype), gpu.FieldPair(S.shape, S.wdtype)
+    upb = (gpu.DeviceArray((S.nt, S.shape[0], S.shape[1], S.order // 2), S.wdtype)
+           if hbc else None)
     for itstep in range(0, S.nt, S.ntstep):
         gpu.forward_steps(S.shot(s), hbc, S.V, S.coefs, S.g, S.src, F, upb, itstep, S.ntstep)
     img = gpu.DeviceArray(S.shape).zero()
@@ -85,9 +88,9 @@ def gpu_shot_image(gpu, S, s, records, hbc, fused):
     return out
 
 
-def gpu_migration(gpu, p, vpe, records, fused=True):
+def gpu_migration(gpu, p, vpe, records, fused=True, fmt="f32"):
     hbc = p["boundary_condition"] != "rbc"
-    S = Setup(gpu, p, vpe)
+    S = Setup(gpu, p, vpe, fmt)
     total = gpu.DeviceArray(S.shape).zero()
     simg = gpu.DeviceArray(S.shape)
     for k, s in enumerate(_shots(p)):
@@ -176,23 +179,32 @@ def _pair(gpu, A, B):
     return F
 
 
-@pytest.fixture(params=[0, 1, 5])
+@pytest.fixture(params=[((0, 0, 0), 0), ((1, 4, 4), 1), ((1, 4, 4), 5), ((1, 2, 6), 0),
+                        ((1, 2, 7), 3), ((64, 4, 4), 0), ((64, 4, 4), 5)],
+                ids=lambda p: "t%d.%d.%d-x%d" % (*p[0], p[1]))
 def xchunk(request, gpu):
-    """Planes marched per block: automatic, one, five (multi-plane marching
-    through the register ring is exercised even on small grids)."""
+    """Stencil shape and planes marched per block: the built-in default, each
+    full-row shape and the z-tiled shape, automatic or fixed short marches
+    (multi-plane marching through the register ring on small grids)."""
+    tiling, xc = request.param
     for cls in range(3):
-        gpu.set_tiling(cls, *(gpu.TILINGS[0] if request.param else (0, 0, 0)), request.param)
+        gpu.set_tiling(cls, *tiling, xc)
     yield request.param
     for cls in range(3):
         gpu.set_tiling(cls, 0)
 
 
+# (40, 44, 90): two waves per z row; (32, 36, 340): six waves per z row.
+GRIDS = [(40, 44, 90), (32, 36, 340)]
+
+
+@pytest.mark.parametrize("shape", GRIDS)
 @pytest.mark.parametrize("splits", [(5,), (2, 1, 2)])
-def test_fused_taper_forward_matches_operator_sequence(gpu, oracle, splits, xchunk):
+def test_fused_taper_forward_matches_operator_sequence(gpu, oracle, splits, xchunk, shape):
     """HBC forward steps with the upper taper fused into the stencil equal
     the reference's operator sequence (taper, taper, step, source, save
     UPB) on non-zero fields, whether run in one call or split into calls."""
-    shape, b, order, nt = (40, 44, 90), 12, 8, 5
+    b, order, nt = 12, 8, 5
     rng = np.random.default_rng(21)
     P0 = rng.uniform(-1, 1, shape).astype(np.float32)
     PP0 = rng.uniform(-1, 1, shape).astype(np.float32)
@@ -225,11 +237,12 @@ def test_fused_taper_forward_matches_operator_sequence(gpu, oracle, splits, xchu
         d.free()
 
 
+@pytest.mark.parametrize("shape", GRIDS)
 @pytest.mark.parametrize("splits", [(3,), (1, 2)])
-def test_fused_backward_matches_operator_sequence(gpu, oracle, splits, xchunk):
+def test_fused_backward_matches_operator_sequence(gpu, oracle, splits, xchunk, shape):
     """Fused receiver step (tapers, stencil, receiver injection, imaging)
     against the reference's operator sequence on non-zero fields."""
-    shape, b, order, nt = (40, 44, 90), 12, 8, 8
+    b, order, nt = 12, 8, 8
     rng = np.random.default_rng(22)
     Ps, PPs, Pr, PPr, img = (rng.uniform(-1, 1, shape).astype(np.float32) for _ in range(5))
     v2 = rng.uniform(0.0, 0.15, shape).astype(np.float32)
@@ -265,7 +278,8 @@ def test_fused_backward_matches_operator_sequence(gpu, oracle, splits, xchunk):
         x.free()
 
 
-@pytest.mark.parametrize("tiling", [(64, 4, 2), (64, 8, 2), (32, 8, 4), (32, 8, 2)])
+@pytest.mark.parametrize("tiling", [(64, 4, 4), (64, 4, 2), (64, 8, 2), (32, 8, 4), (32, 8, 2),
+                                    (1, 4, 4), (1, 2, 6), (1, 2, 7)])
 def test_tile_shapes_bit_exact(gpu, oracle, cases, golden, tiling):
     """Every compiled tile shape, for all three stencil classes, reproduces
     the reference HBC image bit for bit."""
