@@ -614,6 +614,261 @@ __global__ void __launch_bounds__(64 * NWM * BY)
     if (bad) atomicOr(flag, 1);
 }
 
+// ---------------------------------------------------------------------------
+// Barrier-free wave variant.  Every wave is its own tile: 64 z lanes x RY
+// rows, marching a chunk of x planes, with no block-wide synchronisation.
+//   * x taps: the register ring of 2R+2 planes (as above);
+//   * y taps: registers -- the wave's own RY rows of the centre plane plus
+//     2R halo rows (R above, R below) loaded a plane ahead;
+//   * z taps: the wave's private LDS rows (64 + 2R wide) holding the centre
+//     plane and the 2R z-halo cells of each row (one load per lane for
+//     RY <= 8).  Only lanes of the same wave touch a region, so the wave's
+//     in-order LDS queue plus a wave barrier (a compiler fence, no
+//     instruction) orders writes and reads.
+// Waves of one block take consecutive z tiles of the same rows, so z-halo
+// loads hit lines their neighbours fetch.
+struct WaveGrid {
+    int nzt, nyt, nxc, xchunk;
+    long long waves;  // nzt * nyt * nxc
+};
+
+template <typename T, int R, int RY, int WPB, bool IMG, bool TAPER>
+__global__ void __launch_bounds__(64 * WPB)
+    k_stencil_wave(const T *__restrict__ P, T *__restrict__ PP, const float *__restrict__ V,
+                   const Coefs cf, const Grid g, const WaveGrid wg, const ImageFuse fz,
+                   const TaperFuse tf, const float *__restrict__ gt, int *__restrict__ flag)
+{
+    constexpr int NR = 2 * R + 2;
+    constexpr int LW = 64 + 2 * R;              // LDS row: z0 - R .. z0 + 63 + R
+    constexpr int NZH = 2 * R;                  // z-halo cells per row
+    static_assert(NZH * RY <= 64, "one z-halo load per lane");
+    constexpr unsigned ES = sizeof(T);
+    constexpr unsigned VS = 4u / sizeof(T);
+    __shared__ float tile[WPB][RY][LW];
+    extern __shared__ float tab[];              // taper table g[b] (TAPER only)
+    const T *__restrict__ PS = static_cast<const T *>(fz.ps);
+
+    const int lane = threadIdx.x;
+    const int wid = __builtin_amdgcn_readfirstlane(threadIdx.y);
+    if (TAPER) {
+        const int tid = threadIdx.y * 64 + lane;
+        for (int i = tid; i < tf.b; i += 64 * WPB) tab[i] = gt[i];
+        __syncthreads();  // the only block barrier, before any wave leaves
+    }
+    // XCD-aware block order, then waves of a block in z-tile order
+    long long w;
+    {
+        const long long nb = (wg.waves + WPB - 1) / WPB;
+        const long long h = blockIdx.x;
+        const long long q = nb >> 3, rr = nb & 7, k = h & 7, j = h >> 3;
+        const long long t = k * q + (k < rr ? k : rr) + j;
+        w = t * WPB + wid;
+    }
+    if (w >= wg.waves) return;
+    const int zt = (int)(w % wg.nzt);
+    const long long w2 = w / wg.nzt;
+    const int yt = (int)(w2 % wg.nyt);
+    const int xc = (int)(w2 / wg.nyt);
+    const int z0 = zt * 64, y0 = R + yt * RY;
+    const int xb = R + xc * wg.xchunk;
+    const int xe = min(xb + wg.xchunk, g.nx - R);
+    const int z = z0 + lane;
+    const unsigned zc = (unsigned)min(z, g.nz - 1);
+    const unsigned zoff = zc * ES, zoffv = zc * 4u;
+    const bool zcomp = (z >= R) && (z < g.nz - R);
+
+    auto lat = [&](int i, int n) -> float {
+        const int j = i < tf.b ? tf.b - 1 - i : (i >= n - tf.b ? i - (n - tf.b) : -1);
+        return j >= 0 ? tab[max(j, 0)] : 1.0f;
+    };
+    const bool wtap = TAPER && (z0 - R < tf.b);  // wave holds some z < b (halo included)
+    const bool ztap = wtap && (z < tf.b);
+    float fzl = 1.0f;
+    if (ztap) fzl = tab[tf.b - 1 - z];
+
+    // own rows (uniform)
+    unsigned rowb[RY];
+    unsigned ymask = 0, rmask = 0;
+    float fyr[RY];
+#pragma unroll
+    for (int r = 0; r < RY; ++r) {
+        const int y = y0 + r;
+        const int yc = min(y, g.ny - 1);
+        rowb[r] = (unsigned)yc * (unsigned)g.sy * ES;
+        if (y < g.ny - R) ymask |= 1u << r;
+        if (IMG) {
+            const int dy = y - fz.ry0;
+            if (dy >= 0 && (dy % fz.rdy) == 0 && (dy / fz.rdy) < fz.rny) rmask |= 1u << r;
+        }
+        fyr[r] = 1.0f;
+        if (wtap) fyr[r] = __builtin_bit_cast(
+            float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, lat(yc, g.ny))));
+    }
+    // y-halo rows (uniform): h < R above the tile, h >= R below
+    unsigned yhb[2 * R];
+    float yhf[2 * R];
+#pragma unroll
+    for (int h = 0; h < 2 * R; ++h) {
+        const int gy = min(h < R ? y0 - R + h : y0 + RY + (h - R), g.ny - 1);
+        yhb[h] = (unsigned)gy * (unsigned)g.sy * ES;
+        yhf[h] = 1.0f;
+        if (wtap) yhf[h] = __builtin_bit_cast(
+            float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, lat(gy, g.ny))));
+    }
+    // this lane's z-halo cell: row zr, column zcol of the LDS row
+    const bool zh_on = lane < NZH * RY;
+    const int zr = min(lane / NZH, RY - 1), zk = lane % NZH;
+    const int zhz = zk < R ? z0 - R + zk : z0 + 64 + (zk - R);
+    const int zhzc = min(max(zhz, 0), g.nz - 1);
+    const int zcol = zk < R ? zk : 64 + zk;
+    const int zhy = min(y0 + zr, g.ny - 1);
+    const unsigned zhoff = ((unsigned)zhy * (unsigned)g.sy + (unsigned)zhzc) * ES;
+    const bool zhtap = wtap && zhzc < tf.b;
+    float zhfy = 1.0f, zhfz = 1.0f;
+    if (zhtap) {
+        zhfy = lat(zhy, g.ny);
+        zhfz = tab[tf.b - 1 - zhzc];
+    }
+
+    float q[NR][RY];
+    float yh[2][2 * R], zh[2];
+
+    auto plane_fx = [&](int x) -> float {
+        return wtap ? __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(
+                                                    __builtin_bit_cast(int, lat(x, g.nx))))
+                    : 1.0f;
+    };
+    // one in-place taper pass of the reference, rounded to the storage type;
+    // factors are exactly 1 where no taper applies
+    auto tap = [&](float v, float fx, float fy) -> float {
+        return wrnd<T>(taper1(v, ztap ? fx : 1.0f, ztap ? fy : 1.0f, fzl));
+    };
+    auto load_ring = [&](int x, float(&dst)[RY]) {
+        const auto rs = plane_rsrc(P + (size_t)x * (size_t)g.sx);
+#pragma unroll
+        for (int r = 0; r < RY; ++r) dst[r] = ld_buf<T>(rs, rowb[r], zoff);
+    };
+    auto taper_ring = [&](int x, float(&dst)[RY]) {
+        if (TAPER && wtap) {
+            const float fx = plane_fx(x);
+#pragma unroll
+            for (int r = 0; r < RY; ++r) dst[r] = tap(dst[r], fx, fyr[r]);
+        }
+    };
+    auto fetch_halo = [&](int x, int set) {
+        const auto rs = plane_rsrc(P + (size_t)x * (size_t)g.sx);
+#pragma unroll
+        for (int h = 0; h < 2 * R; ++h) yh[set][h] = ld_buf<T>(rs, yhb[h], zoff);
+        zh[set] = ld_buf<T>(rs, 0u, zhoff);
+    };
+    auto taper_halo = [&](int x, int set) {
+        if (TAPER && wtap) {
+            const float fx = plane_fx(x);
+#pragma unroll
+            for (int h = 0; h < 2 * R; ++h) yh[set][h] = tap(yh[set][h], fx, yhf[h]);
+            zh[set] = wrnd<T>(taper1(zh[set], zhtap ? fx : 1.0f, zhfy, zhfz));
+        }
+    };
+
+#pragma unroll
+    for (int s = 0; s <= 2 * R; ++s) load_ring(xb - R + s, q[s]);
+    fetch_halo(xb, 0);
+#pragma unroll
+    for (int s = 0; s < 2 * R; ++s) taper_ring(xb - R + s, q[s]);
+
+    float(*const lt)[LW] = tile[wid];
+    bool bad = false;
+    int x = xb;
+    auto step = [&]<int K>(std::integral_constant<int, K>) -> bool {
+        if (x >= xe) return false;
+        constexpr int cur = K & 1;
+        constexpr int ctr = (K + R) % NR;
+        taper_ring(x + R, q[(K + 2 * R) % NR]);
+        taper_halo(x, cur);
+        const size_t pl = (size_t)x * (size_t)g.sx;
+        const auto rpp = plane_rsrc(PP + pl);
+        const auto rv = plane_rsrc(V + pl);
+        const auto rps = plane_rsrc(IMG ? (const void *)(PS + pl) : (const void *)PP);
+        const auto rimg = plane_rsrc(IMG ? (const void *)(fz.img + pl) : (const void *)PP);
+        float c_pp[RY], c_v[RY], c_ps[RY], c_img[RY];
+#pragma unroll
+        for (int r = 0; r < RY; ++r) {
+            c_pp[r] = ld_buf<T>(rpp, rowb[r], zoff);
+            c_v[r] = ld_buf<float>(rv, rowb[r] * VS, zoffv);
+        }
+        if (IMG) {
+#pragma unroll
+            for (int r = 0; r < RY; ++r) {
+                c_ps[r] = ld_buf<T>(rps, rowb[r], zoff);
+                c_img[r] = ld_buf<float>(rimg, rowb[r] * VS, zoffv);
+            }
+        }
+        // stage the centre plane's rows and z halo in this wave's LDS rows
+        __builtin_amdgcn_wave_barrier();
+#pragma unroll
+        for (int r = 0; r < RY; ++r) lt[r][R + lane] = q[ctr][r];
+        if (zh_on) lt[zr][zcol] = zh[cur];
+        __builtin_amdgcn_wave_barrier();
+        float yc[2 * R];  // this plane's y-halo values (registers)
+#pragma unroll
+        for (int h = 0; h < 2 * R; ++h) yc[h] = yh[cur][h];
+        if (x + 1 < xe) {
+            load_ring(x + 1 + R, q[(K + 2 * R + 1) % NR]);
+            fetch_halo(x + 1, cur ^ 1);
+        }
+        bool rcvx = false;
+        if (IMG) {
+            const int dx = x - fz.rx0;
+            rcvx = dx >= 0 && (dx % fz.rdx) == 0 && (dx / fz.rdx) < fz.rnx;
+        }
+        float fxc = 1.0f;
+        if (TAPER) fxc = plane_fx(x);
+        float lapl[RY];
+#pragma unroll
+        for (int r = 0; r < RY; ++r) {
+            const float *t = &lt[r][R + lane];
+            const float p = q[ctr][r];
+            float ax = 0.f, ay = 0.f, az = 0.f;
+#pragma unroll
+            for (int io = 0; io <= 2 * R; ++io) {
+                const int ry = r + io - R;  // compile-time row of the y tap
+                const float vy = ry < 0 ? yc[ry + R] : (ry >= RY ? yc[R + ry - RY] : q[ctr][ry]);
+                const float vz = (io == R) ? p : t[io - R];
+                ax += q[(K + io) % NR][r] * cf.c[0][io];
+                ay += vy * cf.c[1][io];
+                az += vz * cf.c[2][io];
+            }
+            lapl[r] = ax + ay + az;
+            asm volatile("" : "+v"(lapl[r]));
+        }
+#pragma unroll
+        for (int r = 0; r < RY; ++r) {
+            const float p = q[ctr][r];
+            float pp = c_pp[r];
+            if (TAPER && wtap) {
+                pp = tap(pp, fxc, fyr[r]);
+                if (tf.pp_taps > 1) pp = tap(pp, fxc, fyr[r]);
+            }
+            const float npp = 2.f * p - pp + c_v[r] * lapl[r];
+            if (((ymask >> r) & 1u) && zcomp) {
+                st_buf<T>(rpp, rowb[r], zoff, npp);
+                bad |= (npp != npp);
+                if (IMG && (z >= fz.z0) && (z < fz.z1) &&
+                    !(rcvx && ((rmask >> r) & 1u) && z == fz.rz))
+                    st_buf<float>(rimg, rowb[r] * VS, zoffv, c_img[r] + c_ps[r] * wrnd<T>(npp));
+            }
+        }
+        ++x;
+        return true;
+    };
+    while (x < xe) {
+        [&]<int... Ks>(std::integer_sequence<int, Ks...>) {
+            (step(std::integral_constant<int, Ks>{}) && ...);
+        }(std::make_integer_sequence<int, NR>{});
+    }
+    if (bad) atomicOr(flag, 1);
+}
+
 struct LaunchKey {
     const void *fn;
     int threads;
@@ -777,6 +1032,46 @@ hipError_t launch_rows(const T *P, T *PP, const float *V, const Coefs &cf, const
     });
 }
 
+template <typename T, int R, int RY, int WPB, bool IMG, bool TAPER>
+hipError_t launch_wave_cfg(const T *P, T *PP, const float *V, const Coefs &cf, const Grid &g,
+                           const ImageFuse &fz, const TaperFuse &tf, int *flag, hipStream_t s,
+                           int xchunk)
+{
+    auto kern = k_stencil_wave<T, R, RY, WPB, IMG, TAPER>;
+    WaveGrid wg;
+    wg.nzt = (g.nz + 63) / 64;
+    wg.nyt = (g.ny - 2 * R + RY - 1) / RY;
+    const int nxi = g.nx - 2 * R;
+    if (wg.nyt <= 0 || nxi <= 0 || g.nz - 2 * R <= 0) return hipSuccess;
+    const size_t lds = TAPER ? (size_t)tf.b * sizeof(float) : 0;
+    if (xchunk <= 0) {
+        const int cap = device_capacity((const void *)kern, 64 * WPB, lds) * WPB;
+        const int n = choose_chunks((long long)wg.nzt * wg.nyt, nxi, R, cap);
+        xchunk = (nxi + n - 1) / n;
+    }
+    wg.xchunk = xchunk;
+    wg.nxc = (nxi + xchunk - 1) / xchunk;
+    wg.waves = (long long)wg.nzt * wg.nyt * wg.nxc;
+    const long long nb = (wg.waves + WPB - 1) / WPB;
+    hipLaunchKernelGGL(kern, dim3((unsigned)nb), dim3(64, WPB), lds, s, P, PP, V, cf, g, wg, fz,
+                       tf, tf.t, flag);
+    return hipGetLastError();
+}
+
+template <typename T, int R, int RY, int WPB>
+hipError_t launch_wave(const T *P, T *PP, const float *V, const Coefs &cf, const Grid &g,
+                       const ImageFuse *fuse, const TaperFuse *taper, int *flag, hipStream_t s,
+                       int xchunk)
+{
+    ImageFuse fz;
+    TaperFuse tf;
+    fuse_args(fuse, taper, fz, tf);
+    return by_variant(fuse, taper, [&](auto img, auto tap) {
+        return launch_wave_cfg<T, R, RY, WPB, decltype(img)::value, decltype(tap)::value>(
+            P, PP, V, cf, g, fz, tf, flag, s, xchunk);
+    });
+}
+
 // Full-row shapes compiled for radius 4 (by thread rows x ry rows each):
 // (4, 4) up to 4 waves per row (nz <= 256); (2, 6) and (2, 7) up to 6 waves
 // per row (nz <= 384).
@@ -798,6 +1093,15 @@ hipError_t launch_radius(const T *P, T *PP, const float *V, const Coefs &cf, con
 {
     constexpr bool full = std::is_same<T, float>::value;
     const int xchunk = t ? t->xchunk : 0;
+    if constexpr (R == 4 && std::is_same<T, float>::value) {
+        // barrier-free waves: 64 x 1 x ry
+        if (t && t->tz == 64 && t->by == 1 && (!taper || taper->b <= 8192)) {
+            if (t->ry == 6)
+                return launch_wave<T, R, 6, 4>(P, PP, V, cf, g, fuse, taper, flag, s, xchunk);
+            if (t->ry == 8)
+                return launch_wave<T, R, 8, 4>(P, PP, V, cf, g, fuse, taper, flag, s, xchunk);
+        }
+    }
     if constexpr (R == 4) {
         // full-row blocks by default whenever a compiled shape covers nz (and
         // the taper table fits in LDS beside the plane buffers)
@@ -849,7 +1153,8 @@ bool stencil_tiling_supported(int tz, int by, int ry)
 {
     return (tz == 64 && by == 4 && (ry == 4 || ry == 2)) || (tz == 64 && by == 8 && ry == 2) ||
            (tz == 32 && by == 8 && (ry == 4 || ry == 2)) ||
-           (tz == 1 && ((by == 4 && ry == 4) || (by == 2 && (ry == 6 || ry == 7))));
+           (tz == 1 && ((by == 4 && ry == 4) || (by == 2 && (ry == 6 || ry == 7)))) ||
+           (tz == 64 && by == 1 && (ry == 6 || ry == 8));
 }
 
 template <typename T>
