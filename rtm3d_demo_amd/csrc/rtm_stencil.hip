@@ -869,6 +869,280 @@ __global__ void __launch_bounds__(64 * WPB)
     if (bad) atomicOr(flag, 1);
 }
 
+// ---------------------------------------------------------------------------
+// Full-row variant for 16-bit wavefields with two z per lane.  A lane owns
+// z = 2*tx and 2*tx + 1, so every wavefield access moves one 32-bit word (a
+// pair of 16-bit values): with one 2-byte value per lane the memory system
+// streams at well under half its 4-byte rate.  The block layout is the
+// full-row one (BY thread rows x RY rows, 64 * NW lanes per row, NW =
+// ceil(nz / 128)); needs nz even (pairs never straddle a row end).
+template <typename T, int R, int BY, int RY, int NWM, bool IMG, bool TAPER>
+__global__ void __launch_bounds__(64 * NWM * BY)
+    k_stencil_rows2(const T *__restrict__ P, T *__restrict__ PP, const float *__restrict__ V,
+                    const Coefs cf, const Grid g, const RowGrid rg, const ImageFuse fz,
+                    const TaperFuse tf, const float *__restrict__ gt, int *__restrict__ flag)
+{
+    static_assert(sizeof(T) == 2, "16-bit wavefields");
+    static_assert(R % 2 == 0, "8-byte aligned LDS pairs");
+    constexpr int TY = BY * RY;
+    constexpr int LW = 128 * NWM + 2 * R;  // LDS row: z in [-R, 128*NWM + R)
+    constexpr int LH = TY + 2 * R;
+    constexpr int NH = 2 * R;
+    constexpr int NYH = (NH + BY - 1) / BY;
+    constexpr int NR = 2 * R + 2;
+    extern __shared__ float smem[];
+    const T *__restrict__ PS = static_cast<const T *>(fz.ps);
+
+    int by, bx;
+    row_block_coords(rg, by, bx);
+    const int tx = threadIdx.x;
+    const int ty = __builtin_amdgcn_readfirstlane(threadIdx.y);
+    const int wz0 = 2 * (__builtin_amdgcn_readfirstlane(threadIdx.x) & ~63);
+    const int y0 = R + by * TY;
+    const int xb = R + bx * rg.xchunk;
+    const int xe = min(xb + rg.xchunk, g.nx - R);
+    const int zl = 2 * tx;                          // first z of the pair
+    const unsigned zp = (unsigned)min(zl, g.nz - 2);  // clamped pair start
+    const unsigned zoff = zp * 2u;                  // wavefield byte offset
+    const unsigned zoffv = zp * 4u;                 // float byte offset
+    bool zcomp[2];
+#pragma unroll
+    for (int e = 0; e < 2; ++e) zcomp[e] = (zl + e >= R) && (zl + e < g.nz - R);
+
+    float *tab = smem + 2 * LH * LW;
+    if (TAPER) {
+        const int tid = threadIdx.y * blockDim.x + threadIdx.x;
+        for (int i = tid; i < tf.b; i += blockDim.x * blockDim.y) tab[i] = gt[i];
+        __syncthreads();
+    }
+    auto lat = [&](int i, int n) -> float {
+        const int j = i < tf.b ? tf.b - 1 - i : (i >= n - tf.b ? i - (n - tf.b) : -1);
+        const float v = j >= 0 ? tab[max(j, 0)] : 1.0f;
+        return __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, v)));
+    };
+    const bool wtap = TAPER && (wz0 < tf.b);
+    bool ztap[2];
+    float fzl[2];
+#pragma unroll
+    for (int e = 0; e < 2; ++e) {
+        ztap[e] = wtap && (zl + e < tf.b);
+        fzl[e] = ztap[e] ? tab[tf.b - 1 - (zl + e)] : 1.0f;
+    }
+
+    unsigned rowb[RY];  // wavefield row byte offsets (uniform)
+    unsigned ymask = 0, rmask = 0;
+    float fyr[RY];
+#pragma unroll
+    for (int r = 0; r < RY; ++r) {
+        const int y = y0 + ty + BY * r;
+        const int yc = min(y, g.ny - 1);
+        rowb[r] = (unsigned)yc * (unsigned)g.sy * 2u;
+        if (y < g.ny - R) ymask |= 1u << r;
+        if (IMG) {
+            const int dy = y - fz.ry0;
+            if (dy >= 0 && (dy % fz.rdy) == 0 && (dy / fz.rdy) < fz.rny) rmask |= 1u << r;
+        }
+        fyr[r] = 1.0f;
+        if (wtap) fyr[r] = lat(yc, g.ny);
+    }
+    unsigned yh_b[NYH];
+    int yh_lrow[NYH];
+    bool yh_valid[NYH];
+    float yh_fy[NYH];
+#pragma unroll
+    for (int i = 0; i < NYH; ++i) {
+        const int h = ty + BY * i;
+        yh_valid[i] = h < NH;
+        const int gy = min(h < R ? y0 - R + h : y0 + TY + (h - R), g.ny - 1);
+        yh_lrow[i] = h < R ? h : R + TY + (h - R);
+        yh_b[i] = (unsigned)gy * (unsigned)g.sy * 2u;
+        yh_fy[i] = 1.0f;
+        if (wtap) yh_fy[i] = lat(gy, g.ny);
+    }
+
+    // pair load / store of the wavefield, widened to / rounded from float
+    auto ld2 = [&](__amdgpu_buffer_rsrc_t rs, unsigned soff, float(&d)[2]) {
+        const unsigned w = __builtin_amdgcn_raw_buffer_load_b32(rs, (int)zoff, (int)soff, 0);
+        d[0] = wld(__builtin_bit_cast(T, (unsigned short)(w & 0xffffu)));
+        d[1] = wld(__builtin_bit_cast(T, (unsigned short)(w >> 16)));
+    };
+    auto ldf2 = [&](__amdgpu_buffer_rsrc_t rs, unsigned soff, float(&d)[2]) {
+        d[0] = ld_buf<float>(rs, soff, zoffv);
+        d[1] = ld_buf<float>(rs, soff, zoffv + 4u);
+    };
+
+    float q[NR][RY][2];
+    float n_yh[2][NYH][2];
+
+    auto plane_fx = [&](int x) -> float { return wtap ? lat(x, g.nx) : 1.0f; };
+    auto tap = [&](float v, int e, float fx, float fy) -> float {
+        return wrnd<T>(taper1(v, ztap[e] ? fx : 1.0f, ztap[e] ? fy : 1.0f, fzl[e]));
+    };
+    auto load_ring = [&](int x, float(&dst)[RY][2]) {
+        const auto rs = plane_rsrc(P + (size_t)x * (size_t)g.sx);
+#pragma unroll
+        for (int r = 0; r < RY; ++r) ld2(rs, rowb[r], dst[r]);
+    };
+    auto taper_ring = [&](int x, float(&dst)[RY][2]) {
+        if (TAPER && wtap) {
+            const float fx = plane_fx(x);
+#pragma unroll
+            for (int r = 0; r < RY; ++r)
+#pragma unroll
+                for (int e = 0; e < 2; ++e) dst[r][e] = tap(dst[r][e], e, fx, fyr[r]);
+        }
+    };
+    auto fetch_halo = [&](int x, int set) {
+        const auto rs = plane_rsrc(P + (size_t)x * (size_t)g.sx);
+#pragma unroll
+        for (int i = 0; i < NYH; ++i) {
+            if (yh_valid[i]) {
+                ld2(rs, yh_b[i], n_yh[set][i]);
+            } else {
+                n_yh[set][i][0] = 0.f;
+                n_yh[set][i][1] = 0.f;
+            }
+        }
+    };
+    auto taper_halo = [&](int x, int set) {
+        if (TAPER && wtap) {
+            const float fx = plane_fx(x);
+#pragma unroll
+            for (int i = 0; i < NYH; ++i)
+#pragma unroll
+                for (int e = 0; e < 2; ++e)
+                    n_yh[set][i][e] = tap(n_yh[set][i][e], e, fx, yh_fy[i]);
+        }
+    };
+
+#pragma unroll
+    for (int s = 0; s <= 2 * R; ++s) load_ring(xb - R + s, q[s]);
+    fetch_halo(xb, 0);
+#pragma unroll
+    for (int s = 0; s < 2 * R; ++s) taper_ring(xb - R + s, q[s]);
+
+    const int lbase = (R + ty) * LW + R + zl;  // first element of the pair, first row
+    bool bad = false;
+    int x = xb;
+    auto step = [&]<int K>(std::integral_constant<int, K>) -> bool {
+        if (x >= xe) return false;
+        constexpr int cur = K & 1;
+        constexpr int ctr = (K + R) % NR;
+        taper_ring(x + R, q[(K + 2 * R) % NR]);
+        taper_halo(x, cur);
+        float *tl = smem + cur * (LH * LW);
+        const size_t pl = (size_t)x * (size_t)g.sx;
+        const auto rpp = plane_rsrc(PP + pl);
+        const auto rv = plane_rsrc(V + pl);
+        const auto rps = plane_rsrc(IMG ? (const void *)(PS + pl) : (const void *)PP);
+        const auto rimg = plane_rsrc(IMG ? (const void *)(fz.img + pl) : (const void *)PP);
+        float c_pp[RY][2], c_v[RY][2], c_ps[RY][2], c_img[RY][2];
+#pragma unroll
+        for (int r = 0; r < RY; ++r) {
+            ld2(rpp, rowb[r], c_pp[r]);
+            ldf2(rv, rowb[r] * 2u, c_v[r]);
+        }
+        if (IMG) {
+#pragma unroll
+            for (int r = 0; r < RY; ++r) {
+                ld2(rps, rowb[r], c_ps[r]);
+                ldf2(rimg, rowb[r] * 2u, c_img[r]);
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < RY; ++r) {
+            float2 v2v;
+            v2v.x = q[ctr][r][0];
+            v2v.y = q[ctr][r][1];
+            *reinterpret_cast<float2 *>(&tl[lbase + BY * r * LW]) = v2v;
+        }
+#pragma unroll
+        for (int i = 0; i < NYH; ++i)
+            if (yh_valid[i]) {
+                float2 v2v;
+                v2v.x = n_yh[cur][i][0];
+                v2v.y = n_yh[cur][i][1];
+                *reinterpret_cast<float2 *>(&tl[yh_lrow[i] * LW + R + zl]) = v2v;
+            }
+        if (x + 1 < xe) {
+            load_ring(x + 1 + R, q[(K + 2 * R + 1) % NR]);
+            fetch_halo(x + 1, cur ^ 1);
+        }
+        __syncthreads();
+
+        bool rcvx = false;
+        if (IMG) {
+            const int dx = x - fz.rx0;
+            rcvx = dx >= 0 && (dx % fz.rdx) == 0 && (dx / fz.rdx) < fz.rnx;
+        }
+        float fxc = 1.0f;
+        if (TAPER) fxc = plane_fx(x);
+        float lapl[RY][2];
+#pragma unroll
+        for (int r = 0; r < RY; ++r)
+#pragma unroll
+            for (int e = 0; e < 2; ++e) {
+                const float *t = tl + lbase + BY * r * LW + e;
+                const float p = q[ctr][r][e];
+                float ax = 0.f, ay = 0.f, az = 0.f;
+#pragma unroll
+                for (int io = 0; io <= 2 * R; ++io) {
+                    const float vy = (io == R) ? p : t[(io - R) * LW];
+                    const float vz = (io == R) ? p : t[io - R];
+                    ax += q[(K + io) % NR][r][e] * cf.c[0][io];
+                    ay += vy * cf.c[1][io];
+                    az += vz * cf.c[2][io];
+                }
+                lapl[r][e] = ax + ay + az;
+                asm volatile("" : "+v"(lapl[r][e]));
+            }
+#pragma unroll
+        for (int r = 0; r < RY; ++r) {
+            float npp[2];
+#pragma unroll
+            for (int e = 0; e < 2; ++e) {
+                float pp = c_pp[r][e];
+                if (TAPER && wtap) {
+                    pp = tap(pp, e, fxc, fyr[r]);
+                    if (tf.pp_taps > 1) pp = tap(pp, e, fxc, fyr[r]);
+                }
+                npp[e] = 2.f * q[ctr][r][e] - pp + c_v[r][e] * lapl[r][e];
+            }
+            if ((ymask >> r) & 1u) {
+                const unsigned short b0 = __builtin_bit_cast(unsigned short, wst<T>(npp[0]));
+                const unsigned short b1 = __builtin_bit_cast(unsigned short, wst<T>(npp[1]));
+                if (zcomp[0] && zcomp[1]) {
+                    __builtin_amdgcn_raw_buffer_store_b32((unsigned)b0 | ((unsigned)b1 << 16), rpp,
+                                                          (int)zoff, (int)rowb[r], 0);
+                } else if (zcomp[0]) {
+                    __builtin_amdgcn_raw_buffer_store_b16(b0, rpp, (int)zoff, (int)rowb[r], 0);
+                } else if (zcomp[1]) {
+                    __builtin_amdgcn_raw_buffer_store_b16(b1, rpp, (int)(zoff + 2u), (int)rowb[r],
+                                                          0);
+                }
+#pragma unroll
+                for (int e = 0; e < 2; ++e) {
+                    if (!zcomp[e]) continue;
+                    bad |= (npp[e] != npp[e]);
+                    const int ze = zl + e;
+                    if (IMG && ze >= fz.z0 && ze < fz.z1 &&
+                        !(rcvx && ((rmask >> r) & 1u) && ze == fz.rz))
+                        st_buf<float>(rimg, rowb[r] * 2u, zoffv + 4u * e,
+                                      c_img[r][e] + c_ps[r][e] * wrnd<T>(npp[e]));
+                }
+            }
+        }
+        ++x;
+        return true;
+    };
+    while (x < xe) {
+        [&]<int... Ks>(std::integer_sequence<int, Ks...>) {
+            (step(std::integral_constant<int, Ks>{}) && ...);
+        }(std::make_integer_sequence<int, NR>{});
+    }
+    if (bad) atomicOr(flag, 1);
+}
+
 struct LaunchKey {
     const void *fn;
     int threads;
@@ -1032,6 +1306,48 @@ hipError_t launch_rows(const T *P, T *PP, const float *V, const Coefs &cf, const
     });
 }
 
+template <typename T, int R, int BY, int RY, int NWM, bool IMG, bool TAPER>
+hipError_t launch_rows2_cfg(const T *P, T *PP, const float *V, const Coefs &cf, const Grid &g,
+                            const ImageFuse &fz, const TaperFuse &tf, int *flag, hipStream_t s,
+                            int xchunk)
+{
+    constexpr int TY = BY * RY;
+    auto kern = k_stencil_rows2<T, R, BY, RY, NWM, IMG, TAPER>;
+    const int nw = (g.nz + 127) / 128;
+    if (nw > NWM || (g.nz & 1)) return hipErrorInvalidConfiguration;
+    RowGrid rg;
+    rg.nty = (g.ny - 2 * R + TY - 1) / TY;
+    const int nxi = g.nx - 2 * R;
+    if (rg.nty <= 0 || nxi <= 0 || g.nz - 2 * R <= 0) return hipSuccess;
+    const size_t lds = 2u * (size_t)(TY + 2 * R) * (size_t)(128 * NWM + 2 * R) * sizeof(float) +
+                       (TAPER ? (size_t)tf.b * sizeof(float) : 0);
+    const int threads = 64 * nw * BY;
+    const int cap = device_capacity((const void *)kern, threads, lds);
+    if (xchunk <= 0) {
+        const int n = choose_chunks(rg.nty, nxi, R, cap);
+        xchunk = (nxi + n - 1) / n;
+    }
+    rg.xchunk = xchunk;
+    rg.nxc = (nxi + xchunk - 1) / xchunk;
+    hipLaunchKernelGGL(kern, dim3((unsigned)(rg.nty * rg.nxc)), dim3(64 * nw, BY), lds, s, P, PP,
+                       V, cf, g, rg, fz, tf, tf.t, flag);
+    return hipGetLastError();
+}
+
+template <typename T, int R, int BY, int RY, int NWM>
+hipError_t launch_rows2(const T *P, T *PP, const float *V, const Coefs &cf, const Grid &g,
+                        const ImageFuse *fuse, const TaperFuse *taper, int *flag, hipStream_t s,
+                        int xchunk)
+{
+    ImageFuse fz;
+    TaperFuse tf;
+    fuse_args(fuse, taper, fz, tf);
+    return by_variant(fuse, taper, [&](auto img, auto tap) {
+        return launch_rows2_cfg<T, R, BY, RY, NWM, decltype(img)::value, decltype(tap)::value>(
+            P, PP, V, cf, g, fz, tf, flag, s, xchunk);
+    });
+}
+
 template <typename T, int R, int RY, int WPB, bool IMG, bool TAPER>
 hipError_t launch_wave_cfg(const T *P, T *PP, const float *V, const Coefs &cf, const Grid &g,
                            const ImageFuse &fz, const TaperFuse &tf, int *flag, hipStream_t s,
@@ -1101,6 +1417,12 @@ hipError_t launch_radius(const T *P, T *PP, const float *V, const Coefs &cf, con
             if (t->ry == 8)
                 return launch_wave<T, R, 8, 4>(P, PP, V, cf, g, fuse, taper, flag, s, xchunk);
         }
+    }
+    if constexpr (R == 4 && !std::is_same<T, float>::value) {
+        // 16-bit wavefields: full rows with a pair of z per lane (32-bit
+        // accesses) whenever nz is even and at most 384
+        if ((g.nz & 1) == 0 && (g.nz + 127) / 128 <= 3 && (!taper || taper->b <= 4096))
+            return launch_rows2<T, R, 4, 3, 3>(P, PP, V, cf, g, fuse, taper, flag, s, xchunk);
     }
     if constexpr (R == 4) {
         // full-row blocks by default whenever a compiled shape covers nz (and

@@ -14,6 +14,8 @@ Fields hold random values; counters do not depend on them.
 
     python3 rtm3d_demo_amd/profile_stencil.py --summarize FETCH_DIR WRITE_DIR
 
+(--tiling TZ,BY,RY as the only arguments runs the plain step with that shape).
+
 turns the counter_collection.csv files of a FETCH_SIZE pass and a WRITE_SIZE
 pass into per-kernel bytes per dispatch.  Each counter is scaled by what it
 reads for k_stack, whose true traffic is known (8 B read + 4 B written per
@@ -97,10 +99,15 @@ def main():
         summarize(sys.argv[2], sys.argv[3])
         return 0
     from rtm3d_demo_amd import hip, host
+    tiling = None
+    if len(sys.argv) == 3 and sys.argv[1] == "--tiling":  # e.g. --tiling 64,1,8 (plain step)
+        tiling = tuple(int(v) for v in sys.argv[2].split(","))
 
     shape = SHAPE
     b, order, nt = 40, 8, 16
     hip.init(0)
+    if tiling:
+        hip.set_tiling(0, *tiling, 0)
     rng = np.random.default_rng(0)
 
     def rand(lo, hi):
