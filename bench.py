@@ -112,16 +112,20 @@ class Shot:
         for f in (F.a, F.b, R.a, R.b, self.img):
             f.zero()
         hip.forward_steps(shot, True, self.V, self.coefs, self.g, self.src, F, upb, 0, self.nt)
+        # mode 1: fused receiver step (the one-pass source + receiver mode 2
+        # is bit-identical but, with its 4-row tiles, not faster here)
         hip.backward_steps(shot, True, self.V, self.coefs, self.g, F, R, upb, self.rcv, self.img,
-                           self.nt - 1, self.nt, True)
+                           self.nt - 1, self.nt, 1)
 
 
-def read_traffic():
-    """HBM bytes per stencil launch from the committed rocprofv3 PMC summary."""
+def read_traffic(cls):
+    """HBM bytes per launch of a stencil kernel class, from the committed
+    rocprofv3 PMC summary ("traffic_bytes_per_launch[class N]: bytes")."""
     path = os.path.join(REPO, "profiles", "stencil_salt512_pmc.md")
     if not os.path.exists(path):
         return None
-    m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", open(path).read())
+    m = re.search(r"traffic_bytes_per_launch\[class %d\]:\s*([0-9.eE+]+)" % cls,
+                  open(path).read())
     return float(m.group(1)) if m else None
 
 
@@ -264,6 +268,7 @@ def main():
     n0, ms0 = hip.timing_read(0)
     n1, ms1 = hip.timing_read(1)
     n2, ms2 = hip.timing_read(2)
+    n3, ms3 = hip.timing_read(3)
     hip.timing_enable(False)
     img = sh.img.download()
     if not np.isfinite(img).all() or not np.abs(img).max() > 0:
@@ -273,12 +278,23 @@ def main():
     shot_s = (t1 - t0) / args.steps
     updates = sh.U * (3 * args.nt - 2)
     value = updates / shot_s / 1e6
-    bytes_per_launch = 16.0 * sh.U
-    avg0 = ms0 / max(n0, 1) * 1e-3
-    avg1 = ms1 / max(n1, 1) * 1e-3
-    achieved = bytes_per_launch / avg0 / 1e9
-    achieved_fused = (bytes_per_launch + 12.0 * sh.U) / avg1 / 1e9 if n1 else None
-    traffic = read_traffic()
+    # stencil kernel classes in the timed region: launches, total ms and
+    # algorithmic bytes per updated cell
+    classes = {0: ("plain step", n0, ms0, 16.0), 1: ("fused receiver step", n1, ms1, 28.0),
+               2: ("tapered forward step", n2, ms2, 16.0),
+               3: ("dual backward step (source + receiver + image)", n3, ms3, 36.0)}
+
+    def cls_stats(c):
+        name, n, ms, bpc = classes[c]
+        avg = ms / n * 1e-3 if n else None
+        return {"kernel": name, "launches": n, "avg_ms": round(avg * 1e3, 4) if n else None,
+                "achieved_GBs": round(bpc * sh.U / avg / 1e9, 1) if n else None,
+                "algorithmic_bytes_per_launch": bpc * sh.U}
+
+    # roofline: the class with the most kernel time in the timed region
+    dom = max(classes, key=lambda c: classes[c][2])
+    dstat = cls_stats(dom)
+    achieved = dstat["achieved_GBs"]
     line = {
         "metric": METRIC, "value": round(value, 1), "unit": "Mcell-updates/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -290,19 +306,13 @@ def main():
                    "shots_per_step": 1, "parallelism": "single_gpu"},
         "shots_per_hour": round(3600.0 / shot_s, 1),
         "modeling_mcell_updates_per_s": round(sh.U * args.nt / sh.modeling_s / 1e6, 1),
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": traffic, "kernel": "k_stencil (plain step)",
-                     "launches": n0, "avg_ms": round(avg0 * 1e3, 4),
-                     "algorithmic_bytes_per_launch": bytes_per_launch},
-        "fused_receiver_step": {"launches": n1, "avg_ms": round(avg1 * 1e3, 4),
-                                "achieved_GBs": round(achieved_fused, 1) if achieved_fused else None,
-                                "algorithmic_bytes_per_launch": 28.0 * sh.U},
-        "tapered_forward_step": {"launches": n2,
-                                 "avg_ms": round(ms2 / max(n2, 1), 4),
-                                 "achieved_GBs": round(bytes_per_launch / (ms2 / max(n2, 1) * 1e-3)
-                                                       / 1e9, 1) if n2 else None,
-                                 "algorithmic_bytes_per_launch": bytes_per_launch},
+                     "traffic": read_traffic(dom), "kernel": dstat["kernel"],
+                     "kernel_class": dom, "launches": dstat["launches"],
+                     "avg_ms": dstat["avg_ms"],
+                     "algorithmic_bytes_per_launch": dstat["algorithmic_bytes_per_launch"]},
+        "stencil_classes": {str(c): cls_stats(c) for c in classes if classes[c][1]},
         "tiling": tiling,
     }
     # 16-bit wavefield storage: the same shot, one warm-up and one timed shot
@@ -321,6 +331,7 @@ def main():
             k0, m0 = hip.timing_read(0)
             hip.timing_read(1)
             hip.timing_read(2)
+            hip.timing_read(3)
             hip.timing_enable(False)
             sh.release(fmt)
             a0 = m0 / max(k0, 1) * 1e-3

@@ -184,7 +184,10 @@ int rtmhip_forward_steps(const rtmhip_shot *shot, int hbc, const float *V2DT2,
  * call (it_hi = nt-1) *Ps and *PPs must be the forward's final *P and *PP
  * (the reference's snap0/snap1); *Pr, *PPr and img must be zero.  rcv_chunk
  * holds the chunk containing it_hi.  With fused != 0 the receiver step,
- * receiver injection and imaging condition run as one pass over the grid. */
+ * receiver injection and imaging condition run as one pass; with fused == 2
+ * (FP32 wavefields, order 8) the source field's backward step joins that
+ * pass for every it < nt-2, so V is read once and the image takes the new
+ * source value from registers.  Results are identical in all modes. */
 int rtmhip_backward_steps(const rtmhip_shot *shot, int hbc, const float *V2DT2,
                           const float *coefs, const float *taper, float **Ps, float **PPs,
                           float **Pr, float **PPr, const float *upb, const float *rcv_chunk,
@@ -193,7 +196,8 @@ int rtmhip_backward_steps(const rtmhip_shot *shot, int hbc, const float *V2DT2,
 /* ================= tuning and measurement =================
  * Stencil kernel classes: 0 = plain step (rtmhip_step, backward source
  * reconstruction, RBC), 1 = receiver step fused with receiver injection and
- * imaging, 2 = HBC forward step with the upper taper fused.  No reference
+ * imaging, 2 = HBC forward step with the upper taper fused, 3 = source and
+ * receiver backward steps in one pass (fused == 2; shape 64x1x4).  No reference
  * counterpart: the reference's launch shape is fixed (10x10x10 blocks,
  * RTMGPUPlatform.cpp:176-185).
  *

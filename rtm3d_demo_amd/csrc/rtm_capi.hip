@@ -29,8 +29,8 @@ struct State {
     hipStream_t stream = nullptr;
     int *flag = nullptr;  // NaN ("numerical dispersion") flag written by the stencil
     bool timing = false;
-    Timing tm[3];
-    StencilTiling tiling[3] = {};  // per kernel class; tz == 0: built-in default
+    Timing tm[4];
+    StencilTiling tiling[4] = {};  // per kernel class; tz == 0: built-in default
 };
 
 State g_state;
@@ -105,12 +105,11 @@ int check_rcv(const rtmhip_rcv_geom *g, int nze, int blen)
     return RTMHIP_OK;
 }
 
-template <typename T>
-hipError_t stencil(int radius, const T *P, T *PP, const float *V, const Coefs &cf, const Grid &g,
-                   const ImageFuse *fuse, const TaperFuse *taper = nullptr)
+// Runs launch() on the library stream, bracketed by HIP events of timing
+// class cls when timing is on.
+template <typename L>
+hipError_t timed(int cls, L &&launch)
 {
-    // kernel class: 0 plain step, 1 fused receiver step, 2 tapered HBC step
-    const int cls = fuse ? 1 : (taper ? 2 : 0);
     Timing *tm = g_state.timing ? &g_state.tm[cls] : nullptr;
     if (tm) {
         if (tm->used == tm->start.size()) {
@@ -125,9 +124,7 @@ hipError_t stencil(int radius, const T *P, T *PP, const float *V, const Coefs &c
         hipError_t e = hipEventRecord(tm->start[tm->used], g_state.stream);
         if (e != hipSuccess) return e;
     }
-    const StencilTiling *tl = g_state.tiling[cls].tz ? &g_state.tiling[cls] : nullptr;
-    hipError_t e = launch_stencil(radius, P, PP, V, cf, g, fuse, taper, g_state.flag,
-                                  g_state.stream, tl);
+    hipError_t e = launch();
     if (e != hipSuccess) return e;
     if (tm) {
         e = hipEventRecord(tm->stop[tm->used], g_state.stream);
@@ -135,6 +132,23 @@ hipError_t stencil(int radius, const T *P, T *PP, const float *V, const Coefs &c
         tm->used++;
     }
     return hipSuccess;
+}
+
+const StencilTiling *class_tiling(int cls)
+{
+    return g_state.tiling[cls].tz ? &g_state.tiling[cls] : nullptr;
+}
+
+template <typename T>
+hipError_t stencil(int radius, const T *P, T *PP, const float *V, const Coefs &cf, const Grid &g,
+                   const ImageFuse *fuse, const TaperFuse *taper = nullptr)
+{
+    // kernel class: 0 plain step, 1 fused receiver step, 2 tapered HBC step
+    const int cls = fuse ? 1 : (taper ? 2 : 0);
+    return timed(cls, [&] {
+        return launch_stencil(radius, P, PP, V, cf, g, fuse, taper, g_state.flag,
+                              g_state.stream, class_tiling(cls));
+    });
 }
 
 int check_dispersion()
@@ -521,7 +535,54 @@ int backward_loop(const rtmhip_shot *shot, const ShotCtx &c, int hbc, const floa
     hipStream_t s = g_state.stream;
     const int nt = shot->nt, b = shot->blen;
     const int z0 = b, z1 = shot->nze - b;
+    const bool dual = fused == 2 && std::is_same<T, float>::value &&
+                      dual_step_supported(c.radius, shot->wave_format);
     for (int it = it_hi; it > it_hi - nsteps; --it) {
+        const int lt = it % shot->ntstep;
+        if constexpr (std::is_same<T, float>::value) {
+            if (dual && it < nt - 2) {
+                // both backward steps in one pass (class 3), then the UPB
+                // restore of the source field, its swap, and the receiver
+                // field's frame taper, injection and imaging as below
+                const TaperFuse tf{taper, b, it == it_hi ? 1 : 2};
+                ImageFuse fz{};
+                fz.img = img;
+                fz.z0 = z0;
+                fz.z1 = z1;
+                fz.rz = c.rz;
+                fz.rx0 = c.rx0;
+                fz.rdx = shot->rcv.dist_x;
+                fz.rnx = shot->rcv.count_x;
+                fz.ry0 = c.ry0;
+                fz.rdy = shot->rcv.dist_y;
+                fz.rny = shot->rcv.count_y;
+                HIPCHK(timed(3, [&] {
+                           return launch_stencil_dual(c.radius, *Ps, *PPs, *Pr, *PPr, V2DT2, c.cf,
+                                                      c.g, fz, hbc ? &tf : nullptr, g_state.flag,
+                                                      s, class_tiling(3));
+                       }),
+                       "dual stencil");
+                if (hbc)
+                    HIPCHK(launch_upb(*PPs, const_cast<T *>(upb), c.g, b, c.hf, it, true, s),
+                           "upb restore");
+                T *t = *Ps;
+                *Ps = *PPs;
+                *PPs = t;
+                if (hbc)
+                    HIPCHK(launch_taper_frame(*PPr, taper, c.g, b, c.radius, tf.pp_taps, s),
+                           "frame taper");
+                HIPCHK(launch_receivers_image(*PPr, rcv_chunk, c.g, c.rx0, c.ry0, c.rz,
+                                              shot->rcv.dist_x, shot->rcv.dist_y,
+                                              shot->rcv.count_x, shot->rcv.count_y,
+                                              shot->ntstep, lt, img, (const T *)*Ps, z0, z1,
+                                              c.radius, s),
+                       "receivers+image");
+                t = *Pr;
+                *Pr = *PPr;
+                *PPr = t;
+                continue;
+            }
+        }
         // Source wavefield at time it (RTMFiniteDifferencesKernel_HBC.cpp:283-309).
         const T *S;
         if (it == nt - 1) {
@@ -543,7 +604,6 @@ int backward_loop(const rtmhip_shot *shot, const ShotCtx &c, int hbc, const floa
             HIPCHK(launch_taper(*Pr, taper, c.g, b, true, s), "taper");
             HIPCHK(launch_taper(*PPr, taper, c.g, b, true, s), "taper");
         }
-        const int lt = it % shot->ntstep;
         if (fused) {
             // tapers fused as in the forward loop; finishing taper below
             const TaperFuse tf{taper, b, it == it_hi ? 1 : 2};
@@ -657,13 +717,15 @@ int rtmhip_backward_steps(const rtmhip_shot *shot, int hbc, const float *V2DT2,
 int rtmhip_set_tiling(int kernel_class, int tz, int by, int ry, int xchunk)
 {
     NEED_INIT();
-    if (kernel_class < 0 || kernel_class > 2)
-        return fail(RTMHIP_ERR_INVALID, "kernel class %d not in 0..2", kernel_class);
+    if (kernel_class < 0 || kernel_class > 3)
+        return fail(RTMHIP_ERR_INVALID, "kernel class %d not in 0..3", kernel_class);
     if (tz == 0) {
         g_state.tiling[kernel_class] = StencilTiling{};
         return RTMHIP_OK;
     }
-    if (!stencil_tiling_supported(tz, by, ry) || xchunk < 0)
+    const bool ok = kernel_class == 3 ? (tz == 64 && by == 1 && ry == 4)
+                                      : stencil_tiling_supported(tz, by, ry);
+    if (!ok || xchunk < 0)
         return fail(RTMHIP_ERR_UNSUPPORTED, "tile shape %dx%dx%d not compiled", tz, by, ry);
     g_state.tiling[kernel_class] = StencilTiling{tz, by, ry, xchunk};
     return RTMHIP_OK;
@@ -672,7 +734,7 @@ int rtmhip_set_tiling(int kernel_class, int tz, int by, int ry, int xchunk)
 int rtmhip_get_tiling(int kernel_class, int *tz, int *by, int *ry, int *xchunk)
 {
     NEED_INIT();
-    if (kernel_class < 0 || kernel_class > 2 || !tz || !by || !ry || !xchunk)
+    if (kernel_class < 0 || kernel_class > 3 || !tz || !by || !ry || !xchunk)
         return fail(RTMHIP_ERR_INVALID, "bad arguments");
     const StencilTiling &t = g_state.tiling[kernel_class];
     *tz = t.tz;
@@ -692,7 +754,7 @@ int rtmhip_timing_enable(int on)
 int rtmhip_timing_read(int kernel_class, long long *launches, double *total_ms)
 {
     NEED_INIT();
-    if (kernel_class < 0 || kernel_class > 2 || !launches || !total_ms)
+    if (kernel_class < 0 || kernel_class > 3 || !launches || !total_ms)
         return fail(RTMHIP_ERR_INVALID, "bad arguments");
     HIPCHK(hipStreamSynchronize(g_state.stream), "hipStreamSynchronize");
     Timing &t = g_state.tm[kernel_class];

@@ -89,6 +89,14 @@ template <typename T>
 hipError_t launch_stencil(int radius, const T *P, T *PP, const float *V, const Coefs &cf,
                           const Grid &g, const ImageFuse *fuse, const TaperFuse *taper,
                           int *flag, hipStream_t s, const StencilTiling *tiling);
+// Backward step of the source and the receiver field in one pass (FP32,
+// radius 4): source plain step into PPs, receiver step (tapered when taper
+// is given) into PPr, img += Ps_new * PPr_new off the receiver cells.
+bool dual_step_supported(int radius, int wave_format);
+hipError_t launch_stencil_dual(int radius, const float *Ps, float *PPs, const float *Pr,
+                               float *PPr, const float *V, const Coefs &cf, const Grid &g,
+                               const ImageFuse &fuse, const TaperFuse *taper, int *flag,
+                               hipStream_t s, const StencilTiling *tiling);
 template <typename T>
 hipError_t launch_taper(T *P, const float *taper, const Grid &g, int blen, bool upper_only,
                         hipStream_t s);

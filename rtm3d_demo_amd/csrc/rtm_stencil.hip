@@ -1143,6 +1143,267 @@ __global__ void __launch_bounds__(64 * NWM * BY)
     if (bad) atomicOr(flag, 1);
 }
 
+// ---------------------------------------------------------------------------
+// Backward step of both wavefields in one pass (barrier-free waves).  At
+// time it the reference steps the source field back (RTMHBCKernel backward:
+// step, UPB restore, swap), then tapers and steps the receiver field,
+// injects the receivers and images img += Ps * PPr on z in [blen,
+// nze-blen).  Both updates march the same x planes here: V is read once and
+// the image takes the new source value from registers instead of reading it
+// back (36 instead of 44 bytes per cell).  The UPB restore rewrites only
+// z < blen, which the image never reads, so it runs after this kernel.
+// Source: plain step.  Receiver: TAPER (HBC) as in the tapered variants.
+// Receiver cells are imaged by the receiver kernel after injection.
+template <typename T, int R, int RY, int WPB, bool TAPER>
+__global__ void __launch_bounds__(64 * WPB)
+    k_stencil_dual(const T *__restrict__ Ps, T *__restrict__ PPs, const T *__restrict__ Pr,
+                   T *__restrict__ PPr, const float *__restrict__ V, const Coefs cf,
+                   const Grid g, const WaveGrid wg, const ImageFuse fz, const TaperFuse tf,
+                   const float *__restrict__ gt, int *__restrict__ flag)
+{
+    constexpr int NR = 2 * R + 2;
+    constexpr int LW = 64 + 2 * R;
+    constexpr int NZH = 2 * R;
+    static_assert(NZH * RY <= 64, "one z-halo load per lane and field");
+    constexpr unsigned ES = sizeof(T);
+    constexpr unsigned VS = 4u / sizeof(T);
+    __shared__ float tile[WPB][2][RY][LW];  // [wave][field][row][z]
+    extern __shared__ float tab[];
+
+    const int lane = threadIdx.x;
+    const int wid = __builtin_amdgcn_readfirstlane(threadIdx.y);
+    if (TAPER) {
+        const int tid = threadIdx.y * 64 + lane;
+        for (int i = tid; i < tf.b; i += 64 * WPB) tab[i] = gt[i];
+        __syncthreads();
+    }
+    long long w;
+    {
+        const long long nb = (wg.waves + WPB - 1) / WPB;
+        const long long h = blockIdx.x;
+        const long long q = nb >> 3, rr = nb & 7, k = h & 7, j = h >> 3;
+        const long long t = k * q + (k < rr ? k : rr) + j;
+        w = t * WPB + wid;
+    }
+    if (w >= wg.waves) return;
+    const int zt = (int)(w % wg.nzt);
+    const long long w2 = w / wg.nzt;
+    const int yt = (int)(w2 % wg.nyt);
+    const int xc = (int)(w2 / wg.nyt);
+    const int z0 = zt * 64, y0 = R + yt * RY;
+    const int xb = R + xc * wg.xchunk;
+    const int xe = min(xb + wg.xchunk, g.nx - R);
+    const int z = z0 + lane;
+    const unsigned zc = (unsigned)min(z, g.nz - 1);
+    const unsigned zoff = zc * ES, zoffv = zc * 4u;
+    const bool zcomp = (z >= R) && (z < g.nz - R);
+    const bool imgz = (z >= fz.z0) && (z < fz.z1);
+
+    auto lat = [&](int i, int n) -> float {
+        const int j = i < tf.b ? tf.b - 1 - i : (i >= n - tf.b ? i - (n - tf.b) : -1);
+        return j >= 0 ? tab[max(j, 0)] : 1.0f;
+    };
+    const bool wtap = TAPER && (z0 - R < tf.b);
+    const bool ztap = wtap && (z < tf.b);
+    float fzl = 1.0f;
+    if (ztap) fzl = tab[tf.b - 1 - z];
+
+    unsigned rowb[RY];
+    unsigned ymask = 0, rmask = 0;
+    float fyr[RY];
+#pragma unroll
+    for (int r = 0; r < RY; ++r) {
+        const int y = y0 + r;
+        const int yc = min(y, g.ny - 1);
+        rowb[r] = (unsigned)yc * (unsigned)g.sy * ES;
+        if (y < g.ny - R) ymask |= 1u << r;
+        const int dy = y - fz.ry0;
+        if (dy >= 0 && (dy % fz.rdy) == 0 && (dy / fz.rdy) < fz.rny) rmask |= 1u << r;
+        fyr[r] = 1.0f;
+        if (wtap) fyr[r] = __builtin_bit_cast(
+            float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, lat(yc, g.ny))));
+    }
+    unsigned yhb[2 * R];
+    float yhf[2 * R];
+#pragma unroll
+    for (int h = 0; h < 2 * R; ++h) {
+        const int gy = min(h < R ? y0 - R + h : y0 + RY + (h - R), g.ny - 1);
+        yhb[h] = (unsigned)gy * (unsigned)g.sy * ES;
+        yhf[h] = 1.0f;
+        if (wtap) yhf[h] = __builtin_bit_cast(
+            float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, lat(gy, g.ny))));
+    }
+    const bool zh_on = lane < NZH * RY;
+    const int zr = min(lane / NZH, RY - 1), zk = lane % NZH;
+    const int zhz = zk < R ? z0 - R + zk : z0 + 64 + (zk - R);
+    const int zhzc = min(max(zhz, 0), g.nz - 1);
+    const int zcol = zk < R ? zk : 64 + zk;
+    const int zhy = min(y0 + zr, g.ny - 1);
+    const unsigned zhoff = ((unsigned)zhy * (unsigned)g.sy + (unsigned)zhzc) * ES;
+    const bool zhtap = wtap && zhzc < tf.b;
+    float zhfy = 1.0f, zhfz = 1.0f;
+    if (zhtap) {
+        zhfy = lat(zhy, g.ny);
+        zhfz = tab[tf.b - 1 - zhzc];
+    }
+
+    // field 0: source (plain), field 1: receiver (tapered when TAPER)
+    float qs[NR][RY], qr[NR][RY];
+    float ys[2][2 * R], yr[2][2 * R], zs[2], zr2[2];
+
+    auto plane_fx = [&](int x) -> float {
+        return wtap ? __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(
+                                                    __builtin_bit_cast(int, lat(x, g.nx))))
+                    : 1.0f;
+    };
+    auto tap = [&](float v, float fx, float fy) -> float {
+        return wrnd<T>(taper1(v, ztap ? fx : 1.0f, ztap ? fy : 1.0f, fzl));
+    };
+    auto load_ring = [&](const T *F, int x, float(&dst)[RY]) {
+        const auto rs = plane_rsrc(F + (size_t)x * (size_t)g.sx);
+#pragma unroll
+        for (int r = 0; r < RY; ++r) dst[r] = ld_buf<T>(rs, rowb[r], zoff);
+    };
+    auto taper_ring = [&](int x, float(&dst)[RY]) {
+        if (TAPER && wtap) {
+            const float fx = plane_fx(x);
+#pragma unroll
+            for (int r = 0; r < RY; ++r) dst[r] = tap(dst[r], fx, fyr[r]);
+        }
+    };
+    auto fetch_halo = [&](const T *F, int x, float(&yh)[2 * R], float &zh) {
+        const auto rs = plane_rsrc(F + (size_t)x * (size_t)g.sx);
+#pragma unroll
+        for (int h = 0; h < 2 * R; ++h) yh[h] = ld_buf<T>(rs, yhb[h], zoff);
+        zh = ld_buf<T>(rs, 0u, zhoff);
+    };
+    auto taper_halo = [&](int x, float(&yh)[2 * R], float &zh) {
+        if (TAPER && wtap) {
+            const float fx = plane_fx(x);
+#pragma unroll
+            for (int h = 0; h < 2 * R; ++h) yh[h] = tap(yh[h], fx, yhf[h]);
+            zh = wrnd<T>(taper1(zh, zhtap ? fx : 1.0f, zhfy, zhfz));
+        }
+    };
+
+#pragma unroll
+    for (int s = 0; s <= 2 * R; ++s) {
+        load_ring(Ps, xb - R + s, qs[s]);
+        load_ring(Pr, xb - R + s, qr[s]);
+    }
+    fetch_halo(Ps, xb, ys[0], zs[0]);
+    fetch_halo(Pr, xb, yr[0], zr2[0]);
+#pragma unroll
+    for (int s = 0; s < 2 * R; ++s) taper_ring(xb - R + s, qr[s]);
+
+    float(*const lts)[LW] = tile[wid][0];
+    float(*const ltr)[LW] = tile[wid][1];
+    bool bad = false;
+    int x = xb;
+    auto step = [&]<int K>(std::integral_constant<int, K>) -> bool {
+        if (x >= xe) return false;
+        constexpr int cur = K & 1;
+        constexpr int ctr = (K + R) % NR;
+        taper_ring(x + R, qr[(K + 2 * R) % NR]);
+        taper_halo(x, yr[cur], zr2[cur]);
+        const size_t pl = (size_t)x * (size_t)g.sx;
+        const auto rpps = plane_rsrc(PPs + pl);
+        const auto rppr = plane_rsrc(PPr + pl);
+        const auto rv = plane_rsrc(V + pl);
+        const auto rimg = plane_rsrc(fz.img + pl);
+        float c_pps[RY], c_ppr[RY], c_v[RY], c_img[RY];
+#pragma unroll
+        for (int r = 0; r < RY; ++r) {
+            c_pps[r] = ld_buf<T>(rpps, rowb[r], zoff);
+            c_ppr[r] = ld_buf<T>(rppr, rowb[r], zoff);
+            c_v[r] = ld_buf<float>(rv, rowb[r] * VS, zoffv);
+            c_img[r] = ld_buf<float>(rimg, rowb[r] * VS, zoffv);
+        }
+        __builtin_amdgcn_wave_barrier();
+#pragma unroll
+        for (int r = 0; r < RY; ++r) {
+            lts[r][R + lane] = qs[ctr][r];
+            ltr[r][R + lane] = qr[ctr][r];
+        }
+        if (zh_on) {
+            lts[zr][zcol] = zs[cur];
+            ltr[zr][zcol] = zr2[cur];
+        }
+        __builtin_amdgcn_wave_barrier();
+        float ycs[2 * R], ycr[2 * R];
+#pragma unroll
+        for (int h = 0; h < 2 * R; ++h) {
+            ycs[h] = ys[cur][h];
+            ycr[h] = yr[cur][h];
+        }
+        if (x + 1 < xe) {
+            load_ring(Ps, x + 1 + R, qs[(K + 2 * R + 1) % NR]);
+            load_ring(Pr, x + 1 + R, qr[(K + 2 * R + 1) % NR]);
+            fetch_halo(Ps, x + 1, ys[cur ^ 1], zs[cur ^ 1]);
+            fetch_halo(Pr, x + 1, yr[cur ^ 1], zr2[cur ^ 1]);
+        }
+        bool rcvx = false;
+        {
+            const int dx = x - fz.rx0;
+            rcvx = dx >= 0 && (dx % fz.rdx) == 0 && (dx / fz.rdx) < fz.rnx;
+        }
+        float fxc = 1.0f;
+        if (TAPER) fxc = plane_fx(x);
+        float lapls[RY], laplr[RY];
+#pragma unroll
+        for (int r = 0; r < RY; ++r) {
+#pragma unroll
+            for (int f = 0; f < 2; ++f) {
+                const float(&q)[NR][RY] = f ? qr : qs;
+                const float(&yc)[2 * R] = f ? ycr : ycs;
+                const float *t = f ? &ltr[r][R + lane] : &lts[r][R + lane];
+                const float p = q[ctr][r];
+                float ax = 0.f, ay = 0.f, az = 0.f;
+#pragma unroll
+                for (int io = 0; io <= 2 * R; ++io) {
+                    const int ry = r + io - R;
+                    const float vy =
+                        ry < 0 ? yc[ry + R] : (ry >= RY ? yc[R + ry - RY] : q[ctr][ry]);
+                    const float vz = (io == R) ? p : t[io - R];
+                    ax += q[(K + io) % NR][r] * cf.c[0][io];
+                    ay += vy * cf.c[1][io];
+                    az += vz * cf.c[2][io];
+                }
+                float l = ax + ay + az;
+                asm volatile("" : "+v"(l));
+                if (f) laplr[r] = l;
+                else lapls[r] = l;
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < RY; ++r) {
+            const float nps = 2.f * qs[ctr][r] - c_pps[r] + c_v[r] * lapls[r];
+            float pp = c_ppr[r];
+            if (TAPER && wtap) {
+                pp = tap(pp, fxc, fyr[r]);
+                if (tf.pp_taps > 1) pp = tap(pp, fxc, fyr[r]);
+            }
+            const float npr = 2.f * qr[ctr][r] - pp + c_v[r] * laplr[r];
+            if (((ymask >> r) & 1u) && zcomp) {
+                st_buf<T>(rpps, rowb[r], zoff, nps);
+                st_buf<T>(rppr, rowb[r], zoff, npr);
+                bad |= (nps != nps) | (npr != npr);
+                if (imgz && !(rcvx && ((rmask >> r) & 1u) && z == fz.rz))
+                    st_buf<float>(rimg, rowb[r] * VS, zoffv,
+                                  c_img[r] + wrnd<T>(nps) * wrnd<T>(npr));
+            }
+        }
+        ++x;
+        return true;
+    };
+    while (x < xe) {
+        [&]<int... Ks>(std::integer_sequence<int, Ks...>) {
+            (step(std::integral_constant<int, Ks>{}) && ...);
+        }(std::make_integer_sequence<int, NR>{});
+    }
+    if (bad) atomicOr(flag, 1);
+}
+
 struct LaunchKey {
     const void *fn;
     int threads;
@@ -1469,7 +1730,55 @@ hipError_t launch_radius(const T *P, T *PP, const float *V, const Coefs &cf, con
     }
 }
 
+template <typename T, int R, int RY, int WPB, bool TAPER>
+hipError_t launch_dual_cfg(const T *Ps, T *PPs, const T *Pr, T *PPr, const float *V,
+                           const Coefs &cf, const Grid &g, const ImageFuse &fz,
+                           const TaperFuse &tf, int *flag, hipStream_t s, int xchunk)
+{
+    auto kern = k_stencil_dual<T, R, RY, WPB, TAPER>;
+    WaveGrid wg;
+    wg.nzt = (g.nz + 63) / 64;
+    wg.nyt = (g.ny - 2 * R + RY - 1) / RY;
+    const int nxi = g.nx - 2 * R;
+    if (wg.nyt <= 0 || nxi <= 0 || g.nz - 2 * R <= 0) return hipSuccess;
+    const size_t lds = TAPER ? (size_t)tf.b * sizeof(float) : 0;
+    if (xchunk <= 0) {
+        const int cap = device_capacity((const void *)kern, 64 * WPB, lds) * WPB;
+        const int n = choose_chunks((long long)wg.nzt * wg.nyt, nxi, R, cap);
+        xchunk = (nxi + n - 1) / n;
+    }
+    wg.xchunk = xchunk;
+    wg.nxc = (nxi + xchunk - 1) / xchunk;
+    wg.waves = (long long)wg.nzt * wg.nyt * wg.nxc;
+    const long long nb = (wg.waves + WPB - 1) / WPB;
+    hipLaunchKernelGGL(kern, dim3((unsigned)nb), dim3(64, WPB), lds, s, Ps, PPs, Pr, PPr, V, cf,
+                       g, wg, fz, tf, tf.t, flag);
+    return hipGetLastError();
+}
+
 }  // namespace
+
+bool dual_step_supported(int radius, int wave_format)
+{
+    return radius == 4 && wave_format == 0;
+}
+
+hipError_t launch_stencil_dual(int radius, const float *Ps, float *PPs, const float *Pr,
+                               float *PPr, const float *V, const Coefs &cf, const Grid &g,
+                               const ImageFuse &fuse, const TaperFuse *taper, int *flag,
+                               hipStream_t s, const StencilTiling *tiling)
+{
+    if (radius != 4) return hipErrorNotSupported;
+    const int xchunk = tiling ? tiling->xchunk : 0;
+    ImageFuse fz;
+    TaperFuse tf;
+    fuse_args(&fuse, taper, fz, tf);
+    if (taper && taper->b > 0)
+        return launch_dual_cfg<float, 4, 4, 4, true>(Ps, PPs, Pr, PPr, V, cf, g, fz, tf, flag, s,
+                                                     xchunk);
+    return launch_dual_cfg<float, 4, 4, 4, false>(Ps, PPs, Pr, PPr, V, cf, g, fz, tf, flag, s,
+                                                  xchunk);
+}
 
 bool stencil_tiling_supported(int tz, int by, int ry)
 {

@@ -9,7 +9,9 @@ On the 592x592x336 grid of the 512x512x256 + 40 salt workload it launches:
     writes 4 B per cell, so its counters give the scale between the counter
     and true bytes;
   * the plain stencil step 5 times (rtmhip_step);
-  * 5 HBC forward steps (tapered stencil class) and 5 fused backward steps.
+  * 5 HBC forward steps (tapered stencil class) and 5 fused backward steps;
+  * 7 backward steps with the source and receiver steps in one pass
+    (mode 2: two seeding steps, then 5 dual steps).
 Fields hold random values; counters do not depend on them.
 
     python3 rtm3d_demo_amd/profile_stencil.py --summarize FETCH_DIR WRITE_DIR
@@ -19,8 +21,9 @@ Fields hold random values; counters do not depend on them.
 turns the counter_collection.csv files of a FETCH_SIZE pass and a WRITE_SIZE
 pass into per-kernel bytes per dispatch.  Each counter is scaled by what it
 reads for k_stack, whose true traffic is known (8 B read + 4 B written per
-cell), and the plain stencil's total is printed as
-"traffic_bytes_per_launch: N", the line bench.py reads.
+cell), and the first kernel of each stencil class is printed as
+"traffic_bytes_per_launch[class C]: N", the lines bench.py reads (0 plain,
+1 fused receiver, 2 tapered forward, 3 dual backward).
 """
 import csv
 import glob
@@ -76,7 +79,7 @@ def summarize(fetch_dir, write_dir):
     print()
     print("| kernel | dispatches | read MB | write MB | total MB | B per updated cell |")
     print("|---|---|---|---|---|---|")
-    plain = None
+    first = {}
     for name in sorted(fetch, key=lambda k: -fetch[k][0]):
         if name not in write:
             continue
@@ -85,13 +88,22 @@ def summarize(fetch_dir, write_dir):
         short = _short(name)
         print("| %s | %d | %.1f | %.1f | %.1f | %.2f |" % (
             short, fetch[name][1], r * 1e-6, w * 1e-6, (r + w) * 1e-6, (r + w) / interior))
-        if short.startswith("k_stencil") and "false, false" in name and plain is None:
-            plain = r + w
-    if plain is not None:
-        print()
-        print("algorithmic bytes per plain launch: %.0f (16 B x %d updated cells)"
-              % (16.0 * interior, interior))
-        print("traffic_bytes_per_launch: %.0f" % plain)
+        cls = None
+        if short.startswith("k_stencil_dual"):
+            cls = 3
+        elif short.startswith("k_stencil"):
+            tail = short.rsplit(",", 2)[-2:]  # the IMG and TAPER flags
+            img, tap = (t.strip(" >") == "true" for t in tail)
+            cls = 1 if img else (2 if tap else 0)
+        if cls is not None and cls not in first:
+            first[cls] = (short, r + w)
+    print()
+    algo = {0: 16.0, 1: 28.0, 2: 16.0, 3: 36.0}
+    for cls in sorted(first):
+        short, t = first[cls]
+        print("class %d (%s): algorithmic %.0f B per launch (%.0f B x %d updated cells)"
+              % (cls, short, algo[cls] * interior, algo[cls], interior))
+        print("traffic_bytes_per_launch[class %d]: %.0f" % (cls, t))
 
 
 def main():
@@ -134,6 +146,7 @@ def main():
         hip.step(F.a, F.b, V, coefs, order)
     hip.forward_steps(shot, True, V, coefs, g, src, F, upb, 0, 5)
     hip.backward_steps(shot, True, V, coefs, g, F, R, upb, rcv, img, nt - 1, 7, True)
+    hip.backward_steps(shot, True, V, coefs, g, F, R, upb, rcv, img, nt - 1, 7, 2)
     hip.synchronize()
     print("profile sequence done", flush=True)
     hip.finalize()

@@ -114,7 +114,7 @@ def test_modeling_matches_reference(gpu, cases, golden, name):
     assert _bits_equal(rec, fx["records"])
 
 
-@pytest.mark.parametrize("fused", [True, False])
+@pytest.mark.parametrize("fused", [True, False, 2])
 @pytest.mark.parametrize("bc", ["hbc", "rbc"])
 def test_migration_matches_reference(gpu, cases, golden, bc, fused):
     print("migration", bc, "fused" if fused else "unfused", flush=True)
@@ -238,10 +238,12 @@ def test_fused_taper_forward_matches_operator_sequence(gpu, oracle, splits, xchu
         d.free()
 
 
+@pytest.mark.parametrize("mode", [1, 2])
 @pytest.mark.parametrize("shape", GRIDS)
 @pytest.mark.parametrize("splits", [(3,), (1, 2)])
-def test_fused_backward_matches_operator_sequence(gpu, oracle, splits, xchunk, shape):
-    """Fused receiver step (tapers, stencil, receiver injection, imaging)
+def test_fused_backward_matches_operator_sequence(gpu, oracle, splits, xchunk, shape, mode):
+    """Fused receiver step (tapers, stencil, receiver injection, imaging) --
+    and with mode 2 the source field's backward step in the same pass --
     against the reference's operator sequence on non-zero fields."""
     b, order, nt = 12, 8, 8
     rng = np.random.default_rng(22)
@@ -260,7 +262,7 @@ def test_fused_backward_matches_operator_sequence(gpu, oracle, splits, xchunk, s
     hi = nt - 3
     for n in splits:
         gpu.backward_steps(shot, True, d["V"], coefs, d["g"], S, R, d["upb"], d["rcv"],
-                           d["img"], hi, n, True)
+                           d["img"], hi, n, mode)
         hi -= n
     for it in range(nt - 3, nt - 3 - sum(splits), -1):
         oracle.step(Ps, PPs, v2, coefs, order)
