@@ -869,20 +869,38 @@ __global__ void __launch_bounds__(64 * WPB)
     if (bad) atomicOr(flag, 1);
 }
 
+// Global access at a wave-uniform base plus a per-lane 32-bit byte offset.
+// The offset is made opaque at each use so that loop-invariant code motion
+// cannot fold it into per-lane 64-bit pointers (one VGPR pair per row and
+// array); the access then selects the scalar-base + 32-bit-offset form.
+template <typename U>
+__device__ __forceinline__ U ld_at(const char *base, unsigned off)
+{
+    asm volatile("" : "+v"(off));
+    return *reinterpret_cast<const U *>(base + off);
+}
+template <typename U>
+__device__ __forceinline__ void st_at(char *base, unsigned off, U v)
+{
+    asm volatile("" : "+v"(off));
+    *reinterpret_cast<U *>(base + off) = v;
+}
+
 // ---------------------------------------------------------------------------
-// Full-row variant for 16-bit wavefields with two z per lane.  A lane owns
-// z = 2*tx and 2*tx + 1, so every wavefield access moves one 32-bit word (a
-// pair of 16-bit values): with one 2-byte value per lane the memory system
-// streams at well under half its 4-byte rate.  The block layout is the
-// full-row one (BY thread rows x RY rows, 64 * NW lanes per row, NW =
-// ceil(nz / 128)); needs nz even (pairs never straddle a row end).
+// Full-row variant with two z per lane.  A lane owns z = 2*tx and 2*tx + 1,
+// so every access moves a pair: 8 bytes for FP32 wavefields, one 32-bit word
+// of two 16-bit values otherwise (one 2-byte value per lane streams at well
+// under half the 4-byte rate; 8-byte lanes stream about 10 % faster than
+// 4-byte ones).  The block layout is the full-row one (BY thread rows x RY
+// rows, 64 * NW lanes per row, NW = ceil(nz / 128)); needs nz even (pairs
+// never straddle a row end).  Accesses are global loads at a wave-uniform
+// row base plus the lane's 32-bit byte offset (see ld_at).
 template <typename T, int R, int BY, int RY, int NWM, bool IMG, bool TAPER>
 __global__ void __launch_bounds__(64 * NWM * BY)
     k_stencil_rows2(const T *__restrict__ P, T *__restrict__ PP, const float *__restrict__ V,
                     const Coefs cf, const Grid g, const RowGrid rg, const ImageFuse fz,
                     const TaperFuse tf, const float *__restrict__ gt, int *__restrict__ flag)
 {
-    static_assert(sizeof(T) == 2, "16-bit wavefields");
     static_assert(R % 2 == 0, "8-byte aligned LDS pairs");
     constexpr int TY = BY * RY;
     constexpr int LW = 128 * NWM + 2 * R;  // LDS row: z in [-R, 128*NWM + R)
@@ -903,7 +921,9 @@ __global__ void __launch_bounds__(64 * NWM * BY)
     const int xe = min(xb + rg.xchunk, g.nx - R);
     const int zl = 2 * tx;                          // first z of the pair
     const unsigned zp = (unsigned)min(zl, g.nz - 2);  // clamped pair start
-    const unsigned zoff = zp * 2u;                  // wavefield byte offset
+    constexpr unsigned ES = sizeof(T);
+    constexpr unsigned VS = 4u / sizeof(T);         // float / wavefield offset ratio
+    const unsigned zoff = zp * ES;                  // wavefield byte offset
     const unsigned zoffv = zp * 4u;                 // float byte offset
     bool zcomp[2];
 #pragma unroll
@@ -936,7 +956,7 @@ __global__ void __launch_bounds__(64 * NWM * BY)
     for (int r = 0; r < RY; ++r) {
         const int y = y0 + ty + BY * r;
         const int yc = min(y, g.ny - 1);
-        rowb[r] = (unsigned)yc * (unsigned)g.sy * 2u;
+        rowb[r] = (unsigned)yc * (unsigned)g.sy * ES;
         if (y < g.ny - R) ymask |= 1u << r;
         if (IMG) {
             const int dy = y - fz.ry0;
@@ -955,20 +975,29 @@ __global__ void __launch_bounds__(64 * NWM * BY)
         yh_valid[i] = h < NH;
         const int gy = min(h < R ? y0 - R + h : y0 + TY + (h - R), g.ny - 1);
         yh_lrow[i] = h < R ? h : R + TY + (h - R);
-        yh_b[i] = (unsigned)gy * (unsigned)g.sy * 2u;
+        yh_b[i] = (unsigned)gy * (unsigned)g.sy * ES;
         yh_fy[i] = 1.0f;
         if (wtap) yh_fy[i] = lat(gy, g.ny);
     }
 
-    // pair load / store of the wavefield, widened to / rounded from float
-    auto ld2 = [&](__amdgpu_buffer_rsrc_t rs, unsigned soff, float(&d)[2]) {
-        const unsigned w = __builtin_amdgcn_raw_buffer_load_b32(rs, (int)zoff, (int)soff, 0);
-        d[0] = wld(__builtin_bit_cast(T, (unsigned short)(w & 0xffffu)));
-        d[1] = wld(__builtin_bit_cast(T, (unsigned short)(w >> 16)));
+    // pair load / store of the wavefield (widened to / rounded from float)
+    // and of float arrays, at plane + row byte offset + the lane's offset
+    auto ld2 = [&](const T *plane, unsigned soff, float(&d)[2]) {
+        const char *base = reinterpret_cast<const char *>(plane) + soff;
+        if constexpr (sizeof(T) == 4) {
+            const float2 v = ld_at<float2>(base, zoff);
+            d[0] = v.x;
+            d[1] = v.y;
+        } else {
+            const unsigned w = ld_at<unsigned>(base, zoff);
+            d[0] = wld(__builtin_bit_cast(T, (unsigned short)(w & 0xffffu)));
+            d[1] = wld(__builtin_bit_cast(T, (unsigned short)(w >> 16)));
+        }
     };
-    auto ldf2 = [&](__amdgpu_buffer_rsrc_t rs, unsigned soff, float(&d)[2]) {
-        d[0] = ld_buf<float>(rs, soff, zoffv);
-        d[1] = ld_buf<float>(rs, soff, zoffv + 4u);
+    auto ldf2 = [&](const float *plane, unsigned soff, float(&d)[2]) {
+        const float2 v = ld_at<float2>(reinterpret_cast<const char *>(plane) + soff, zoffv);
+        d[0] = v.x;
+        d[1] = v.y;
     };
 
     float q[NR][RY][2];
@@ -979,9 +1008,9 @@ __global__ void __launch_bounds__(64 * NWM * BY)
         return wrnd<T>(taper1(v, ztap[e] ? fx : 1.0f, ztap[e] ? fy : 1.0f, fzl[e]));
     };
     auto load_ring = [&](int x, float(&dst)[RY][2]) {
-        const auto rs = plane_rsrc(P + (size_t)x * (size_t)g.sx);
+        const T *pl = P + (size_t)x * (size_t)g.sx;
 #pragma unroll
-        for (int r = 0; r < RY; ++r) ld2(rs, rowb[r], dst[r]);
+        for (int r = 0; r < RY; ++r) ld2(pl, rowb[r], dst[r]);
     };
     auto taper_ring = [&](int x, float(&dst)[RY][2]) {
         if (TAPER && wtap) {
@@ -993,11 +1022,11 @@ __global__ void __launch_bounds__(64 * NWM * BY)
         }
     };
     auto fetch_halo = [&](int x, int set) {
-        const auto rs = plane_rsrc(P + (size_t)x * (size_t)g.sx);
+        const T *pl = P + (size_t)x * (size_t)g.sx;
 #pragma unroll
         for (int i = 0; i < NYH; ++i) {
             if (yh_valid[i]) {
-                ld2(rs, yh_b[i], n_yh[set][i]);
+                ld2(pl, yh_b[i], n_yh[set][i]);
             } else {
                 n_yh[set][i][0] = 0.f;
                 n_yh[set][i][1] = 0.f;
@@ -1032,21 +1061,19 @@ __global__ void __launch_bounds__(64 * NWM * BY)
         taper_halo(x, cur);
         float *tl = smem + cur * (LH * LW);
         const size_t pl = (size_t)x * (size_t)g.sx;
-        const auto rpp = plane_rsrc(PP + pl);
-        const auto rv = plane_rsrc(V + pl);
-        const auto rps = plane_rsrc(IMG ? (const void *)(PS + pl) : (const void *)PP);
-        const auto rimg = plane_rsrc(IMG ? (const void *)(fz.img + pl) : (const void *)PP);
+        T *const ppl = PP + pl;
+        const float *const vpl = V + pl;
         float c_pp[RY][2], c_v[RY][2], c_ps[RY][2], c_img[RY][2];
 #pragma unroll
         for (int r = 0; r < RY; ++r) {
-            ld2(rpp, rowb[r], c_pp[r]);
-            ldf2(rv, rowb[r] * 2u, c_v[r]);
+            ld2(ppl, rowb[r], c_pp[r]);
+            ldf2(vpl, rowb[r] * VS, c_v[r]);
         }
         if (IMG) {
 #pragma unroll
             for (int r = 0; r < RY; ++r) {
-                ld2(rps, rowb[r], c_ps[r]);
-                ldf2(rimg, rowb[r] * 2u, c_img[r]);
+                ld2(PS + pl, rowb[r], c_ps[r]);
+                ldf2(fz.img + pl, rowb[r] * VS, c_img[r]);
             }
         }
 #pragma unroll
@@ -1109,16 +1136,18 @@ __global__ void __launch_bounds__(64 * NWM * BY)
                 npp[e] = 2.f * q[ctr][r][e] - pp + c_v[r][e] * lapl[r][e];
             }
             if ((ymask >> r) & 1u) {
-                const unsigned short b0 = __builtin_bit_cast(unsigned short, wst<T>(npp[0]));
-                const unsigned short b1 = __builtin_bit_cast(unsigned short, wst<T>(npp[1]));
-                if (zcomp[0] && zcomp[1]) {
-                    __builtin_amdgcn_raw_buffer_store_b32((unsigned)b0 | ((unsigned)b1 << 16), rpp,
-                                                          (int)zoff, (int)rowb[r], 0);
-                } else if (zcomp[0]) {
-                    __builtin_amdgcn_raw_buffer_store_b16(b0, rpp, (int)zoff, (int)rowb[r], 0);
-                } else if (zcomp[1]) {
-                    __builtin_amdgcn_raw_buffer_store_b16(b1, rpp, (int)(zoff + 2u), (int)rowb[r],
-                                                          0);
+                char *base = reinterpret_cast<char *>(ppl) + rowb[r];
+                if constexpr (sizeof(T) == 4) {
+                    if (zcomp[0] && zcomp[1]) st_at<float2>(base, zoff, make_float2(npp[0], npp[1]));
+                    else if (zcomp[0]) st_at<float>(base, zoff, npp[0]);
+                    else if (zcomp[1]) st_at<float>(base, zoff + 4u, npp[1]);
+                } else {
+                    const unsigned short b0 = __builtin_bit_cast(unsigned short, wst<T>(npp[0]));
+                    const unsigned short b1 = __builtin_bit_cast(unsigned short, wst<T>(npp[1]));
+                    if (zcomp[0] && zcomp[1])
+                        st_at<unsigned>(base, zoff, (unsigned)b0 | ((unsigned)b1 << 16));
+                    else if (zcomp[0]) st_at<unsigned short>(base, zoff, b0);
+                    else if (zcomp[1]) st_at<unsigned short>(base, zoff + 2u, b1);
                 }
 #pragma unroll
                 for (int e = 0; e < 2; ++e) {
@@ -1127,8 +1156,8 @@ __global__ void __launch_bounds__(64 * NWM * BY)
                     const int ze = zl + e;
                     if (IMG && ze >= fz.z0 && ze < fz.z1 &&
                         !(rcvx && ((rmask >> r) & 1u) && ze == fz.rz))
-                        st_buf<float>(rimg, rowb[r] * 2u, zoffv + 4u * e,
-                                      c_img[r][e] + c_ps[r][e] * wrnd<T>(npp[e]));
+                        st_at<float>(reinterpret_cast<char *>(fz.img + pl) + rowb[r] * VS,
+                                     zoffv + 4u * e, c_img[r][e] + c_ps[r][e] * wrnd<T>(npp[e]));
                 }
             }
         }
@@ -1671,6 +1700,12 @@ hipError_t launch_radius(const T *P, T *PP, const float *V, const Coefs &cf, con
     constexpr bool full = std::is_same<T, float>::value;
     const int xchunk = t ? t->xchunk : 0;
     if constexpr (R == 4 && std::is_same<T, float>::value) {
+        // full rows with two z per lane (8-byte accesses): 2 x 4 x 3
+        if (t && t->tz == 2 && t->by == 4 && t->ry == 3 && (g.nz & 1) == 0 &&
+            (g.nz + 127) / 128 <= 3 && (!taper || taper->b <= 4096))
+            return launch_rows2<T, R, 4, 3, 3>(P, PP, V, cf, g, fuse, taper, flag, s, xchunk);
+    }
+    if constexpr (R == 4 && std::is_same<T, float>::value) {
         // barrier-free waves: 64 x 1 x ry
         if (t && t->tz == 64 && t->by == 1 && (!taper || taper->b <= 8192)) {
             if (t->ry == 6)
@@ -1785,7 +1820,7 @@ bool stencil_tiling_supported(int tz, int by, int ry)
     return (tz == 64 && by == 4 && (ry == 4 || ry == 2)) || (tz == 64 && by == 8 && ry == 2) ||
            (tz == 32 && by == 8 && (ry == 4 || ry == 2)) ||
            (tz == 1 && ((by == 4 && ry == 4) || (by == 2 && (ry == 6 || ry == 7)))) ||
-           (tz == 64 && by == 1 && (ry == 6 || ry == 8));
+           (tz == 64 && by == 1 && (ry == 6 || ry == 8)) || (tz == 2 && by == 4 && ry == 3);
 }
 
 template <typename T>
