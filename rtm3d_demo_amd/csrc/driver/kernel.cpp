@@ -3,9 +3,12 @@
 
 #include <chrono>
 #include <cstdio>
+#include <cstdlib>
+#include <cstring>
 #include <fstream>
 #include <iostream>
 #include <sstream>
+#include <stdexcept>
 
 namespace rtm3d {
 
@@ -36,6 +39,31 @@ std::string KernelReport::to_string() const
 
 RTMKernel::RTMKernel(const RTMParam &p, RTMHIPPlatform &plat) : p_(p), plat_(plat) {}
 
+// Options beyond the reference's JSON, from the environment (the JSON schema
+// and the command line stay the reference's):
+//   RTM3D_WAVE_FORMAT   f32 (default) | f16 | bf16: wavefield and UPB storage
+//                       (FP32 arithmetic; order 8 only; no x-section snapshots)
+//   RTM3D_BACKWARD_MODE 1 (default): fused receiver step; 2: source and
+//                       receiver steps in one pass; 0: unfused operators
+void RTMKernel::read_options()
+{
+    if (const char *f = std::getenv("RTM3D_WAVE_FORMAT")) {
+        if (!std::strcmp(f, "f16")) wave_format_ = RTMHIP_WAVE_F16;
+        else if (!std::strcmp(f, "bf16")) wave_format_ = RTMHIP_WAVE_BF16;
+        else if (!std::strcmp(f, "f32")) wave_format_ = RTMHIP_WAVE_F32;
+        else throw std::runtime_error(std::string("RTM3D_WAVE_FORMAT must be f32, f16 or bf16, not ") + f);
+        if (wave_format_ != RTMHIP_WAVE_F32 && p_.stencil_order != 8)
+            throw std::runtime_error("RTM3D_WAVE_FORMAT f16/bf16 needs stencil_order 8");
+        if (wave_format_ != RTMHIP_WAVE_F32 && p_.save_snapshots)
+            throw std::runtime_error("x-section snapshots are written for FP32 wavefields only");
+    }
+    if (const char *m = std::getenv("RTM3D_BACKWARD_MODE")) {
+        backward_mode_ = std::atoi(m);
+        if (backward_mode_ < 0 || backward_mode_ > 2)
+            throw std::runtime_error("RTM3D_BACKWARD_MODE must be 0, 1 or 2");
+    }
+}
+
 RTMKernel::~RTMKernel() = default;
 
 void RTMKernel::initKernel()
@@ -46,19 +74,23 @@ void RTMKernel::initKernel()
     d_.nze = (int)p_.nz + 2 * d_.blen;
     d_.order = (int)p_.stencil_order;
     hbc_ = parse_bc(p_.boundary_condition) != BoundaryCondition::RBC;
+    read_options();
     const size_t E = (size_t)d_.nxe * d_.nye * d_.nze;
+    // wavefield / UPB buffers in float slots: 16-bit elements take half
+    auto wave_slots = [&](size_t n) { return wave_format_ == RTMHIP_WAVE_F32 ? n : (n + 1) / 2; };
     coefs_ = stencil_coefs(d_.order, p_.dx, p_.dy, p_.dz);
     const std::vector<float> g = taper_table(d_.blen, p_.taper_factor);
     taper_.createDeviceBuffer(g.size() ? g.size() : 1);
     if (!g.empty()) taper_.moveToDevice(g.data());
     const int nsrc = p_.modeling ? 2 : 4;
-    for (int i = 0; i < nsrc; ++i) f_[i].createDeviceBuffer(E);
+    for (int i = 0; i < nsrc; ++i) f_[i].createDeviceBuffer(wave_slots(E));
     const size_t nrcv = (size_t)p_.receiver_count_x * p_.receiver_count_y * p_.ntstep;
     rcv_.createDeviceBuffer(nrcv ? nrcv : 1);
     host_rcv_.assign(nrcv ? nrcv : 1, 0.f);
     if (!p_.modeling) {
         img_.createDeviceBuffer(E);
-        if (hbc_) upb_.createDeviceBuffer((size_t)p_.nt * d_.nxe * d_.nye * (d_.order / 2));
+        if (hbc_)
+            upb_.createDeviceBuffer(wave_slots((size_t)p_.nt * d_.nxe * d_.nye * (d_.order / 2)));
     }
 }
 
@@ -78,7 +110,7 @@ rtmhip_rcv_geom RTMKernel::geom() const
 rtmhip_shot RTMKernel::make_shot(const ShotPos &s) const
 {
     rtmhip_shot sh{};
-    sh.wave_format = RTMHIP_WAVE_F32;
+    sh.wave_format = wave_format_;
     sh.nxe = d_.nxe;
     sh.nye = d_.nye;
     sh.nze = d_.nze;
@@ -236,7 +268,7 @@ void RTMKernel::rtmMigrate(const ShotPos &s, const DeviceGrid &v2dt2)
             hipcheck(rtmhip_backward_steps(&sh, hbc_ ? 1 : 0, v2dt2.ptr(), coefs_.data(),
                                            taper_.ptr(), &Ps, &PPs, &Pr, &PPr,
                                            hbc_ ? upb_.ptr() : nullptr, rcv_.ptr(), img_.ptr(),
-                                           it, n, 1),
+                                           it, n, backward_mode_),
                      "backward steps");
             it -= n;
             if (p_.save_snapshots && (it + 1) % (int)p_.snapshot_step == 0)

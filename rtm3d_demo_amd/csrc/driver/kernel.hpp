@@ -45,12 +45,15 @@ private:
     RTMHIPPlatform &plat_;
     Dims d_{};
     bool hbc_ = true;
+    int wave_format_ = RTMHIP_WAVE_F32;  // RTM3D_WAVE_FORMAT
+    int backward_mode_ = 1;              // RTM3D_BACKWARD_MODE
     std::vector<float> coefs_;
     DeviceGrid taper_, ricker_, rcv_, upb_, img_;
     DeviceGrid f_[4];  // two source and two receiver wavefields
     KernelReport report_;
     std::vector<float> host_rcv_;
 
+    void read_options();
     rtmhip_shot make_shot(const ShotPos &s) const;
     rtmhip_rcv_geom geom() const;
     void snapshot(const float *field, int sx, const std::string &file);

@@ -84,3 +84,38 @@ def test_driver_reports_dispersion(workdir, cases):
         json.dump(p, f)
     r = subprocess.run([DRIVER, path], cwd=workdir, capture_output=True, text=True, timeout=300)
     assert r.returncode == 1 and "Numerical Dispersion" in r.stdout
+
+
+@pytest.mark.parametrize("fmt, mode", [("f32", "2"), ("f16", "1"), ("bf16", "0")])
+def test_driver_wave_format_and_backward_mode(workdir, cases, golden, oracle, fmt, mode):
+    """RTM3D_WAVE_FORMAT / RTM3D_BACKWARD_MODE: modeling records and the HBC
+    image written by the driver equal the oracle's pipeline in that storage
+    format (the backward mode never changes results)."""
+    print("driver", fmt, "mode", mode, flush=True)
+    env = dict(os.environ, RTM3D_WAVE_FORMAT=fmt, RTM3D_BACKWARD_MODE=mode)
+    p = dict(cases["small_mod"])
+    path = os.path.join(workdir, "input.json")
+    with open(path, "w") as f:
+        json.dump(p, f)
+    r = subprocess.run([DRIVER, path], cwd=workdir, capture_output=True, text=True,
+                       timeout=300, env=env)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    vel = golden("small_mod")["vel"]
+    ofmt = None if fmt == "f32" else fmt
+    recs = oracle.pipeline_modeling(p, vel, np.float32, ofmt)
+    s0 = _shots(p)[0]
+    got = np.fromfile(os.path.join(workdir, "dat", _seism(p, s0, 0)), np.float32)
+    assert np.array_equal(got.view(np.uint32), recs[0, 0].ravel().view(np.uint32))
+    q = dict(cases["small_hbc"], load_vpe_from_file=False, save_vpe_file=True, nthreads=1)
+    with open(path, "w") as f:
+        json.dump(q, f)
+    r = subprocess.run([DRIVER, path], cwd=workdir, capture_output=True, text=True,
+                       timeout=300, env=env)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    vpe = np.fromfile(os.path.join(workdir, "vel", "model_ext.bin"), np.float32)
+    vpe = vpe.reshape(golden("small_hbc")["vpe"].shape)
+    name = "RTMIMG_%s_NS2_NXNYNZ_%dx%dx%d_NT%d.bin.NP1" % (
+        q["mname"].upper(), q["nx"], q["ny"], q["nz"], q["nt"])
+    img = np.fromfile(os.path.join(workdir, "out", name), np.float32)
+    want = oracle.pipeline_migration(q, vpe, recs, np.float32, ofmt)
+    assert np.array_equal(img.view(np.uint32), want.ravel().view(np.uint32))
