@@ -193,6 +193,60 @@ int rtmhip_backward_steps(const rtmhip_shot *shot, int hbc, const float *V2DT2,
                           float **Pr, float **PPr, const float *upb, const float *rcv_chunk,
                           float *img, int it_hi, int nsteps, int fused);
 
+/* ================= batches of shots =================
+ * No reference counterpart: the reference migrates one shot at a time per
+ * process (rtm-3d/src/controller/RTMAcousticController.cpp:141-200).  A batch
+ * advances many HBC shots, each on its own extended grid (uneven apertures),
+ * in lockstep: every step is one stencil launch per kernel class covering
+ * all shots plus one launch of the small per-step operators (frame taper,
+ * source, UPB, receivers, imaging off the stencil's cells) of all shots, and
+ * runs of steps replay a captured HIP graph.  Each shot's results are
+ * bit-identical to rtmhip_forward_steps (hbc = 1) and rtmhip_backward_steps
+ * (hbc = 1, fused = 1) on that shot alone.
+ *
+ * Every shot: stencil order 8, the same blen, nt, wave_format, coefficient
+ * plane and taper table; ntstep == nt (rcv holds the whole record
+ * [count_x][count_y][nt]); upb holds all nt layers.  The caller owns every
+ * buffer; the batch borrows them until rtmhip_batch_destroy.  fs and fr are
+ * the source and receiver wavefield pairs, (fs[0], fs[1]) taken as
+ * (P, PP) at creation; rtmhip_batch_fields returns the current roles.  A
+ * forward run (rtmhip_batch_forward_steps from it0 = 0 to nt-1) leaves the
+ * source pair as the reference's snap0 / snap1 for the backward run
+ * (rtmhip_batch_backward_steps from it_hi = nt-1 down), which needs fr and
+ * img zeroed.  Calls may split the time range into chunks. */
+typedef struct {
+    rtmhip_shot shot;
+    const float *V2DT2;    /* v^2 dt^2, extended grid */
+    const float *ricker;   /* [nt] source samples */
+    float *fs[2];          /* source wavefield pair */
+    float *fr[2];          /* receiver wavefield pair */
+    float *upb;            /* [nt][nxe][nye][order/2] */
+    const float *rcv;      /* [count_x][count_y][nt] */
+    float *img;            /* extended-grid image (accumulated) */
+} rtmhip_batch_shot;
+
+typedef struct rtmhip_batch rtmhip_batch;
+
+/* coefs (host): the shared [3][9] stencil plane; taper: device table g[blen]. */
+int rtmhip_batch_create(rtmhip_batch **batch, const rtmhip_batch_shot *shots, int nshots,
+                        const float *coefs, const float *taper);
+int rtmhip_batch_destroy(rtmhip_batch *batch);
+int rtmhip_batch_forward_steps(rtmhip_batch *batch, int it0, int nsteps);
+int rtmhip_batch_backward_steps(rtmhip_batch *batch, int it_hi, int nsteps);
+/* Current (P, PP) of shot i's source pair and (Pr, PPr) of its receiver
+ * pair; any output may be NULL. */
+int rtmhip_batch_fields(const rtmhip_batch *batch, int i, float **P, float **PP, float **Pr,
+                        float **PPr);
+/* Stencil launches of one kernel class (0 backward source step, 1 receiver
+ * step with imaging, 2 forward tapered step): mode 0 = one launch per shot
+ * with the class tiling of rtmhip_set_tiling; 6 or 8 = one launch for every
+ * shot (barrier-free 64-lane waves with 6 or 8 rows each).  Defaults: class
+ * 0 mode 8, classes 1 and 2 mode 0. */
+int rtmhip_batch_set_stencil(rtmhip_batch *batch, int kernel_class, int mode);
+/* Steps per captured graph: 0 = launch every step (no graph), else an even
+ * count (default 8). */
+int rtmhip_batch_set_graph(rtmhip_batch *batch, int steps);
+
 /* ================= tuning and measurement =================
  * Stencil kernel classes: 0 = plain step (rtmhip_step, backward source
  * reconstruction, RBC), 1 = receiver step fused with receiver injection and

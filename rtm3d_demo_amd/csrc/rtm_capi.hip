@@ -6,9 +6,12 @@
 // reference's per-step loops with no host synchronisation in between.
 #include "rtm_internal.hpp"
 
+#include <algorithm>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <map>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -551,10 +554,10 @@ int backward_loop(const rtmhip_shot *shot, const ShotCtx &c, int hbc, const floa
                 fz.z1 = z1;
                 fz.rz = c.rz;
                 fz.rx0 = c.rx0;
-                fz.rdx = shot->rcv.dist_x;
+                fz.rdx = std::max(1, shot->rcv.dist_x);
                 fz.rnx = shot->rcv.count_x;
                 fz.ry0 = c.ry0;
-                fz.rdy = shot->rcv.dist_y;
+                fz.rdy = std::max(1, shot->rcv.dist_y);
                 fz.rny = shot->rcv.count_y;
                 HIPCHK(timed(3, [&] {
                            return launch_stencil_dual(c.radius, *Ps, *PPs, *Pr, *PPr, V2DT2, c.cf,
@@ -614,10 +617,10 @@ int backward_loop(const rtmhip_shot *shot, const ShotCtx &c, int hbc, const floa
             fz.z1 = z1;
             fz.rz = c.rz;
             fz.rx0 = c.rx0;
-            fz.rdx = shot->rcv.dist_x;
+            fz.rdx = std::max(1, shot->rcv.dist_x);
             fz.rnx = shot->rcv.count_x;
             fz.ry0 = c.ry0;
-            fz.rdy = shot->rcv.dist_y;
+            fz.rdy = std::max(1, shot->rcv.dist_y);
             fz.rny = shot->rcv.count_y;
             HIPCHK(stencil(c.radius, *Pr, *PPr, V2DT2, c.cf, c.g, &fz, hbc ? &tf : nullptr),
                    "fused stencil");
@@ -768,6 +771,428 @@ int rtmhip_timing_read(int kernel_class, long long *launches, double *total_ms)
     *total_ms = sum;
     t.used = 0;
     return RTMHIP_OK;
+}
+
+}  // extern "C"
+
+// ===========================================================================
+// Batches of shots.  Every shot keeps its own grid and buffers; a step of the
+// batch is one launch per stencil class covering every shot (or one launch
+// per shot, per class, with the class tiling) plus one launch of the small
+// per-step operators of every shot.  Runs of steps replay a captured HIP
+// graph whose small operators read the time index from device memory, so
+// one captured sequence serves every time of the run.
+// ===========================================================================
+
+struct rtmhip_batch {
+    int n = 0;
+    int wave_format = 0;
+    int blen = 0, nt = 0, hf = 0, radius = 0;
+    Coefs cf{};
+    const float *taper = nullptr;
+    std::vector<rtmhip_shot> shots;
+    std::vector<BatchShot> hs[2];   // host tables: wave rows 6 and 8
+    BatchShot *ds[2] = {nullptr, nullptr};
+    long long nblk[2] = {0, 0};
+    long long fwd_items = 0, bwd_items = 0;
+    int *ctr = nullptr;             // device time index of graph replays
+    int cs = 0, cr = 0;             // P = f[c], PP = f[c ^ 1]
+    int mode[3] = {8, 0, 0};        // per stencil class: 0 per-shot launches, 6/8 batched
+    int graph_steps = 8;            // steps per captured graph (0: no graphs)
+    std::map<int, hipGraphExec_t> graphs;
+};
+
+namespace {
+
+int batch_tab(const rtmhip_batch *b, int cls) { return b->mode[cls] == 6 ? 0 : 1; }
+
+// One forward step (HBC) of every shot at time it: with itp != nullptr the
+// small operators read it from *itp (plus itoff).
+template <typename T>
+int batch_fwd_step(rtmhip_batch *b, int taps, const int *itp, int itoff)
+{
+    hipStream_t s = g_state.stream;
+    b->cs ^= 1;
+    const TaperFuse tf{b->taper, b->blen, taps};
+    const int cls = 2;
+    if (b->mode[cls]) {
+        const int k = batch_tab(b, cls);
+        HIPCHK(launch_stencil_batch<T>(b->mode[cls], b->ds[k], b->hs[k].data(), b->n, b->nblk[k],
+                                       false, b->cs, 0, b->cf, &tf, g_state.flag, s),
+               "batched stencil");
+    } else {
+        for (int i = 0; i < b->n; ++i) {
+            const BatchShot &h = b->hs[0][i];
+            HIPCHK(launch_stencil<T>(b->radius, (const T *)h.fs[b->cs], (T *)h.fs[b->cs ^ 1], h.V,
+                                     b->cf, h.g, nullptr, &tf, g_state.flag, s,
+                                     class_tiling(cls)),
+                   "stencil");
+        }
+    }
+    HIPCHK(launch_tail_fwd<T>(b->ds[0], b->n, b->fwd_items, b->cs, b->taper, b->blen, b->radius,
+                              b->hf, taps, itp, itoff, s),
+           "forward tail");
+    return RTMHIP_OK;
+}
+
+// One backward step (HBC, it < nt-2) of every shot: source reconstruction,
+// receiver step fused with imaging, then the small operators.
+template <typename T>
+int batch_bwd_step(rtmhip_batch *b, int taps, const int *itp, int itoff)
+{
+    hipStream_t s = g_state.stream;
+    const TaperFuse tf{b->taper, b->blen, taps};
+    if (b->mode[0]) {
+        const int k = batch_tab(b, 0);
+        HIPCHK(launch_stencil_batch<T>(b->mode[0], b->ds[k], b->hs[k].data(), b->n, b->nblk[k],
+                                       false, b->cs, 0, b->cf, nullptr, g_state.flag, s),
+               "batched stencil");
+    } else {
+        for (int i = 0; i < b->n; ++i) {
+            const BatchShot &h = b->hs[0][i];
+            HIPCHK(launch_stencil<T>(b->radius, (const T *)h.fs[b->cs], (T *)h.fs[b->cs ^ 1], h.V,
+                                     b->cf, h.g, nullptr, nullptr, g_state.flag, s,
+                                     class_tiling(0)),
+                   "stencil");
+        }
+    }
+    if (b->mode[1]) {
+        const int k = batch_tab(b, 1);
+        HIPCHK(launch_stencil_batch<T>(b->mode[1], b->ds[k], b->hs[k].data(), b->n, b->nblk[k],
+                                       true, b->cr, b->cs ^ 1, b->cf, &tf, g_state.flag, s),
+               "batched stencil");
+    } else {
+        for (int i = 0; i < b->n; ++i) {
+            const BatchShot &h = b->hs[0][i];
+            ImageFuse fz{};
+            fz.img = h.img;
+            fz.ps = h.fs[b->cs ^ 1];
+            fz.z0 = b->blen;
+            fz.z1 = h.g.nz - b->blen;
+            fz.rz = h.rz;
+            fz.rx0 = h.rx0;
+            fz.rdx = h.rdx;
+            fz.rnx = h.rnx;
+            fz.ry0 = h.ry0;
+            fz.rdy = h.rdy;
+            fz.rny = h.rny;
+            HIPCHK(launch_stencil<T>(b->radius, (const T *)h.fr[b->cr], (T *)h.fr[b->cr ^ 1], h.V,
+                                     b->cf, h.g, &fz, &tf, g_state.flag, s, class_tiling(1)),
+                   "stencil");
+        }
+    }
+    HIPCHK(launch_tail_bwd<T>(b->ds[0], b->n, b->bwd_items, b->cs, b->cr, b->taper, b->blen,
+                              b->radius, b->hf, taps, b->nt, itp, itoff, s),
+           "backward tail");
+    b->cs ^= 1;
+    b->cr ^= 1;
+    return RTMHIP_OK;
+}
+
+// Captured sequence of graph_steps steps (forward dir = +1, backward -1)
+// starting from the current parities; they return to those parities, so one
+// executable graph per (direction, parities) serves every replay.
+template <typename T>
+int batch_graph(rtmhip_batch *b, int dir, hipGraphExec_t *out)
+{
+    const int key = (dir > 0 ? 0 : 4) | (b->cs << 1) | b->cr;
+    auto f = b->graphs.find(key);
+    if (f != b->graphs.end()) {
+        *out = f->second;
+        return RTMHIP_OK;
+    }
+    hipStream_t s = g_state.stream;
+    const int cs = b->cs, cr = b->cr;
+    HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+    int rc = RTMHIP_OK;
+    for (int k = 0; k < b->graph_steps && rc == RTMHIP_OK; ++k)
+        rc = dir > 0 ? batch_fwd_step<T>(b, 2, b->ctr, k) : batch_bwd_step<T>(b, 2, b->ctr, -k);
+    hipError_t e = rc == RTMHIP_OK ? launch_tick(b->ctr, dir * b->graph_steps, false, s)
+                                   : hipSuccess;
+    hipGraph_t graph = nullptr;
+    hipError_t e2 = hipStreamEndCapture(s, &graph);
+    b->cs = cs;
+    b->cr = cr;
+    if (rc != RTMHIP_OK) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return rc;
+    }
+    if (e != hipSuccess) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return hip_fail(e, "graph capture");
+    }
+    HIPCHK(e2, "hipStreamEndCapture");
+    hipGraphExec_t exec = nullptr;
+    e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    HIPCHK(e, "hipGraphInstantiate");
+    b->graphs[key] = exec;
+    *out = exec;
+    return RTMHIP_OK;
+}
+
+// Steps it0 .. it0+nsteps-1 (dir +1) or it0 down to it0-nsteps+1 (dir -1)
+// of the batched loop; the first step of a call tapers its PP once, later
+// steps twice (the pending in-place taper of the previous step).
+template <typename T>
+int batch_run(rtmhip_batch *b, int dir, int it0, int nsteps)
+{
+    hipStream_t s = g_state.stream;
+    auto step = [&](int taps, int it) {
+        return dir > 0 ? batch_fwd_step<T>(b, taps, nullptr, it)
+                       : batch_bwd_step<T>(b, taps, nullptr, it);
+    };
+    int rc = step(1, it0);
+    if (rc) return rc;
+    int done = 1;
+    const int G = b->graph_steps;
+    if (G > 0 && nsteps - done >= G) {
+        hipGraphExec_t exec;
+        if ((rc = batch_graph<T>(b, dir, &exec))) return rc;
+        HIPCHK(launch_tick(b->ctr, it0 + dir * done, true, s), "time index");
+        while (nsteps - done >= G) {
+            HIPCHK(hipGraphLaunch(exec, s), "hipGraphLaunch");
+            done += G;  // parities unchanged: G is even
+        }
+    }
+    for (; done < nsteps; ++done)
+        if ((rc = step(2, it0 + dir * done))) return rc;
+    // the pending in-place taper: P of the source field (forward) or the
+    // older receiver field (backward), as the per-shot loops leave them
+    for (int i = 0; i < b->n; ++i) {
+        const BatchShot &h = b->hs[0][i];
+        T *f = dir > 0 ? (T *)h.fs[b->cs] : (T *)h.fr[b->cr ^ 1];
+        HIPCHK(launch_taper<T>(f, b->taper, h.g, b->blen, true, s), "taper");
+    }
+    return RTMHIP_OK;
+}
+
+template <typename T>
+int batch_backward(rtmhip_batch *b, int it_hi, int nsteps)
+{
+    // the two seeding steps (it = nt-1, nt-2: the source field is the
+    // forward's last two states, not reconstructed) run per shot
+    int it = it_hi;
+    int left = nsteps;
+    const int seed = std::min(left, std::max(0, it - (b->nt - 3)));
+    if (seed > 0) {
+        for (int i = 0; i < b->n; ++i) {
+            BatchShot &h = b->hs[0][i];
+            ShotCtx c;
+            c.g = h.g;
+            c.radius = b->radius;
+            c.hf = b->hf;
+            c.cf = b->cf;
+            c.src_off = h.src_off;
+            c.rx0 = h.rx0;
+            c.ry0 = h.ry0;
+            c.rz = h.rz;
+            T *Ps = (T *)h.fs[b->cs], *PPs = (T *)h.fs[b->cs ^ 1];
+            T *Pr = (T *)h.fr[b->cr], *PPr = (T *)h.fr[b->cr ^ 1];
+            int rc = backward_loop<T>(&b->shots[i], c, 1, h.V, b->taper, &Ps, &PPs, &Pr, &PPr,
+                                      (const T *)h.upb, h.rcv, h.img, it, seed, 1);
+            if (rc) return rc;
+            if (i == b->n - 1) {
+                b->cs = Ps == (T *)h.fs[0] ? 0 : 1;
+                b->cr = Pr == (T *)h.fr[0] ? 0 : 1;
+            }
+        }
+        it -= seed;
+        left -= seed;
+    }
+    if (left <= 0) return RTMHIP_OK;
+    return batch_run<T>(b, -1, it, left);
+}
+
+}  // namespace
+
+extern "C" {
+
+int rtmhip_batch_create(rtmhip_batch **out, const rtmhip_batch_shot *shots, int nshots,
+                        const float *coefs, const float *taper)
+{
+    NEED_INIT();
+    if (!out || !shots || !coefs || !taper) return fail(RTMHIP_ERR_INVALID, "NULL pointer");
+    *out = nullptr;
+    if (nshots <= 0) return fail(RTMHIP_ERR_INVALID, "empty batch");
+    auto b = std::make_unique<rtmhip_batch>();
+    b->n = nshots;
+    const rtmhip_shot &s0 = shots[0].shot;
+    for (int i = 0; i < nshots; ++i) {
+        const rtmhip_batch_shot &bs = shots[i];
+        const rtmhip_shot &sh = bs.shot;
+        ShotCtx c;
+        int rc;
+        if ((rc = prepare_shot(&sh, coefs, &c))) return rc;
+        if ((rc = check_format(&sh))) return rc;
+        if (sh.order != 8) return fail(RTMHIP_ERR_UNSUPPORTED, "batches need stencil order 8");
+        if (sh.blen != s0.blen || sh.nt != s0.nt || sh.wave_format != s0.wave_format)
+            return fail(RTMHIP_ERR_INVALID, "shot %d: blen, nt and wave_format must match shot 0",
+                        i);
+        if (sh.blen > 8192)
+            return fail(RTMHIP_ERR_UNSUPPORTED, "shot %d: blen %d above 8192", i, sh.blen);
+        if (sh.ntstep != sh.nt)
+            return fail(RTMHIP_ERR_INVALID, "shot %d: batches keep whole records (ntstep == nt)",
+                        i);
+        if (!bs.V2DT2 || !bs.ricker || !bs.fs[0] || !bs.fs[1] || !bs.fr[0] || !bs.fr[1] ||
+            !bs.upb || !bs.rcv || !bs.img)
+            return fail(RTMHIP_ERR_INVALID, "shot %d: NULL pointer", i);
+        BatchShot h{};
+        h.fs[0] = bs.fs[0];
+        h.fs[1] = bs.fs[1];
+        h.fr[0] = bs.fr[0];
+        h.fr[1] = bs.fr[1];
+        h.V = bs.V2DT2;
+        h.img = bs.img;
+        h.upb = bs.upb;
+        h.rcv = bs.rcv;
+        h.ricker = bs.ricker;
+        h.g = c.g;
+        h.src_off = c.src_off;
+        h.rx0 = c.rx0;
+        h.ry0 = c.ry0;
+        h.rz = c.rz;
+        h.rdx = std::max(1, sh.rcv.dist_x);
+        h.rdy = std::max(1, sh.rcv.dist_y);
+        h.rnx = sh.rcv.count_x;
+        h.rny = sh.rcv.count_y;
+        b->hs[0].push_back(h);
+        b->shots.push_back(sh);
+        if (i == 0) {
+            b->radius = c.radius;
+            b->hf = c.hf;
+            b->cf = c.cf;
+        }
+    }
+    b->wave_format = s0.wave_format;
+    b->blen = s0.blen;
+    b->nt = s0.nt;
+    b->taper = taper;
+    long long fw = 0, bw = 0;
+    for (auto &h : b->hs[0]) {
+        h.fwd0 = fw;
+        h.bwd0 = bw;
+        fw += tail_fwd_items(h.g, b->blen, b->radius, b->hf);
+        bw += tail_bwd_items(h.g, b->blen, b->radius, b->hf, h.rnx, h.rny);
+    }
+    b->fwd_items = fw;
+    b->bwd_items = bw;
+    b->hs[1] = b->hs[0];
+    b->nblk[0] = plan_stencil_batch(6, b->hs[0].data(), nshots, b->radius, b->blen);
+    b->nblk[1] = plan_stencil_batch(8, b->hs[1].data(), nshots, b->radius, b->blen);
+    if (b->nblk[0] <= 0 || b->nblk[1] <= 0)
+        return fail(RTMHIP_ERR_INVALID, "a shot's grid leaves no interior");
+    const size_t bytes = sizeof(BatchShot) * (size_t)nshots;
+    for (int k = 0; k < 2; ++k) {
+        void *p = nullptr;
+        HIPCHK(hipMalloc(&p, bytes), "hipMalloc");
+        b->ds[k] = (BatchShot *)p;
+        hipError_t e = hipMemcpyAsync(p, b->hs[k].data(), bytes, hipMemcpyHostToDevice,
+                                      g_state.stream);
+        if (e != hipSuccess) {
+            rtmhip_batch_destroy(b.release());
+            return hip_fail(e, "hipMemcpyAsync");
+        }
+    }
+    void *p = nullptr;
+    hipError_t e = hipMalloc(&p, sizeof(int));
+    if (e != hipSuccess) {
+        rtmhip_batch_destroy(b.release());
+        return hip_fail(e, "hipMalloc");
+    }
+    b->ctr = (int *)p;
+    e = hipStreamSynchronize(g_state.stream);  // the host tables are uploaded
+    if (e != hipSuccess) {
+        rtmhip_batch_destroy(b.release());
+        return hip_fail(e, "hipStreamSynchronize");
+    }
+    *out = b.release();
+    return RTMHIP_OK;
+}
+
+int rtmhip_batch_destroy(rtmhip_batch *b)
+{
+    if (!b) return RTMHIP_OK;
+    if (g_state.init) (void)hipStreamSynchronize(g_state.stream);
+    for (auto &kv : b->graphs) (void)hipGraphExecDestroy(kv.second);
+    for (int k = 0; k < 2; ++k)
+        if (b->ds[k]) (void)hipFree(b->ds[k]);
+    if (b->ctr) (void)hipFree(b->ctr);
+    delete b;
+    return RTMHIP_OK;
+}
+
+int rtmhip_batch_set_stencil(rtmhip_batch *b, int kernel_class, int mode)
+{
+    NEED_INIT();
+    if (!b) return fail(RTMHIP_ERR_INVALID, "NULL batch");
+    if (kernel_class < 0 || kernel_class > 2)
+        return fail(RTMHIP_ERR_INVALID, "kernel class %d not in 0..2", kernel_class);
+    if (mode != 0 && mode != 6 && mode != 8)
+        return fail(RTMHIP_ERR_UNSUPPORTED, "batched stencil mode %d (0, 6 or 8)", mode);
+    if (b->mode[kernel_class] != mode) {
+        HIPCHK(hipStreamSynchronize(g_state.stream), "hipStreamSynchronize");
+        for (auto &kv : b->graphs) (void)hipGraphExecDestroy(kv.second);
+        b->graphs.clear();
+        b->mode[kernel_class] = mode;
+    }
+    return RTMHIP_OK;
+}
+
+int rtmhip_batch_set_graph(rtmhip_batch *b, int steps)
+{
+    NEED_INIT();
+    if (!b) return fail(RTMHIP_ERR_INVALID, "NULL batch");
+    if (steps < 0 || (steps & 1))
+        return fail(RTMHIP_ERR_INVALID, "steps per graph must be 0 or even (got %d)", steps);
+    if (b->graph_steps != steps) {
+        HIPCHK(hipStreamSynchronize(g_state.stream), "hipStreamSynchronize");
+        for (auto &kv : b->graphs) (void)hipGraphExecDestroy(kv.second);
+        b->graphs.clear();
+        b->graph_steps = steps;
+    }
+    return RTMHIP_OK;
+}
+
+int rtmhip_batch_fields(const rtmhip_batch *b, int i, float **P, float **PP, float **Pr,
+                        float **PPr)
+{
+    NEED_INIT();
+    if (!b || i < 0 || i >= b->n) return fail(RTMHIP_ERR_INVALID, "bad batch or shot index");
+    const BatchShot &h = b->hs[0][i];
+    if (P) *P = (float *)h.fs[b->cs];
+    if (PP) *PP = (float *)h.fs[b->cs ^ 1];
+    if (Pr) *Pr = (float *)h.fr[b->cr];
+    if (PPr) *PPr = (float *)h.fr[b->cr ^ 1];
+    return RTMHIP_OK;
+}
+
+int rtmhip_batch_forward_steps(rtmhip_batch *b, int it0, int nsteps)
+{
+    NEED_INIT();
+    if (!b) return fail(RTMHIP_ERR_INVALID, "NULL batch");
+    if (nsteps <= 0) return RTMHIP_OK;
+    if (it0 < 0 || it0 + nsteps > b->nt)
+        return fail(RTMHIP_ERR_INVALID, "time range outside [0,%d)", b->nt);
+    const rtmhip_shot *s = &b->shots[0];
+    return with_format(s, [&](auto t) {
+        using T = decltype(t);
+        return batch_run<T>(b, +1, it0, nsteps);
+    });
+}
+
+int rtmhip_batch_backward_steps(rtmhip_batch *b, int it_hi, int nsteps)
+{
+    NEED_INIT();
+    if (!b) return fail(RTMHIP_ERR_INVALID, "NULL batch");
+    if (nsteps <= 0) return RTMHIP_OK;
+    if (it_hi >= b->nt || it_hi - nsteps + 1 < 0)
+        return fail(RTMHIP_ERR_INVALID, "time range outside [0,%d)", b->nt);
+    const rtmhip_shot *s = &b->shots[0];
+    return with_format(s, [&](auto t) {
+        using T = decltype(t);
+        return batch_backward<T>(b, it_hi, nsteps);
+    });
 }
 
 }  // extern "C"

@@ -78,6 +78,28 @@ struct StencilTiling {
     int xchunk;      // planes marched per block (0 = automatic)
 };
 
+// One shot of a batch (many shots advanced in lockstep by single launches),
+// as the batched kernels read it from device memory.  The wavefield pairs
+// are fixed buffers; which one is "P" and which "PP" at a step is a
+// batch-wide parity passed to each launch.
+struct BatchShot {
+    void *fs[2];         // source wavefield pair (storage type T)
+    void *fr[2];         // receiver wavefield pair (backward pass)
+    const float *V;      // v^2 dt^2
+    float *img;
+    void *upb;           // upper-border history [nt][nx][ny][hf], storage type T
+    const float *rcv;    // receiver record [rnx][rny][nt]
+    const float *ricker; // source samples [nt]
+    Grid g;
+    long long src_off;   // source cell
+    int rx0, ry0, rz, rdx, rdy, rnx, rny;  // receivers (extended coordinates)
+    int nzt, nyt, xchunk;                   // waves of the batched stencil
+    long long nwaves;                       // nzt * nyt * (x chunks)
+    long long blk0;      // first block of this shot in a batched stencil launch
+    long long fwd0;      // first work item of this shot in the forward tail
+    long long bwd0;      // first work item of this shot in the backward tail
+};
+
 // Tile shapes compiled into the stencil kernel (tz lanes x by rows, ry rows
 // per thread); the default is 64x4x4.
 bool stencil_tiling_supported(int tz, int by, int ry);
@@ -124,6 +146,37 @@ hipError_t launch_image(float *img, const T *PS, const T *PR, const Grid &g, int
                         hipStream_t s);
 hipError_t launch_image_finish(const float *img, float *out, const Grid &g, int blen,
                                const Coefs &fc, hipStream_t s);
+// Batched launches over `n` shots (device table `shots`, host copy `hs`
+// for the launch geometry).  Stencil: every shot's step in one launch of
+// the barrier-free wave kernel with ry rows per wave (6 or 8);
+// IMG: receiver step fused with imaging (field fr, source value fs[pspar]),
+// else the source field fs.  par selects P = f[par], PP = f[par ^ 1].
+template <typename T>
+hipError_t launch_stencil_batch(int ry, const BatchShot *shots, const BatchShot *hs, int n,
+                                long long nblocks, bool img, int par, int pspar, const Coefs &cf,
+                                const TaperFuse *taper, int *flag, hipStream_t s);
+// Block count and per-shot wave geometry (nzt, nyt, xchunk, nwaves, blk0) of
+// a batched stencil launch, written into hs.
+long long plan_stencil_batch(int ry, BatchShot *hs, int n, int R, int blen);
+// Forward tail of one HBC step on every shot: frame taper (taps times) of
+// PP = fs[par ^ 1], source injection, UPB save at time it.
+// Backward tail: UPB restore into fs[spar ^ 1], frame taper of
+// fr[rpar ^ 1], receiver injection and the imaging condition off the
+// stencil's cells.  it = (itp ? *itp : 0) + itoff.
+template <typename T>
+hipError_t launch_tail_fwd(const BatchShot *shots, int n, long long items, int par,
+                           const float *taper, int b, int R, int hf, int taps, const int *itp,
+                           int itoff, hipStream_t s);
+template <typename T>
+hipError_t launch_tail_bwd(const BatchShot *shots, int n, long long items, int spar, int rpar,
+                           const float *taper, int b, int R, int hf, int taps, int nt,
+                           const int *itp, int itoff, hipStream_t s);
+// Work items of the tails for one shot.
+long long tail_fwd_items(const Grid &g, int b, int R, int hf);
+long long tail_bwd_items(const Grid &g, int b, int R, int hf, int rnx, int rny);
+// *p += d, or *p = d with set (one thread): the device-resident time index
+// of a captured step sequence.
+hipError_t launch_tick(int *p, int d, bool set, hipStream_t s);
 hipError_t launch_fill(float *p, size_t n, float v, hipStream_t s);
 hipError_t launch_fill_u16(uint16_t *p, size_t n, uint16_t bits, hipStream_t s);
 hipError_t launch_stack(float *dst, const float *src, size_t n, hipStream_t s);

@@ -60,22 +60,26 @@ __global__ void __launch_bounds__(kThreads)
     P[o] = wst<T>(v);
 }
 
-// In-place upper taper, applied `taps` times, of the slab cells (z < b) that
-// the stencil does not update: z < R, or x / y within R of the grid edge.
-// The fused HBC step tapers every other slab value as it is loaded; these
-// cells of PP are not read during the step, so tapering them in place here
-// keeps PP's memory equal to the reference's after its in-place tapers
-// (each application is one in-place pass: rounded to T before the next).
-template <typename T>
-__global__ void __launch_bounds__(kThreads)
-    k_taper_frame(T *__restrict__ P, const float *__restrict__ t, Grid g, int b, int R, int taps)
+// The slab cells (z < b) that the stencil does not update: z < R, or x / y
+// within R of the grid edge.  frame_cell decodes the i-th of them (z < R
+// first, then the x frame, then the y frame); false past the last.
+__device__ __forceinline__ long long frame_count(const Grid &g, int b, int R)
+{
+    const long long lz = b - R;
+    return (long long)g.nx * g.ny * R + (long long)2 * R * g.ny * lz +
+           (long long)(g.nx - 2 * R) * 2 * R * lz;
+}
+__device__ __forceinline__ bool in_frame(const Grid &g, int b, int R, int x, int y, int z)
+{
+    return z < b && (z < R || x < R || x >= g.nx - R || y < R || y >= g.ny - R);
+}
+__device__ __forceinline__ bool frame_cell(const Grid &g, int b, int R, long long i, int &x,
+                                           int &y, int &z)
 {
     const long long nA = (long long)g.nx * g.ny * R;  // z < R, all x, y
     const int lz = b - R;                             // z in [R, b)
     const long long nB = (long long)2 * R * g.ny * lz;              // x frame
     const long long nC = (long long)(g.nx - 2 * R) * 2 * R * lz;    // y frame
-    long long i = (long long)blockIdx.x * kThreads + threadIdx.x;
-    int x, y, z;
     if (i < nA) {
         z = (int)(i % R);
         const long long r = i / R;
@@ -94,22 +98,44 @@ __global__ void __launch_bounds__(kThreads)
         x = R + (int)(r / (2 * R));
         y = yy < R ? yy : g.ny - 2 * R + yy;
     } else {
-        return;
+        return false;
     }
+    return true;
+}
+// The upper taper of one slab cell applied `taps` times in place: each
+// application is one in-place pass of the reference, rounded to T before
+// the next; the last rounding is the caller's store.
+template <typename T>
+__device__ __forceinline__ float frame_taper(float v, const float *__restrict__ t, const Grid &g,
+                                             int b, int x, int y, int z, int taps)
+{
     const bool bx = x < b || x >= g.nx - b;
     const bool by = y < b || y >= g.ny - b;
     const float fy = by ? gfac(t, y, g.ny, b) : 1.0f;
     const float fx = bx ? gfac(t, x, g.nx, b) : 1.0f;
     const float fz = t[b - 1 - z];
-    const long long o = goff(g, x, y, z);
-    float v = wld(P[o]);
     for (int k = 0; k < taps; ++k) {
         if (k) v = wrnd<T>(v);
         v *= fy;
         v *= fx;
         v *= fz;
     }
-    P[o] = wst<T>(v);
+    return v;
+}
+
+// In-place upper taper, applied `taps` times, of the frame cells.  The fused
+// HBC step tapers every other slab value as it is loaded; these cells of PP
+// are not read during the step, so tapering them in place here keeps PP's
+// memory equal to the reference's after its in-place tapers.
+template <typename T>
+__global__ void __launch_bounds__(kThreads)
+    k_taper_frame(T *__restrict__ P, const float *__restrict__ t, Grid g, int b, int R, int taps)
+{
+    const long long i = (long long)blockIdx.x * kThreads + threadIdx.x;
+    int x, y, z;
+    if (!frame_cell(g, b, R, i, x, y, z)) return;
+    const long long o = goff(g, x, y, z);
+    P[o] = wst<T>(frame_taper<T>(wld(P[o]), t, g, b, x, y, z, taps));
 }
 
 // The front-right edge of taper3D (RTMCPUPlatform.cpp:900-904) writes cells
@@ -246,30 +272,20 @@ __device__ __forceinline__ bool is_receiver(int x, int y, int z, int rx0, int ry
            (dx / rdx) < rnx && (dy / rdy) < rny;
 }
 
-template <typename T>
-__global__ void __launch_bounds__(kThreads)
-    k_receivers_image(T *__restrict__ PP, const float *__restrict__ rcv, Grid g, int rx0, int ry0,
-                      int rz, int rdx, int rdy, int rnx, int rny, int ntstep, int lt,
-                      float *__restrict__ img, const T *__restrict__ ps, int z0, int z1, int R)
+// The x / y frame of width R on z in [z0, z1): the cells of the imaging
+// range that the stencil kernel does not visit.
+__device__ __forceinline__ long long image_frame_count(const Grid &g, int R, int z0, int z1)
 {
-    long long i = (long long)blockIdx.x * kThreads + threadIdx.x;
-    const long long nrcv = (long long)rnx * rny;
-    if (i < nrcv) {
-        const int ix = (int)(i / rny), iy = (int)(i % rny);
-        const int rx = rx0 + ix * rdx, ry = ry0 + iy * rdy;
-        if (rx < 0 || rx >= g.nx || ry < 0 || ry >= g.ny) return;
-        const long long o = goff(g, rx, ry, rz);
-        const float v = rcv[i * ntstep + lt];
-        PP[o] = wst<T>(v);
-        if (rz >= z0 && rz < z1) img[o] = img[o] + wld(ps[o]) * wrnd<T>(v);
-        return;
-    }
-    i -= nrcv;
+    const long long lz = z1 > z0 ? z1 - z0 : 0;
+    return (long long)2 * R * g.ny * lz + (long long)(g.nx - 2 * R) * 2 * R * lz;
+}
+__device__ __forceinline__ bool image_frame_cell(const Grid &g, int R, int z0, int z1,
+                                                 long long i, int &x, int &y, int &z)
+{
     const int lz = z1 - z0;
-    if (lz <= 0) return;
+    if (lz <= 0) return false;
     const long long nA = (long long)2 * R * g.ny * lz;
     const long long nB = (long long)(g.nx - 2 * R) * 2 * R * lz;
-    int x, y, z;
     if (i < nA) {
         z = z0 + (int)(i % lz);
         const long long r = i / lz;
@@ -283,8 +299,39 @@ __global__ void __launch_bounds__(kThreads)
         x = R + (int)(r / (2 * R));
         y = yy < R ? yy : g.ny - 2 * R + yy;
     } else {
+        return false;
+    }
+    return true;
+}
+// Receiver i of the grid: injection (SET) of v into PP, then the imaging
+// condition at that cell with the rounded value.
+template <typename T>
+__device__ __forceinline__ void inject_image(T *__restrict__ PP, float *__restrict__ img,
+                                             const T *__restrict__ ps, const Grid &g, int rx,
+                                             int ry, int rz, float v, int z0, int z1)
+{
+    if (rx < 0 || rx >= g.nx || ry < 0 || ry >= g.ny) return;
+    const long long o = goff(g, rx, ry, rz);
+    PP[o] = wst<T>(v);
+    if (rz >= z0 && rz < z1) img[o] = img[o] + wld(ps[o]) * wrnd<T>(v);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kThreads)
+    k_receivers_image(T *__restrict__ PP, const float *__restrict__ rcv, Grid g, int rx0, int ry0,
+                      int rz, int rdx, int rdy, int rnx, int rny, int ntstep, int lt,
+                      float *__restrict__ img, const T *__restrict__ ps, int z0, int z1, int R)
+{
+    long long i = (long long)blockIdx.x * kThreads + threadIdx.x;
+    const long long nrcv = (long long)rnx * rny;
+    if (i < nrcv) {
+        const int ix = (int)(i / rny), iy = (int)(i % rny);
+        inject_image(PP, img, ps, g, rx0 + ix * rdx, ry0 + iy * rdy, rz, rcv[i * ntstep + lt], z0,
+                     z1);
         return;
     }
+    int x, y, z;
+    if (!image_frame_cell(g, R, z0, z1, i - nrcv, x, y, z)) return;
     if (is_receiver(x, y, z, rx0, ry0, rz, rdx, rdy, rnx, rny)) return;
     const long long o = goff(g, x, y, z);
     img[o] = img[o] + wld(ps[o]) * wld(PP[o]);
@@ -373,6 +420,135 @@ __global__ void __launch_bounds__(kThreads)
     for (size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x; i < n;
          i += (size_t)gridDim.x * kThreads)
         d[i] = d[i] + s[i];
+}
+
+// Batched tails: one launch for every shot of a batch.  A thread finds its
+// shot by a binary search over the shots' first work items.
+template <long long BatchShot::*first>
+__device__ __forceinline__ int batch_shot(const BatchShot *__restrict__ shots, int n,
+                                          long long gi)
+{
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (shots[mid].*first <= gi) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// Forward tail of an HBC step on PP = fs[par ^ 1]: the frame taper, the
+// source injection and the UPB save of step `it`, in that order per cell,
+// as launch_taper_frame, launch_source and launch_upb would run them one
+// after the other.  Work items per shot: the UPB layer (z in [b-hf, b),
+// frame cells among them tapered first), the frame cells outside the layer,
+// and the source cell when it lies in neither.
+template <typename T>
+__global__ void __launch_bounds__(kThreads)
+    k_tail_fwd(const BatchShot *__restrict__ shots, int n, long long items, int par,
+               const float *__restrict__ t, int b, int R, int hf, int taps,
+               const int *__restrict__ itp, int itoff)
+{
+    const long long gi = (long long)blockIdx.x * kThreads + threadIdx.x;
+    if (gi >= items) return;
+    const BatchShot &sh = shots[batch_shot<&BatchShot::fwd0>(shots, n, gi)];
+    const Grid g = sh.g;
+    const int it = (itp ? *itp : 0) + itoff;
+    T *PP = static_cast<T *>(sh.fs[par ^ 1]);
+    long long i = gi - sh.fwd0;
+    const long long nl = (long long)g.nx * g.ny * hf;
+    int x, y, z;
+    bool save = false;
+    if (i < nl) {
+        const long long r = i / hf;
+        z = b - hf + (int)(i % hf);
+        y = (int)(r % g.ny);
+        x = (int)(r / g.ny);
+        save = true;
+    } else {
+        i -= nl;
+        const long long nf = frame_count(g, b, R);
+        if (i < nf) {
+            frame_cell(g, b, R, i, x, y, z);
+            if (z >= b - hf) return;  // a layer cell: its layer item handles it
+        } else {
+            const long long so = sh.src_off;
+            x = (int)(so / g.sx);
+            y = (int)((so % g.sx) / g.sy);
+            z = (int)(so % g.sy);
+            if ((z >= b - hf && z < b) || in_frame(g, b, R, x, y, z)) return;
+        }
+    }
+    const long long o = goff(g, x, y, z);
+    const bool fr = in_frame(g, b, R, x, y, z), src = o == sh.src_off;
+    if (!fr && !src && !save) return;
+    T tv = PP[o];
+    if (fr || src) {
+        float v = wld(tv);
+        if (fr) v = wrnd<T>(frame_taper<T>(v, t, g, b, x, y, z, taps));
+        if (src) v = v + sh.ricker[it];
+        tv = wst<T>(v);
+        PP[o] = tv;
+    }
+    if (save) static_cast<T *>(sh.upb)[(long long)it * nl + i] = tv;
+}
+
+// Backward tail of an HBC step: UPB restore of step `it` into the new source
+// field PPs = fs[spar ^ 1]; frame taper, receiver injection (SET) and the
+// imaging condition off the stencil's cells on PPr = fr[rpar ^ 1] with the
+// source value ps = PPs.  The four cell sets are disjoint (the layer and the
+// frame lie above z = b, receivers and the imaging range at or below it), so
+// one launch reproduces launch_upb, launch_taper_frame and
+// launch_receivers_image run in sequence.
+template <typename T>
+__global__ void __launch_bounds__(kThreads)
+    k_tail_bwd(const BatchShot *__restrict__ shots, int n, long long items, int spar, int rpar,
+               const float *__restrict__ t, int b, int R, int hf, int taps, int nt,
+               const int *__restrict__ itp, int itoff)
+{
+    const long long gi = (long long)blockIdx.x * kThreads + threadIdx.x;
+    if (gi >= items) return;
+    const BatchShot &sh = shots[batch_shot<&BatchShot::bwd0>(shots, n, gi)];
+    const Grid g = sh.g;
+    const int it = (itp ? *itp : 0) + itoff;
+    T *PPs = static_cast<T *>(sh.fs[spar ^ 1]);
+    T *PPr = static_cast<T *>(sh.fr[rpar ^ 1]);
+    long long i = gi - sh.bwd0;
+    const long long nl = (long long)g.nx * g.ny * hf;
+    if (i < nl) {
+        const long long r = i / hf;
+        const int k = (int)(i % hf);
+        PPs[goff(g, (int)(r / g.ny), (int)(r % g.ny), b - hf + k)] =
+            static_cast<const T *>(sh.upb)[(long long)it * nl + i];
+        return;
+    }
+    i -= nl;
+    int x, y, z;
+    const long long nf = frame_count(g, b, R);
+    if (i < nf) {
+        frame_cell(g, b, R, i, x, y, z);
+        const long long o = goff(g, x, y, z);
+        PPr[o] = wst<T>(frame_taper<T>(wld(PPr[o]), t, g, b, x, y, z, taps));
+        return;
+    }
+    i -= nf;
+    const int z0 = b, z1 = g.nz - b;
+    const long long nrcv = (long long)sh.rnx * sh.rny;
+    if (i < nrcv) {
+        const int ix = (int)(i / sh.rny), iy = (int)(i % sh.rny);
+        inject_image(PPr, sh.img, (const T *)PPs, g, sh.rx0 + ix * sh.rdx, sh.ry0 + iy * sh.rdy,
+                     sh.rz, sh.rcv[i * nt + it], z0, z1);
+        return;
+    }
+    if (!image_frame_cell(g, R, z0, z1, i - nrcv, x, y, z)) return;
+    if (is_receiver(x, y, z, sh.rx0, sh.ry0, sh.rz, sh.rdx, sh.rdy, sh.rnx, sh.rny)) return;
+    const long long o = goff(g, x, y, z);
+    sh.img[o] = sh.img[o] + wld(PPs[o]) * wld(PPr[o]);
+}
+
+__global__ void k_tick(int *p, int d, int set)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) *p = set ? d : *p + d;
 }
 
 inline unsigned nblocks(long long n)
@@ -509,6 +685,50 @@ hipError_t launch_stack(float *dst, const float *src, size_t n, hipStream_t s)
     return hipGetLastError();
 }
 
+long long tail_fwd_items(const Grid &g, int b, int R, int hf)
+{
+    const long long lz = b - R;
+    return (long long)g.nx * g.ny * hf + (long long)g.nx * g.ny * R +
+           (long long)2 * R * g.ny * lz + (long long)(g.nx - 2 * R) * 2 * R * lz + 1;
+}
+
+long long tail_bwd_items(const Grid &g, int b, int R, int hf, int rnx, int rny)
+{
+    const long long lz = b - R, li = g.nz - 2 * b > 0 ? g.nz - 2 * b : 0;
+    return (long long)g.nx * g.ny * hf + (long long)g.nx * g.ny * R +
+           (long long)2 * R * g.ny * lz + (long long)(g.nx - 2 * R) * 2 * R * lz +
+           (long long)rnx * rny + (long long)2 * R * g.ny * li +
+           (long long)(g.nx - 2 * R) * 2 * R * li;
+}
+
+template <typename T>
+hipError_t launch_tail_fwd(const BatchShot *shots, int n, long long items, int par,
+                           const float *taper, int b, int R, int hf, int taps, const int *itp,
+                           int itoff, hipStream_t s)
+{
+    if (items <= 0 || n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_tail_fwd<T>, dim3(nblocks(items)), dim3(kThreads), 0, s, shots, n, items,
+                       par, taper, b, R, hf, taps, itp, itoff);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_tail_bwd(const BatchShot *shots, int n, long long items, int spar, int rpar,
+                           const float *taper, int b, int R, int hf, int taps, int nt,
+                           const int *itp, int itoff, hipStream_t s)
+{
+    if (items <= 0 || n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_tail_bwd<T>, dim3(nblocks(items)), dim3(kThreads), 0, s, shots, n, items,
+                       spar, rpar, taper, b, R, hf, taps, nt, itp, itoff);
+    return hipGetLastError();
+}
+
+hipError_t launch_tick(int *p, int d, bool set, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_tick, dim3(1), dim3(64), 0, s, p, d, set ? 1 : 0);
+    return hipGetLastError();
+}
+
 #define RTM_INSTANTIATE_OPS(T)                                                                \
     template hipError_t launch_taper<T>(T *, const float *, const Grid &, int, bool,            \
                                         hipStream_t);                                           \
@@ -523,7 +743,13 @@ hipError_t launch_stack(float *dst, const float *src, size_t n, hipStream_t s)
     template hipError_t launch_upb<T>(T *, T *, const Grid &, int, int, long long, bool,        \
                                       hipStream_t);                                             \
     template hipError_t launch_image<T>(float *, const T *, const T *, const Grid &, int, int,  \
-                                        hipStream_t);
+                                        hipStream_t);                                           \
+    template hipError_t launch_tail_fwd<T>(const BatchShot *, int, long long, int,              \
+                                           const float *, int, int, int, int, const int *, int, \
+                                           hipStream_t);                                        \
+    template hipError_t launch_tail_bwd<T>(const BatchShot *, int, long long, int, int,         \
+                                           const float *, int, int, int, int, int, const int *, \
+                                           int, hipStream_t);
 RTM_INSTANTIATE_OPS(float)
 RTM_INSTANTIATE_OPS(_Float16)
 RTM_INSTANTIATE_OPS(__bf16)

@@ -632,46 +632,27 @@ struct WaveGrid {
     long long waves;  // nzt * nyt * nxc
 };
 
-template <typename T, int R, int RY, int WPB, bool IMG, bool TAPER>
-__global__ void __launch_bounds__(64 * WPB)
-    k_stencil_wave(const T *__restrict__ P, T *__restrict__ PP, const float *__restrict__ V,
-                   const Coefs cf, const Grid g, const WaveGrid wg, const ImageFuse fz,
-                   const TaperFuse tf, const float *__restrict__ gt, int *__restrict__ flag)
+// One wave's tile of the barrier-free stencil: z tile zt (64 lanes), rows
+// y0 = R + yt*RY .. +RY, planes [xb, xe); lt is the wave's private LDS rows,
+// tab the taper table in LDS (TAPER).  Shared by the single-grid and the
+// batched kernels.
+template <typename T, int R, int RY, bool IMG, bool TAPER>
+__device__ __forceinline__ void wave_tile(const T *__restrict__ P, T *__restrict__ PP,
+                                          const float *__restrict__ V, const Coefs &cf,
+                                          const Grid &g, int zt, int yt, int xb, int xe,
+                                          const ImageFuse &fz, const TaperFuse &tf,
+                                          const float *tab, float (*lt)[64 + 2 * R],
+                                          int *__restrict__ flag)
 {
     constexpr int NR = 2 * R + 2;
-    constexpr int LW = 64 + 2 * R;              // LDS row: z0 - R .. z0 + 63 + R
-    constexpr int NZH = 2 * R;                  // z-halo cells per row
+    constexpr int LW = 64 + 2 * R;
+    constexpr int NZH = 2 * R;
     static_assert(NZH * RY <= 64, "one z-halo load per lane");
     constexpr unsigned ES = sizeof(T);
     constexpr unsigned VS = 4u / sizeof(T);
-    __shared__ float tile[WPB][RY][LW];
-    extern __shared__ float tab[];              // taper table g[b] (TAPER only)
     const T *__restrict__ PS = static_cast<const T *>(fz.ps);
-
     const int lane = threadIdx.x;
-    const int wid = __builtin_amdgcn_readfirstlane(threadIdx.y);
-    if (TAPER) {
-        const int tid = threadIdx.y * 64 + lane;
-        for (int i = tid; i < tf.b; i += 64 * WPB) tab[i] = gt[i];
-        __syncthreads();  // the only block barrier, before any wave leaves
-    }
-    // XCD-aware block order, then waves of a block in z-tile order
-    long long w;
-    {
-        const long long nb = (wg.waves + WPB - 1) / WPB;
-        const long long h = blockIdx.x;
-        const long long q = nb >> 3, rr = nb & 7, k = h & 7, j = h >> 3;
-        const long long t = k * q + (k < rr ? k : rr) + j;
-        w = t * WPB + wid;
-    }
-    if (w >= wg.waves) return;
-    const int zt = (int)(w % wg.nzt);
-    const long long w2 = w / wg.nzt;
-    const int yt = (int)(w2 % wg.nyt);
-    const int xc = (int)(w2 / wg.nyt);
     const int z0 = zt * 64, y0 = R + yt * RY;
-    const int xb = R + xc * wg.xchunk;
-    const int xe = min(xb + wg.xchunk, g.nx - R);
     const int z = z0 + lane;
     const unsigned zc = (unsigned)min(z, g.nz - 1);
     const unsigned zoff = zc * ES, zoffv = zc * 4u;
@@ -776,7 +757,6 @@ __global__ void __launch_bounds__(64 * WPB)
 #pragma unroll
     for (int s = 0; s < 2 * R; ++s) taper_ring(xb - R + s, q[s]);
 
-    float(*const lt)[LW] = tile[wid];
     bool bad = false;
     int x = xb;
     auto step = [&]<int K>(std::integral_constant<int, K>) -> bool {
@@ -884,6 +864,101 @@ __device__ __forceinline__ void st_at(char *base, unsigned off, U v)
 {
     asm volatile("" : "+v"(off));
     *reinterpret_cast<U *>(base + off) = v;
+}
+
+template <typename T, int R, int RY, int WPB, bool IMG, bool TAPER>
+__global__ void __launch_bounds__(64 * WPB)
+    k_stencil_wave(const T *__restrict__ P, T *__restrict__ PP, const float *__restrict__ V,
+                   const Coefs cf, const Grid g, const WaveGrid wg, const ImageFuse fz,
+                   const TaperFuse tf, const float *__restrict__ gt, int *__restrict__ flag)
+{
+    constexpr int LW = 64 + 2 * R;  // LDS row: z0 - R .. z0 + 63 + R
+    __shared__ float tile[WPB][RY][LW];
+    extern __shared__ float tab[];  // taper table g[b] (TAPER only)
+    const int wid = __builtin_amdgcn_readfirstlane(threadIdx.y);
+    if (TAPER) {
+        const int tid = threadIdx.y * 64 + threadIdx.x;
+        for (int i = tid; i < tf.b; i += 64 * WPB) tab[i] = gt[i];
+        __syncthreads();  // the only block barrier, before any wave leaves
+    }
+    // XCD-aware block order, then waves of a block in z-tile order
+    long long w;
+    {
+        const long long nb = (wg.waves + WPB - 1) / WPB;
+        const long long h = blockIdx.x;
+        const long long q = nb >> 3, rr = nb & 7, k = h & 7, j = h >> 3;
+        const long long t = k * q + (k < rr ? k : rr) + j;
+        w = t * WPB + wid;
+    }
+    if (w >= wg.waves) return;
+    const int zt = (int)(w % wg.nzt);
+    const long long w2 = w / wg.nzt;
+    const int yt = (int)(w2 % wg.nyt);
+    const int xc = (int)(w2 / wg.nyt);
+    const int xb = R + xc * wg.xchunk;
+    const int xe = min(xb + wg.xchunk, g.nx - R);
+    wave_tile<T, R, RY, IMG, TAPER>(P, PP, V, cf, g, zt, yt, xb, xe, fz, tf, tab, tile[wid], flag);
+}
+
+// Batched form: the waves of every shot of a batch in one launch.  Each
+// shot's waves fill whole blocks (blocks never straddle shots), so a block
+// finds its shot by a binary search over the shots' first blocks and keeps
+// one taper table.  The block order is XCD-aware as above, so each XCD
+// marches a contiguous run of tiles.
+template <typename T, int R, int RY, int WPB, bool IMG, bool TAPER>
+__global__ void __launch_bounds__(64 * WPB)
+    k_stencil_batch(const BatchShot *__restrict__ shots, int nshots, long long nblocks, int par,
+                    int pspar, const Coefs cf, const TaperFuse tf, int *__restrict__ flag)
+{
+    constexpr int LW = 64 + 2 * R;
+    __shared__ float tile[WPB][RY][LW];
+    extern __shared__ float tab[];
+    const int wid = __builtin_amdgcn_readfirstlane(threadIdx.y);
+    if (TAPER) {
+        const int tid = threadIdx.y * 64 + threadIdx.x;
+        for (int i = tid; i < tf.b; i += 64 * WPB) tab[i] = tf.t[i];
+        __syncthreads();
+    }
+    long long t;
+    {
+        const long long h = blockIdx.x;
+        const long long q = nblocks >> 3, rr = nblocks & 7, k = h & 7, j = h >> 3;
+        t = k * q + (k < rr ? k : rr) + j;
+    }
+    int lo = 0, hi = nshots - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (shots[mid].blk0 <= t) lo = mid;
+        else hi = mid - 1;
+    }
+    const BatchShot &sh = shots[lo];
+    const long long w = (t - sh.blk0) * WPB + wid;
+    if (w >= sh.nwaves) return;
+    const int zt = (int)(w % sh.nzt);
+    const long long w2 = w / sh.nzt;
+    const int yt = (int)(w2 % sh.nyt);
+    const int xc = (int)(w2 / sh.nyt);
+    const int xb = R + xc * sh.xchunk;
+    const int xe = min(xb + sh.xchunk, sh.g.nx - R);
+    void *const *f = IMG ? sh.fr : sh.fs;
+    const T *P = static_cast<const T *>(f[par]);
+    T *PP = static_cast<T *>(f[par ^ 1]);
+    ImageFuse fz{};
+    if (IMG) {
+        fz.img = sh.img;
+        fz.ps = sh.fs[pspar];
+        fz.z0 = tf.b;
+        fz.z1 = sh.g.nz - tf.b;
+        fz.rz = sh.rz;
+        fz.rx0 = sh.rx0;
+        fz.rdx = sh.rdx;
+        fz.rnx = sh.rnx;
+        fz.ry0 = sh.ry0;
+        fz.rdy = sh.rdy;
+        fz.rny = sh.rny;
+    }
+    wave_tile<T, R, RY, IMG, TAPER>(P, PP, sh.V, cf, sh.g, zt, yt, xb, xe, fz, tf, tab,
+                                    tile[wid], flag);
 }
 
 // ---------------------------------------------------------------------------
@@ -1765,6 +1840,22 @@ hipError_t launch_radius(const T *P, T *PP, const float *V, const Coefs &cf, con
     }
 }
 
+constexpr int kBatchWPB = 4;
+
+template <typename T, int RY, bool IMG, bool TAPER>
+hipError_t launch_batch_cfg(const BatchShot *shots, int n, long long nblocks, int par, int pspar,
+                            const Coefs &cf, const TaperFuse &tf, int *flag, hipStream_t s)
+{
+    constexpr int R = 4;
+    auto kern = k_stencil_batch<T, R, RY, kBatchWPB, IMG, TAPER>;
+    const size_t lds = TAPER ? (size_t)tf.b * sizeof(float) : 0;
+    if (nblocks <= 0) return hipSuccess;
+    if (nblocks > 0x7fffffffLL) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(kern, dim3((unsigned)nblocks), dim3(64, kBatchWPB), lds, s, shots, n,
+                       nblocks, par, pspar, cf, tf, flag);
+    return hipGetLastError();
+}
+
 template <typename T, int R, int RY, int WPB, bool TAPER>
 hipError_t launch_dual_cfg(const T *Ps, T *PPs, const T *Pr, T *PPr, const float *V,
                            const Coefs &cf, const Grid &g, const ImageFuse &fz,
@@ -1841,6 +1932,72 @@ hipError_t launch_stencil(int radius, const T *P, T *PP, const float *V, const C
         return launch_radius<T, 4>(P, PP, V, cf, g, fuse, taper, flag, s, tiling);
     }
 }
+
+long long plan_stencil_batch(int ry, BatchShot *hs, int n, int R, int blen)
+{
+    if (R != 4 || (ry != 6 && ry != 8) || n <= 0) return -1;
+    // x chunks: one count for the batch, from the total tile count and the
+    // resident-wave capacity of the kernel
+    const void *fn = ry == 6 ? (const void *)k_stencil_batch<float, 4, 6, kBatchWPB, true, true>
+                             : (const void *)k_stencil_batch<float, 4, 8, kBatchWPB, true, true>;
+    const int cap = device_capacity(fn, 64 * kBatchWPB, (size_t)blen * sizeof(float)) * kBatchWPB;
+    long long tiles = 0;
+    int nxmin = 1 << 30;
+    for (int i = 0; i < n; ++i) {
+        const Grid &g = hs[i].g;
+        hs[i].nzt = (g.nz + 63) / 64;
+        hs[i].nyt = (g.ny - 2 * R + ry - 1) / ry;
+        if (hs[i].nyt <= 0 || g.nx - 2 * R <= 0 || g.nz - 2 * R <= 0) return -1;
+        tiles += (long long)hs[i].nzt * hs[i].nyt;
+        nxmin = std::min(nxmin, g.nx - 2 * R);
+    }
+    const int nc = choose_chunks(tiles, nxmin, R, cap);
+    long long blk = 0;
+    for (int i = 0; i < n; ++i) {
+        const int nxi = hs[i].g.nx - 2 * R;
+        hs[i].xchunk = (nxi + nc - 1) / nc;
+        const long long nxc = (nxi + hs[i].xchunk - 1) / hs[i].xchunk;
+        hs[i].nwaves = (long long)hs[i].nzt * hs[i].nyt * nxc;
+        hs[i].blk0 = blk;
+        blk += (hs[i].nwaves + kBatchWPB - 1) / kBatchWPB;
+    }
+    return blk;
+}
+
+template <typename T>
+hipError_t launch_stencil_batch(int ry, const BatchShot *shots, const BatchShot *hs, int n,
+                                long long nblocks, bool img, int par, int pspar, const Coefs &cf,
+                                const TaperFuse *taper, int *flag, hipStream_t s)
+{
+    (void)hs;
+    TaperFuse tf{};
+    if (taper) tf = *taper;
+    else tf.b = 0;
+    if (img) tf.b = taper ? taper->b : tf.b;
+    auto go = [&](auto ryc) -> hipError_t {
+        constexpr int RY = decltype(ryc)::value;
+        if (img && taper)
+            return launch_batch_cfg<T, RY, true, true>(shots, n, nblocks, par, pspar, cf, tf, flag,
+                                                       s);
+        if (taper)
+            return launch_batch_cfg<T, RY, false, true>(shots, n, nblocks, par, pspar, cf, tf,
+                                                        flag, s);
+        if (img) return hipErrorInvalidValue;  // the batch images HBC steps only
+        return launch_batch_cfg<T, RY, false, false>(shots, n, nblocks, par, pspar, cf, tf, flag,
+                                                     s);
+    };
+    if (ry == 6) return go(std::integral_constant<int, 6>{});
+    if (ry == 8) return go(std::integral_constant<int, 8>{});
+    return hipErrorInvalidValue;
+}
+
+#define RTM_INSTANTIATE_BATCH(T)                                                                 \
+    template hipError_t launch_stencil_batch<T>(int, const BatchShot *, const BatchShot *, int,  \
+                                                long long, bool, int, int, const Coefs &,        \
+                                                const TaperFuse *, int *, hipStream_t);
+RTM_INSTANTIATE_BATCH(float)
+RTM_INSTANTIATE_BATCH(_Float16)
+RTM_INSTANTIATE_BATCH(__bf16)
 
 template hipError_t launch_stencil<float>(int, const float *, float *, const float *,
                                           const Coefs &, const Grid &, const ImageFuse *,

@@ -22,6 +22,9 @@ EXPORTED = [
     "rtmhip_upper_border", "rtmhip_image_condition", "rtmhip_image_finish", "rtmhip_stack",
     "rtmhip_model_steps", "rtmhip_forward_steps", "rtmhip_backward_steps",
     "rtmhip_set_tiling", "rtmhip_get_tiling", "rtmhip_timing_enable", "rtmhip_timing_read",
+    "rtmhip_batch_create", "rtmhip_batch_destroy", "rtmhip_batch_forward_steps",
+    "rtmhip_batch_backward_steps", "rtmhip_batch_fields", "rtmhip_batch_set_stencil",
+    "rtmhip_batch_set_graph",
 ]
 
 
@@ -40,6 +43,13 @@ class Shot(C.Structure):
     _fields_ = [("nxe", C.c_int), ("nye", C.c_int), ("nze", C.c_int), ("blen", C.c_int),
                 ("order", C.c_int), ("nt", C.c_int), ("ntstep", C.c_int), ("sxe", C.c_int),
                 ("sye", C.c_int), ("sze", C.c_int), ("rcv", RcvGeom), ("wave_format", C.c_int)]
+
+
+class BatchShotDesc(C.Structure):
+    """rtmhip_batch_shot."""
+    _fields_ = [("shot", Shot), ("V2DT2", C.c_void_p), ("ricker", C.c_void_p),
+                ("fs", C.c_void_p * 2), ("fr", C.c_void_p * 2), ("upb", C.c_void_p),
+                ("rcv", C.c_void_p), ("img", C.c_void_p)]
 
 
 # Wavefield storage formats (RTMHIP_WAVE_*): name -> (code, numpy storage dtype).
@@ -84,6 +94,13 @@ def load():
         "rtmhip_get_tiling": [i] + [C.POINTER(C.c_int)] * 4,
         "rtmhip_timing_enable": [i],
         "rtmhip_timing_read": [i, C.POINTER(C.c_longlong), C.POINTER(C.c_double)],
+        "rtmhip_batch_create": [pp, C.POINTER(BatchShotDesc), i, fp, fp],
+        "rtmhip_batch_destroy": [vp],
+        "rtmhip_batch_forward_steps": [vp, i, i],
+        "rtmhip_batch_backward_steps": [vp, i, i],
+        "rtmhip_batch_fields": [vp, i, pp, pp, pp, pp],
+        "rtmhip_batch_set_stencil": [vp, i, i],
+        "rtmhip_batch_set_graph": [vp, i],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -287,3 +304,55 @@ def timing_read(kernel_class):
     ms = C.c_double(0.0)
     check(load().rtmhip_timing_read(kernel_class, C.byref(n), C.byref(ms)))
     return n.value, ms.value
+
+
+# ---------------- batches of shots ----------------
+class Batch:
+    """Many HBC shots advanced in lockstep (rtmhip_batch_*).  `members` is a
+    list of dicts with keys shot (Shot), V, ricker, S (FieldPair, source),
+    R (FieldPair, receiver), upb, rcv, img (DeviceArrays); the FieldPairs'
+    current roles are taken as (P, PP) and are kept in step with the batch."""
+
+    def __init__(self, members, coefs, g):
+        self.members = members
+        n = len(members)
+        arr = (BatchShotDesc * n)()
+        for k, m in enumerate(members):
+            d = arr[k]
+            d.shot = m["shot"]
+            d.V2DT2 = m["V"].ptr.value
+            d.ricker = m["ricker"].ptr.value
+            d.fs[0], d.fs[1] = m["S"].p.value, m["S"].pp.value
+            d.fr[0], d.fr[1] = m["R"].p.value, m["R"].pp.value
+            d.upb = m["upb"].ptr.value
+            d.rcv = m["rcv"].ptr.value
+            d.img = m["img"].ptr.value
+        c = _f32(coefs)
+        h = C.c_void_p()
+        check(load().rtmhip_batch_create(C.byref(h), arr, n, _host_ptr(c), g.ptr))
+        self.h = h
+
+    def _sync_roles(self):
+        L = load()
+        for k, m in enumerate(self.members):
+            check(L.rtmhip_batch_fields(self.h, k, C.byref(m["S"].p), C.byref(m["S"].pp),
+                                        C.byref(m["R"].p), C.byref(m["R"].pp)))
+
+    def forward_steps(self, it0, nsteps):
+        check(load().rtmhip_batch_forward_steps(self.h, it0, nsteps))
+        self._sync_roles()
+
+    def backward_steps(self, it_hi, nsteps):
+        check(load().rtmhip_batch_backward_steps(self.h, it_hi, nsteps))
+        self._sync_roles()
+
+    def set_stencil(self, kernel_class, mode):
+        check(load().rtmhip_batch_set_stencil(self.h, kernel_class, mode))
+
+    def set_graph(self, steps):
+        check(load().rtmhip_batch_set_graph(self.h, steps))
+
+    def destroy(self):
+        if self.h:
+            check(load().rtmhip_batch_destroy(self.h))
+            self.h = C.c_void_p()
