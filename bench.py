@@ -218,6 +218,130 @@ def accuracy_crop(hip, host, workloads, formats=("f32",)):
         "80x80x64 crop of the salt model, border 16, 400 steps, 20x20 receivers"
 
 
+class BatchShots:
+    """Device state of a batch of shots of the 1024 x 1024 x 320 model: each
+    shot's window (uneven apertures), its record synthesised by the modeling
+    path, and the buffers of its migration."""
+
+    def __init__(self, hip, host, workloads, nshots, nt):
+        self.hip, self.nt = hip, nt
+        wins = workloads.salt_1024_shots(64)[:nshots]
+        p0 = workloads.salt_1024_params(wins[0][2], wins[0][3], wins[0][4], nt)
+        b, order = p0["blen"], p0["stencil_order"]
+        self.b = b
+        self.coefs = host.stencil_coefs(order, p0["dx"], p0["dy"], p0["dz"])
+        self.g = hip.DeviceArray.from_host(host.taper_table(b, p0["taper_factor"]))
+        self.src = hip.DeviceArray.from_host(host.ricker(nt, p0["dt"], p0["fpeak"]))
+        self.members, self.grids = [], []
+        self.U = 0.0
+        t = time.time()
+        for x0, y0, nxy, nz, src in wins:
+            p = workloads.salt_1024_params(nxy, nz, src, nt)
+            vel = workloads.salt_1024_window(x0, y0, nxy, nxy, nz)
+            ve = host.extend_velocity(vel, b, "abc")
+            shape = ve.shape
+            V = hip.DeviceArray.from_host(host.v2dt2(ve, p["dt"]))
+            geom = hip.RcvGeom(0, 0, 0, 4, 4, nxy // 4, nxy // 4)
+            shot = hip.make_shot(*shape, b, order, nt, nt, (src[0] + b, src[1] + b, src[2] + b),
+                                 geom)
+            m = dict(shot=shot, V=V, ricker=self.src, S=hip.FieldPair(shape),
+                     R=hip.FieldPair(shape),
+                     rcv=hip.DeviceArray((nxy // 4, nxy // 4, nt)).zero(),
+                     img=hip.DeviceArray(shape).zero(),
+                     upb=hip.DeviceArray((nt, shape[0], shape[1], order // 2)))
+            hip.model_steps(shot, V, self.coefs, self.g, self.src, m["S"], m["rcv"], 0, nt)
+            V.upload(host.v2dt2(host.extend_velocity(vel, b, "hbc"), p["dt"]))
+            self.members.append(m)
+            self.grids.append(list(shape))
+            self.U += float((shape[0] - order) * (shape[1] - order) * (shape[2] - order))
+        hip.synchronize()
+        log("batch: %d shot windows built and modelled in %.1f s" % (nshots, time.time() - t))
+
+    def reset(self):
+        for m in self.members:
+            for f in (m["S"], m["R"]):
+                f.a.zero()
+                f.b.zero()
+                f.p.value, f.pp.value = f.a.ptr.value, f.b.ptr.value
+            m["img"].zero()
+
+    def single(self):
+        """The shots one at a time through the per-shot loops (plain FP32
+        single-shot GPU RTM)."""
+        hip, nt = self.hip, self.nt
+        for m in self.members:
+            hip.forward_steps(m["shot"], True, m["V"], self.coefs, self.g, self.src, m["S"],
+                              m["upb"], 0, nt)
+            hip.backward_steps(m["shot"], True, m["V"], self.coefs, self.g, m["S"], m["R"],
+                               m["upb"], m["rcv"], m["img"], nt - 1, nt, 1)
+
+    def free(self):
+        for m in self.members:
+            for k in ("V", "S", "R", "rcv", "img", "upb"):
+                m[k].free()
+        self.g.free()
+        self.src.free()
+
+
+def batch_leg(hip, host, workloads, nshots, nt, tune=True):
+    """Batched RTM of shots of the 1024 x 1024 x 320 model against the same
+    shots run one at a time; images compared bit for bit."""
+    from rtm3d_demo_amd import autotune
+
+    bs = BatchShots(hip, host, workloads, nshots, nt)
+    bs.reset()
+    hip.synchronize()
+    t = time.perf_counter()
+    bs.single()
+    hip.synchronize()
+    t_single = time.perf_counter() - t
+    log("batch leg: %d shots one at a time: %.2f s" % (nshots, t_single))
+    want = [m["img"].download() for m in bs.members]
+    bs.reset()
+    batch = hip.Batch(bs.members, bs.coefs, bs.g)
+    modes = None
+    if tune:
+        t = time.time()
+        best = autotune.tune_batch(hip, batch, nt, log=log)
+        modes = {str(c): {"mode": m, "ms_per_step": round(ms, 3)} for c, (m, ms) in
+                 sorted(best.items())}
+        log("batch autotune (%.1f s): %s" % (time.time() - t, modes))
+        batch.destroy()
+        bs.reset()
+        batch = hip.Batch(bs.members, bs.coefs, bs.g)
+        for c, (m, _) in best.items():
+            batch.set_stencil(c, m)
+    hip.synchronize()
+    t = time.perf_counter()
+    batch.forward_steps(0, nt)
+    batch.backward_steps(nt - 1, nt)
+    hip.synchronize()
+    t_batch = time.perf_counter() - t
+    log("batch leg: %d shots batched: %.2f s" % (nshots, t_batch))
+    same = all(np.array_equal(w.view(np.uint32), m["img"].download().view(np.uint32))
+               for w, m in zip(want, bs.members))
+    del want
+    batch.destroy()
+    updates = bs.U * (3 * nt - 2)
+    out = {
+        "workload": "salt1024_batch%d_nt%d" % (nshots, nt),
+        "model": [1024, 1024, 320], "dx_m": 12.5, "border": 40, "nt": nt, "shots": nshots,
+        "shot_selection": "the first %d of 64 seeded windows (nx = ny in 256..448 step 32, "
+                          "nz in {256, 288, 320})" % nshots,
+        "extended_grids": bs.grids,
+        "batched_s": round(t_batch, 3), "single_s": round(t_single, 3),
+        "batched_mcell_updates_per_s": round(updates / t_batch / 1e6, 1),
+        "single_mcell_updates_per_s": round(updates / t_single / 1e6, 1),
+        "batched_shots_per_hour": round(3600.0 * nshots / t_batch, 1),
+        "single_shots_per_hour": round(3600.0 * nshots / t_single, 1),
+        "speedup": round(t_single / t_batch, 3),
+        "images_bit_identical": bool(same),
+        "stencil_modes": modes, "graph_steps": 8,
+    }
+    bs.free()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -229,6 +353,9 @@ def main():
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--no-precision", action="store_true",
                     help="skip the FP16 / BF16 wavefield-storage shots")
+    ap.add_argument("--batch-shots", type=int, default=8,
+                    help="shots of the batched leg (0 skips it)")
+    ap.add_argument("--batch-nt", type=int, default=2500)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on one GPU only: the propagation path is measured on a single "
@@ -347,6 +474,9 @@ def main():
         line["wave_formats"] = prec
     for x in (sh.V, sh.src, sh.g, sh.F, sh.R, sh.rcv, sh.img, sh.upb):
         x.free()
+    if args.batch_shots > 0:
+        line["batched"] = batch_leg(hip, host, workloads, args.batch_shots, args.batch_nt,
+                                    tune=not args.no_tune)
 
     if not args.no_cpu:
         cores = min(16, os.cpu_count() or 1)

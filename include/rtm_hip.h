@@ -256,12 +256,17 @@ int rtmhip_batch_set_graph(rtmhip_batch *batch, int steps);
  * RTMGPUPlatform.cpp:176-185).
  *
  * rtmhip_set_tiling selects the tile shape of one class: tz lanes along z,
- * by thread rows, ry rows per thread (compiled set: 64x4x4, 64x4x2, 64x8x2,
- * 32x8x4, 32x8x2; tz = 1 selects full-z-row blocks for order 8, by x ry =
- * 4x4 for nze <= 256, 2x6 or 2x7 for nze <= 384), xchunk planes marched per
- * block (0 = sized from the device's resident-block capacity).  tz = 0
- * restores the default: full-z-row blocks where they fit, else 64x4x4.
- * 16-bit wavefields use the default shapes. */
+ * by thread rows, ry rows per thread, xchunk planes marched per block (0 =
+ * sized from the device's resident-block capacity).  Compiled shapes:
+ *   tz = 64: tiles 64x4x4, 64x4x2, 64x8x2; barrier-free waves 64x1x6, 64x1x8;
+ *   tz = 32: tiles 32x8x4, 32x8x2;
+ *   tz = 1 (order 8): full-z-row blocks 4x4 (nze <= 256), 2x6 and 2x7
+ *           (nze <= 384);
+ *   tz = 2 (order 8, even nze <= 384): full rows, two z per lane, 4x3.
+ * tz = 0 restores the default: full-z-row blocks where they fit, else (FP32,
+ * order 8) 64x1x8 waves, else 64x4x4; a full-row request that does not
+ * cover nze takes the same fallback.  16-bit wavefields use the default
+ * shapes. */
 int rtmhip_set_tiling(int kernel_class, int tz, int by, int ry, int xchunk);
 int rtmhip_get_tiling(int kernel_class, int *tz, int *by, int *ry, int *xchunk);
 

@@ -47,9 +47,45 @@ def tune(hip, shot, V, coefs, g, src, F, R, upb, rcv, img, steps=6, rounds=2, lo
     return best
 
 
+def tune_batch(hip, batch, nt, steps=8, log=None):
+    """Picks, per stencil class, the batch's launch form (0: one launch per
+    shot with the class tiling, 6 / 8: one launch for every shot) by timing
+    `steps` batched forward (class 2) or backward (classes 0, 1) steps with
+    each form; installs the fastest and returns {class: (mode, ms per step)}.
+    The batch's buffers are scratch for the duration of the call."""
+    import time
+
+    out = {}
+    for cls, fwd in ((2, True), (0, False), (1, False)):
+        best = None
+        for mode in (0, 6, 8):
+            batch.set_stencil(cls, mode)
+
+            def run():
+                if fwd:
+                    batch.forward_steps(0, steps)
+                else:
+                    batch.backward_steps(nt - 3, steps)
+
+            run()  # first pass captures the graphs
+            hip.synchronize()
+            t = time.perf_counter()
+            for _ in range(2):
+                run()
+            hip.synchronize()
+            ms = (time.perf_counter() - t) / (2 * steps) * 1e3
+            if log:
+                log("batch class %d mode %d: %.3f ms/step" % (cls, mode, ms))
+            if best is None or ms < best[1]:
+                best = (mode, ms)
+        batch.set_stencil(cls, best[0])
+        out[cls] = best
+    return out
+
+
 def describe(best):
     return {str(c): {"tile": "%dx%dx%d" % t, "avg_ms": round(ms, 4)} for c, (t, ms) in
             sorted(best.items())}
 
 
-__all__ = ["tune", "describe", "np"]
+__all__ = ["tune", "tune_batch", "describe", "np"]

@@ -794,7 +794,7 @@ struct rtmhip_batch {
     std::vector<BatchShot> hs[2];   // host tables: wave rows 6 and 8
     BatchShot *ds[2] = {nullptr, nullptr};
     long long nblk[2] = {0, 0};
-    long long fwd_items = 0, bwd_items = 0;
+    int fwd_blocks = 0, bwd_blocks = 0;
     int *ctr = nullptr;             // device time index of graph replays
     int cs = 0, cr = 0;             // P = f[c], PP = f[c ^ 1]
     int mode[3] = {8, 0, 0};        // per stencil class: 0 per-shot launches, 6/8 batched
@@ -829,7 +829,7 @@ int batch_fwd_step(rtmhip_batch *b, int taps, const int *itp, int itoff)
                    "stencil");
         }
     }
-    HIPCHK(launch_tail_fwd<T>(b->ds[0], b->n, b->fwd_items, b->cs, b->taper, b->blen, b->radius,
+    HIPCHK(launch_tail_fwd<T>(b->ds[0], b->n, b->fwd_blocks, b->cs, b->taper, b->blen, b->radius,
                               b->hf, taps, itp, itoff, s),
            "forward tail");
     return RTMHIP_OK;
@@ -881,7 +881,7 @@ int batch_bwd_step(rtmhip_batch *b, int taps, const int *itp, int itoff)
                    "stencil");
         }
     }
-    HIPCHK(launch_tail_bwd<T>(b->ds[0], b->n, b->bwd_items, b->cs, b->cr, b->taper, b->blen,
+    HIPCHK(launch_tail_bwd<T>(b->ds[0], b->n, b->bwd_blocks, b->cs, b->cr, b->taper, b->blen,
                               b->radius, b->hf, taps, b->nt, itp, itoff, s),
            "backward tail");
     b->cs ^= 1;
@@ -1070,13 +1070,22 @@ int rtmhip_batch_create(rtmhip_batch **out, const rtmhip_batch_shot *shots, int 
     b->taper = taper;
     long long fw = 0, bw = 0;
     for (auto &h : b->hs[0]) {
-        h.fwd0 = fw;
-        h.bwd0 = bw;
-        fw += tail_fwd_items(h.g, b->blen, b->radius, b->hf);
-        bw += tail_bwd_items(h.g, b->blen, b->radius, b->hf, h.rnx, h.rny);
+        const long long nf = tail_fwd_items(h.g, b->blen, b->radius, b->hf);
+        const long long nb = tail_bwd_items(h.g, b->blen, b->radius, b->hf, h.rnx, h.rny);
+        if (nf >= (1LL << 31) || nb >= (1LL << 31) ||
+            (long long)h.g.nx * h.g.ny * b->hf >= (1LL << 31))
+            return fail(RTMHIP_ERR_INVALID, "a shot's border work exceeds 2^31 cells");
+        h.fwd0 = (int)fw;
+        h.bwd0 = (int)bw;
+        h.nfwd = (int)nf;
+        h.nbwd = (int)nb;
+        fw += tail_blocks(nf);
+        bw += tail_blocks(nb);
+        if (fw >= (1LL << 31) || bw >= (1LL << 31))
+            return fail(RTMHIP_ERR_INVALID, "batch too large");
     }
-    b->fwd_items = fw;
-    b->bwd_items = bw;
+    b->fwd_blocks = (int)fw;
+    b->bwd_blocks = (int)bw;
     b->hs[1] = b->hs[0];
     b->nblk[0] = plan_stencil_batch(6, b->hs[0].data(), nshots, b->radius, b->blen);
     b->nblk[1] = plan_stencil_batch(8, b->hs[1].data(), nshots, b->radius, b->blen);

@@ -96,8 +96,8 @@ struct BatchShot {
     int nzt, nyt, xchunk;                   // waves of the batched stencil
     long long nwaves;                       // nzt * nyt * (x chunks)
     long long blk0;      // first block of this shot in a batched stencil launch
-    long long fwd0;      // first work item of this shot in the forward tail
-    long long bwd0;      // first work item of this shot in the backward tail
+    int fwd0, nfwd;      // first block and work items of this shot in the forward tail
+    int bwd0, nbwd;      // ... in the backward tail
 };
 
 // Tile shapes compiled into the stencil kernel (tz lanes x by rows, ry rows
@@ -164,16 +164,17 @@ long long plan_stencil_batch(int ry, BatchShot *hs, int n, int R, int blen);
 // fr[rpar ^ 1], receiver injection and the imaging condition off the
 // stencil's cells.  it = (itp ? *itp : 0) + itoff.
 template <typename T>
-hipError_t launch_tail_fwd(const BatchShot *shots, int n, long long items, int par,
+hipError_t launch_tail_fwd(const BatchShot *shots, int n, int blocks, int par,
                            const float *taper, int b, int R, int hf, int taps, const int *itp,
                            int itoff, hipStream_t s);
 template <typename T>
-hipError_t launch_tail_bwd(const BatchShot *shots, int n, long long items, int spar, int rpar,
+hipError_t launch_tail_bwd(const BatchShot *shots, int n, int blocks, int spar, int rpar,
                            const float *taper, int b, int R, int hf, int taps, int nt,
                            const int *itp, int itoff, hipStream_t s);
-// Work items of the tails for one shot.
+// Work items of the tails for one shot, and blocks for a count of items.
 long long tail_fwd_items(const Grid &g, int b, int R, int hf);
 long long tail_bwd_items(const Grid &g, int b, int R, int hf, int rnx, int rny);
+int tail_blocks(long long items);
 // *p += d, or *p = d with set (one thread): the device-resident time index
 // of a captured step sequence.
 hipError_t launch_tick(int *p, int d, bool set, hipStream_t s);

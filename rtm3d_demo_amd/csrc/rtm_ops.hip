@@ -73,27 +73,28 @@ __device__ __forceinline__ bool in_frame(const Grid &g, int b, int R, int x, int
 {
     return z < b && (z < R || x < R || x >= g.nx - R || y < R || y >= g.ny - R);
 }
-__device__ __forceinline__ bool frame_cell(const Grid &g, int b, int R, long long i, int &x,
-                                           int &y, int &z)
+template <typename I>
+__device__ __forceinline__ bool frame_cell(const Grid &g, int b, int R, I i, int &x, int &y,
+                                           int &z)
 {
-    const long long nA = (long long)g.nx * g.ny * R;  // z < R, all x, y
-    const int lz = b - R;                             // z in [R, b)
-    const long long nB = (long long)2 * R * g.ny * lz;              // x frame
-    const long long nC = (long long)(g.nx - 2 * R) * 2 * R * lz;    // y frame
+    const I nA = (I)g.nx * g.ny * R;  // z < R, all x, y
+    const int lz = b - R;             // z in [R, b)
+    const I nB = (I)2 * R * g.ny * lz;              // x frame
+    const I nC = (I)(g.nx - 2 * R) * 2 * R * lz;    // y frame
     if (i < nA) {
         z = (int)(i % R);
-        const long long r = i / R;
+        const I r = i / R;
         y = (int)(r % g.ny);
         x = (int)(r / g.ny);
     } else if ((i -= nA) < nB) {
         z = R + (int)(i % lz);
-        const long long r = i / lz;
+        const I r = i / lz;
         y = (int)(r % g.ny);
         const int xx = (int)(r / g.ny);
         x = xx < R ? xx : g.nx - 2 * R + xx;
     } else if ((i -= nB) < nC) {
         z = R + (int)(i % lz);
-        const long long r = i / lz;
+        const I r = i / lz;
         const int yy = (int)(r % (2 * R));
         x = R + (int)(r / (2 * R));
         y = yy < R ? yy : g.ny - 2 * R + yy;
@@ -279,22 +280,23 @@ __device__ __forceinline__ long long image_frame_count(const Grid &g, int R, int
     const long long lz = z1 > z0 ? z1 - z0 : 0;
     return (long long)2 * R * g.ny * lz + (long long)(g.nx - 2 * R) * 2 * R * lz;
 }
-__device__ __forceinline__ bool image_frame_cell(const Grid &g, int R, int z0, int z1,
-                                                 long long i, int &x, int &y, int &z)
+template <typename I>
+__device__ __forceinline__ bool image_frame_cell(const Grid &g, int R, int z0, int z1, I i,
+                                                 int &x, int &y, int &z)
 {
     const int lz = z1 - z0;
     if (lz <= 0) return false;
-    const long long nA = (long long)2 * R * g.ny * lz;
-    const long long nB = (long long)(g.nx - 2 * R) * 2 * R * lz;
+    const I nA = (I)2 * R * g.ny * lz;
+    const I nB = (I)(g.nx - 2 * R) * 2 * R * lz;
     if (i < nA) {
         z = z0 + (int)(i % lz);
-        const long long r = i / lz;
+        const I r = i / lz;
         y = (int)(r % g.ny);
         const int xx = (int)(r / g.ny);
         x = xx < R ? xx : g.nx - 2 * R + xx;
     } else if ((i -= nA) < nB) {
         z = z0 + (int)(i % lz);
-        const long long r = i / lz;
+        const I r = i / lz;
         const int yy = (int)(r % (2 * R));
         x = R + (int)(r / (2 * R));
         y = yy < R ? yy : g.ny - 2 * R + yy;
@@ -422,16 +424,17 @@ __global__ void __launch_bounds__(kThreads)
         d[i] = d[i] + s[i];
 }
 
-// Batched tails: one launch for every shot of a batch.  A thread finds its
-// shot by a binary search over the shots' first work items.
-template <long long BatchShot::*first>
-__device__ __forceinline__ int batch_shot(const BatchShot *__restrict__ shots, int n,
-                                          long long gi)
+// Batched tails: one launch for every shot of a batch.  Each shot's work
+// items fill whole blocks, so a block finds its shot (a binary search over
+// the shots' first blocks) with block-uniform scalar loads; per-shot item
+// counts stay below 2^31 (checked when the batch is made).
+template <int BatchShot::*first>
+__device__ __forceinline__ int batch_shot(const BatchShot *__restrict__ shots, int n, int blk)
 {
     int lo = 0, hi = n - 1;
     while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;
-        if (shots[mid].*first <= gi) lo = mid;
+        if (shots[mid].*first <= blk) lo = mid;
         else hi = mid - 1;
     }
     return lo;
@@ -445,29 +448,27 @@ __device__ __forceinline__ int batch_shot(const BatchShot *__restrict__ shots, i
 // and the source cell when it lies in neither.
 template <typename T>
 __global__ void __launch_bounds__(kThreads)
-    k_tail_fwd(const BatchShot *__restrict__ shots, int n, long long items, int par,
-               const float *__restrict__ t, int b, int R, int hf, int taps,
-               const int *__restrict__ itp, int itoff)
+    k_tail_fwd(const BatchShot *__restrict__ shots, int n, int par, const float *__restrict__ t,
+               int b, int R, int hf, int taps, const int *__restrict__ itp, int itoff)
 {
-    const long long gi = (long long)blockIdx.x * kThreads + threadIdx.x;
-    if (gi >= items) return;
-    const BatchShot &sh = shots[batch_shot<&BatchShot::fwd0>(shots, n, gi)];
+    const BatchShot &sh = shots[batch_shot<&BatchShot::fwd0>(shots, n, (int)blockIdx.x)];
+    int i = ((int)blockIdx.x - sh.fwd0) * kThreads + (int)threadIdx.x;
+    if (i >= sh.nfwd) return;
     const Grid g = sh.g;
     const int it = (itp ? *itp : 0) + itoff;
     T *PP = static_cast<T *>(sh.fs[par ^ 1]);
-    long long i = gi - sh.fwd0;
-    const long long nl = (long long)g.nx * g.ny * hf;
+    const int nl = g.nx * g.ny * hf;
     int x, y, z;
     bool save = false;
     if (i < nl) {
-        const long long r = i / hf;
+        const int r = i / hf;
         z = b - hf + (int)(i % hf);
         y = (int)(r % g.ny);
         x = (int)(r / g.ny);
         save = true;
     } else {
         i -= nl;
-        const long long nf = frame_count(g, b, R);
+        const int nf = (int)frame_count(g, b, R);
         if (i < nf) {
             frame_cell(g, b, R, i, x, y, z);
             if (z >= b - hf) return;  // a layer cell: its layer item handles it
@@ -502,21 +503,20 @@ __global__ void __launch_bounds__(kThreads)
 // launch_receivers_image run in sequence.
 template <typename T>
 __global__ void __launch_bounds__(kThreads)
-    k_tail_bwd(const BatchShot *__restrict__ shots, int n, long long items, int spar, int rpar,
+    k_tail_bwd(const BatchShot *__restrict__ shots, int n, int spar, int rpar,
                const float *__restrict__ t, int b, int R, int hf, int taps, int nt,
                const int *__restrict__ itp, int itoff)
 {
-    const long long gi = (long long)blockIdx.x * kThreads + threadIdx.x;
-    if (gi >= items) return;
-    const BatchShot &sh = shots[batch_shot<&BatchShot::bwd0>(shots, n, gi)];
+    const BatchShot &sh = shots[batch_shot<&BatchShot::bwd0>(shots, n, (int)blockIdx.x)];
+    int i = ((int)blockIdx.x - sh.bwd0) * kThreads + (int)threadIdx.x;
+    if (i >= sh.nbwd) return;
     const Grid g = sh.g;
     const int it = (itp ? *itp : 0) + itoff;
     T *PPs = static_cast<T *>(sh.fs[spar ^ 1]);
     T *PPr = static_cast<T *>(sh.fr[rpar ^ 1]);
-    long long i = gi - sh.bwd0;
-    const long long nl = (long long)g.nx * g.ny * hf;
+    const int nl = g.nx * g.ny * hf;
     if (i < nl) {
-        const long long r = i / hf;
+        const int r = i / hf;
         const int k = (int)(i % hf);
         PPs[goff(g, (int)(r / g.ny), (int)(r % g.ny), b - hf + k)] =
             static_cast<const T *>(sh.upb)[(long long)it * nl + i];
@@ -524,7 +524,7 @@ __global__ void __launch_bounds__(kThreads)
     }
     i -= nl;
     int x, y, z;
-    const long long nf = frame_count(g, b, R);
+    const int nf = (int)frame_count(g, b, R);
     if (i < nf) {
         frame_cell(g, b, R, i, x, y, z);
         const long long o = goff(g, x, y, z);
@@ -533,11 +533,11 @@ __global__ void __launch_bounds__(kThreads)
     }
     i -= nf;
     const int z0 = b, z1 = g.nz - b;
-    const long long nrcv = (long long)sh.rnx * sh.rny;
+    const int nrcv = sh.rnx * sh.rny;
     if (i < nrcv) {
         const int ix = (int)(i / sh.rny), iy = (int)(i % sh.rny);
         inject_image(PPr, sh.img, (const T *)PPs, g, sh.rx0 + ix * sh.rdx, sh.ry0 + iy * sh.rdy,
-                     sh.rz, sh.rcv[i * nt + it], z0, z1);
+                     sh.rz, sh.rcv[(long long)i * nt + it], z0, z1);
         return;
     }
     if (!image_frame_cell(g, R, z0, z1, i - nrcv, x, y, z)) return;
@@ -701,25 +701,27 @@ long long tail_bwd_items(const Grid &g, int b, int R, int hf, int rnx, int rny)
            (long long)(g.nx - 2 * R) * 2 * R * li;
 }
 
+int tail_blocks(long long items) { return (int)((items + kThreads - 1) / kThreads); }
+
 template <typename T>
-hipError_t launch_tail_fwd(const BatchShot *shots, int n, long long items, int par,
+hipError_t launch_tail_fwd(const BatchShot *shots, int n, int blocks, int par,
                            const float *taper, int b, int R, int hf, int taps, const int *itp,
                            int itoff, hipStream_t s)
 {
-    if (items <= 0 || n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_tail_fwd<T>, dim3(nblocks(items)), dim3(kThreads), 0, s, shots, n, items,
-                       par, taper, b, R, hf, taps, itp, itoff);
+    if (blocks <= 0 || n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_tail_fwd<T>, dim3(blocks), dim3(kThreads), 0, s, shots, n, par, taper, b,
+                       R, hf, taps, itp, itoff);
     return hipGetLastError();
 }
 
 template <typename T>
-hipError_t launch_tail_bwd(const BatchShot *shots, int n, long long items, int spar, int rpar,
+hipError_t launch_tail_bwd(const BatchShot *shots, int n, int blocks, int spar, int rpar,
                            const float *taper, int b, int R, int hf, int taps, int nt,
                            const int *itp, int itoff, hipStream_t s)
 {
-    if (items <= 0 || n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_tail_bwd<T>, dim3(nblocks(items)), dim3(kThreads), 0, s, shots, n, items,
-                       spar, rpar, taper, b, R, hf, taps, nt, itp, itoff);
+    if (blocks <= 0 || n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_tail_bwd<T>, dim3(blocks), dim3(kThreads), 0, s, shots, n, spar, rpar,
+                       taper, b, R, hf, taps, nt, itp, itoff);
     return hipGetLastError();
 }
 
@@ -744,10 +746,9 @@ hipError_t launch_tick(int *p, int d, bool set, hipStream_t s)
                                       hipStream_t);                                             \
     template hipError_t launch_image<T>(float *, const T *, const T *, const Grid &, int, int,  \
                                         hipStream_t);                                           \
-    template hipError_t launch_tail_fwd<T>(const BatchShot *, int, long long, int,              \
-                                           const float *, int, int, int, int, const int *, int, \
-                                           hipStream_t);                                        \
-    template hipError_t launch_tail_bwd<T>(const BatchShot *, int, long long, int, int,         \
+    template hipError_t launch_tail_fwd<T>(const BatchShot *, int, int, int, const float *,    \
+                                           int, int, int, int, const int *, int, hipStream_t);  \
+    template hipError_t launch_tail_bwd<T>(const BatchShot *, int, int, int, int,               \
                                            const float *, int, int, int, int, int, const int *, \
                                            int, hipStream_t);
 RTM_INSTANTIATE_OPS(float)

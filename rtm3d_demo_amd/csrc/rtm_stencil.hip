@@ -1798,8 +1798,9 @@ hipError_t launch_radius(const T *P, T *PP, const float *V, const Coefs &cf, con
     if constexpr (R == 4) {
         // full-row blocks by default whenever a compiled shape covers nz (and
         // the taper table fits in LDS beside the plane buffers)
+        // (a two-z-per-lane request that does not fit nz lands here too)
         const bool want_rows =
-            (!t || t->tz == 1 || !full) && (!taper || taper->b <= 4096);
+            (!t || t->tz == 1 || t->tz == 2 || !full) && (!taper || taper->b <= 4096);
         if (want_rows) {
             int by = t ? t->by : 0, ry = t ? t->ry : 0;
             if (!t || !full) {
@@ -1819,12 +1820,20 @@ hipError_t launch_radius(const T *P, T *PP, const float *V, const Coefs &cf, con
                 if (by == 2 && ry == 7 && rows_fit(g.nz, 2, 7))
                     return launch_rows<T, R, 2, 7, 6>(P, PP, V, cf, g, fuse, taper, flag, s,
                                                       xchunk);
+                // a full-row shape that does not cover nz: barrier-free
+                // 64 x 1 x 8 waves (faster than the tiles at every measured
+                // nz for the plain and the imaging step)
+                if (!taper || taper->b <= 8192)
+                    return launch_wave<T, R, 8, 4>(P, PP, V, cf, g, fuse, taper, flag, s, xchunk);
             }
-            // a full-row shape that does not cover nz falls back to tiles
+            // 16-bit: a full-row shape that does not cover nz falls back to tiles
         }
     }
     if constexpr (full) {
-        if (!t || t->tz == 1 || (t->tz == 64 && t->by == 4 && t->ry == 4))
+        // the default tiles also serve the full-row, pair and wave shapes
+        // where those do not apply (orders below 8, large tapers)
+        if (!t || t->tz == 1 || t->tz == 2 || (t->tz == 64 && t->by == 1) ||
+            (t->tz == 64 && t->by == 4 && t->ry == 4))
             return launch_variant<T, R, 64, 4, 4>(P, PP, V, cf, g, fuse, taper, flag, s, xchunk);
         if (t->tz == 64 && t->by == 4 && t->ry == 2)
             return launch_variant<T, R, 64, 4, 2>(P, PP, V, cf, g, fuse, taper, flag, s, xchunk);

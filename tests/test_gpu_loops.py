@@ -195,8 +195,10 @@ def xchunk(request, gpu):
         gpu.set_tiling(cls, 0)
 
 
-# (40, 44, 90): two waves per z row; (32, 36, 340): six waves per z row.
-GRIDS = [(40, 44, 90), (32, 36, 340)]
+# (40, 44, 90): two waves per z row; (32, 36, 340): six waves per z row;
+# (30, 32, 420): seven, beyond every full-row and pair shape, so those
+# requests take the fallback (64x1x8 waves).
+GRIDS = [(40, 44, 90), (32, 36, 340), (30, 32, 420)]
 
 
 @pytest.mark.parametrize("shape", GRIDS)
