@@ -285,10 +285,20 @@ class BatchShots:
 
 def batch_leg(hip, host, workloads, nshots, nt, tune=True):
     """Batched RTM of shots of the 1024 x 1024 x 320 model against the same
-    shots run one at a time; images compared bit for bit."""
+    shots run one at a time (plain single-shot GPU RTM, with the per-shot
+    tile shapes the batch tuner found best on these grids); images compared
+    bit for bit."""
     from rtm3d_demo_amd import autotune
 
     bs = BatchShots(hip, host, workloads, nshots, nt)
+    modes = None
+    batch = hip.Batch(bs.members, bs.coefs, bs.g)
+    if tune:
+        t = time.time()
+        modes = autotune.tune_batch(hip, batch, nt, log=log)
+        log("batch autotune (%.1f s): %s" % (time.time() - t, modes))
+    chosen = {c: modes[c]["mode"] for c in modes} if modes else {}
+    batch.destroy()
     bs.reset()
     hip.synchronize()
     t = time.perf_counter()
@@ -299,18 +309,8 @@ def batch_leg(hip, host, workloads, nshots, nt, tune=True):
     want = [m["img"].download() for m in bs.members]
     bs.reset()
     batch = hip.Batch(bs.members, bs.coefs, bs.g)
-    modes = None
-    if tune:
-        t = time.time()
-        best = autotune.tune_batch(hip, batch, nt, log=log)
-        modes = {str(c): {"mode": m, "ms_per_step": round(ms, 3)} for c, (m, ms) in
-                 sorted(best.items())}
-        log("batch autotune (%.1f s): %s" % (time.time() - t, modes))
-        batch.destroy()
-        bs.reset()
-        batch = hip.Batch(bs.members, bs.coefs, bs.g)
-        for c, (m, _) in best.items():
-            batch.set_stencil(c, m)
+    for c, m in chosen.items():
+        batch.set_stencil(c, m)
     hip.synchronize()
     t = time.perf_counter()
     batch.forward_steps(0, nt)
@@ -336,7 +336,8 @@ def batch_leg(hip, host, workloads, nshots, nt, tune=True):
         "single_shots_per_hour": round(3600.0 * nshots / t_single, 1),
         "speedup": round(t_single / t_batch, 3),
         "images_bit_identical": bool(same),
-        "stencil_modes": modes, "graph_steps": 8,
+        "autotune": {str(c): v for c, v in sorted(modes.items())} if modes else None,
+        "graph_steps": 8,
     }
     bs.free()
     return out

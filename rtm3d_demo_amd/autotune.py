@@ -47,39 +47,62 @@ def tune(hip, shot, V, coefs, g, src, F, R, upb, rcv, img, steps=6, rounds=2, lo
     return best
 
 
+BATCH_SHOT_TILINGS = [(1, 4, 4), (1, 2, 6), (1, 2, 7), (2, 4, 3), (64, 1, 8)]
+
+
 def tune_batch(hip, batch, nt, steps=8, log=None):
-    """Picks, per stencil class, the batch's launch form (0: one launch per
-    shot with the class tiling, 6 / 8: one launch for every shot) by timing
-    `steps` batched forward (class 2) or backward (classes 0, 1) steps with
-    each form; installs the fastest and returns {class: (mode, ms per step)}.
-    The batch's buffers are scratch for the duration of the call."""
+    """Autotunes a batch's stencil launches per kernel class: one launch per
+    shot with each candidate tile shape (a full-row shape that does not cover
+    a shot's nz runs that shot with the fallback), or one launch for every
+    shot (barrier-free waves, 6 or 8 rows).  Each candidate is timed over
+    `steps` batched forward (class 2) or backward (classes 0, 1) steps.
+
+    Installs the fastest form per class on the batch, and the fastest
+    per-shot tile shape per class with set_tiling (so single-shot runs of the
+    same grids use it too).  Returns {class: {"mode", "tile", "ms_per_step",
+    "tile_ms_per_step"}}.  The batch's buffers are scratch for the duration
+    of the call."""
     import time
 
     out = {}
     for cls, fwd in ((2, True), (0, False), (1, False)):
-        best = None
-        for mode in (0, 6, 8):
-            batch.set_stencil(cls, mode)
+        def run():
+            if fwd:
+                batch.forward_steps(0, steps)
+            else:
+                batch.backward_steps(nt - 3, steps)
 
-            def run():
-                if fwd:
-                    batch.forward_steps(0, steps)
-                else:
-                    batch.backward_steps(nt - 3, steps)
-
+        def timed():
             run()  # first pass captures the graphs
             hip.synchronize()
             t = time.perf_counter()
             for _ in range(2):
                 run()
             hip.synchronize()
-            ms = (time.perf_counter() - t) / (2 * steps) * 1e3
+            return (time.perf_counter() - t) / (2 * steps) * 1e3
+
+        best_tile, best_mode = None, None
+        for tiling in BATCH_SHOT_TILINGS:
+            batch.set_stencil(cls, 0)
+            hip.set_tiling(cls, *tiling, 0)
+            ms = timed()
             if log:
-                log("batch class %d mode %d: %.3f ms/step" % (cls, mode, ms))
-            if best is None or ms < best[1]:
-                best = (mode, ms)
-        batch.set_stencil(cls, best[0])
-        out[cls] = best
+                log("batch class %d per-shot %s: %.3f ms/step" % (cls, tiling, ms))
+            if best_tile is None or ms < best_tile[1]:
+                best_tile = (tiling, ms)
+        hip.set_tiling(cls, *best_tile[0], 0)
+        best_mode = (0, best_tile[1])
+        for mode in (6, 8):
+            batch.set_stencil(cls, mode)
+            ms = timed()
+            if log:
+                log("batch class %d one launch, %d rows: %.3f ms/step" % (cls, mode, ms))
+            if ms < best_mode[1]:
+                best_mode = (mode, ms)
+        batch.set_stencil(cls, best_mode[0])
+        out[cls] = {"mode": best_mode[0], "ms_per_step": round(best_mode[1], 3),
+                    "tile": "%dx%dx%d" % best_tile[0],
+                    "tile_ms_per_step": round(best_tile[1], 3)}
     return out
 
 

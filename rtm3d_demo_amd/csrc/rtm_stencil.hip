@@ -627,6 +627,14 @@ __global__ void __launch_bounds__(64 * NWM * BY)
 //     instruction) orders writes and reads.
 // Waves of one block take consecutive z tiles of the same rows, so z-halo
 // loads hit lines their neighbours fetch.
+// Minimum waves per SIMD asked of the compiler for the barrier-free kernel:
+// the register budget (512 / waves) it must fit.
+template <int RY, bool IMG, bool TAPER>
+constexpr int wave_min_occupancy()
+{
+    return RY == 8 ? (IMG ? 2 : 3) : (IMG || TAPER ? 3 : 4);
+}
+
 struct WaveGrid {
     int nzt, nyt, nxc, xchunk;
     long long waves;  // nzt * nyt * nxc
@@ -867,7 +875,7 @@ __device__ __forceinline__ void st_at(char *base, unsigned off, U v)
 }
 
 template <typename T, int R, int RY, int WPB, bool IMG, bool TAPER>
-__global__ void __launch_bounds__(64 * WPB)
+__global__ void __launch_bounds__(64 * WPB, (wave_min_occupancy<RY, IMG, TAPER>()))
     k_stencil_wave(const T *__restrict__ P, T *__restrict__ PP, const float *__restrict__ V,
                    const Coefs cf, const Grid g, const WaveGrid wg, const ImageFuse fz,
                    const TaperFuse tf, const float *__restrict__ gt, int *__restrict__ flag)
@@ -906,7 +914,7 @@ __global__ void __launch_bounds__(64 * WPB)
 // one taper table.  The block order is XCD-aware as above, so each XCD
 // marches a contiguous run of tiles.
 template <typename T, int R, int RY, int WPB, bool IMG, bool TAPER>
-__global__ void __launch_bounds__(64 * WPB)
+__global__ void __launch_bounds__(64 * WPB, (wave_min_occupancy<RY, IMG, TAPER>()))
     k_stencil_batch(const BatchShot *__restrict__ shots, int nshots, long long nblocks, int par,
                     int pspar, const Coefs cf, const TaperFuse tf, int *__restrict__ flag)
 {
