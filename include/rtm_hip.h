@@ -247,6 +247,58 @@ int rtmhip_batch_set_stencil(rtmhip_batch *batch, int kernel_class, int mode);
  * count (default 8). */
 int rtmhip_batch_set_graph(rtmhip_batch *batch, int steps);
 
+/* ================= learned snapshot compression =================
+ * No reference counterpart (the reference keeps the source field by HBC
+ * reconstruction or by exact snapshots).  A 4-layer patch-convolution
+ * autoencoder on 32^3 bricks of a source-wavefield snapshot:
+ *   L1 2^3 patches 1->8 channels, tanh;  L2 4^3 patches 8->32 (the code);
+ *   L3, L4 the transposed layers back to 32^3, tanh after L3.
+ * A brick is stored as FP16 values normalised by its largest magnitude
+ * (scale, FP32), in the library's brick element order; its code is 2048
+ * FP16 values: 16x smaller than the FP16 brick (32x than FP32).  Every
+ * layer runs on the matrix cores (FP16 operands, FP32 accumulation), in
+ * inference and in training (forward, backward, Adam), on the device.
+ *
+ * A region [x0,x1) x [y0,y1) x [z0,z1) of an extended grid is cut into
+ * ceil(dx/32) x ceil(dy/32) x ceil(dz/32) bricks, x slowest, z fastest;
+ * cells past the region's end are zero in a brick and never written back. */
+typedef struct rtmhip_ae rtmhip_ae;
+
+int rtmhip_ae_param_count(void);
+/* Glorot-uniform weights from seed; max_bricks (1..4096) sizes the
+ * workspace (bricks per training batch or per encode/decode pass). */
+int rtmhip_ae_create(rtmhip_ae **ae, int max_bricks, unsigned long long seed);
+int rtmhip_ae_destroy(rtmhip_ae *ae);
+/* Parameters (which = 0) or the last gradient (which = 1) to the host; the
+ * layout is W1[8][8] b1[8] W2[512][32] b2[32] W3[32][512] b3[512]
+ * W4[8][8] b4[8], W[k][n] row-major. */
+int rtmhip_ae_get_params(const rtmhip_ae *ae, float *host, int which);
+int rtmhip_ae_set_params(rtmhip_ae *ae, const float *host);
+/* Gradient of 0.5 * sum (decode(encode(x)) - x)^2 over n FP16 bricks (device)
+ * into the gradient buffer; *loss (host) receives that half sum. */
+int rtmhip_ae_gradient(rtmhip_ae *ae, const void *bricks, int n, float *loss);
+/* iters Adam steps (lr; beta 0.9, 0.999; eps 1e-8) on batches of `batch`
+ * bricks drawn without replacement from nbricks FP16 bricks (device);
+ * losses (host, optional): the mean squared error of every batch. */
+int rtmhip_ae_train(rtmhip_ae *ae, const void *bricks, int nbricks, int iters, int batch,
+                    float lr, unsigned long long seed, float *losses);
+/* Cut a region of an FP32 field into normalised FP16 bricks + scales. */
+int rtmhip_ae_bricks(const float *field, int nxe, int nye, int nze, int x0, int y0, int z0,
+                     int x1, int y1, int z1, void *bricks, float *scales);
+/* n FP16 bricks -> n x 2048 FP16 codes. */
+int rtmhip_ae_encode(rtmhip_ae *ae, const void *bricks, int n, void *codes);
+/* Codes + scales of a region's n bricks -> the region's cells of field. */
+int rtmhip_ae_decode(rtmhip_ae *ae, const void *codes, const float *scales, int n, float *field,
+                     int nxe, int nye, int nze, int x0, int y0, int z0, int x1, int y1, int z1);
+/* One matrix-core GEMM of the autoencoder, exposed for tests:
+ * C(m,n) (+)= sum_k A(m,k) B(k,n) (+ bias[n], tanh if act), with
+ * A(m,k) = A[m*sam + k*sak], B(k,n) = B[k*sbk + n*sbn], FP16 (x_half != 0)
+ * or FP32 in memory; accumulate != 0 adds split-K partials atomically. */
+int rtmhip_ae_gemm_test(const void *A, long long sam, long long sak, int a_half, const void *B,
+                        long long sbk, long long sbn, int b_half, float *C, long long ldc,
+                        const float *bias, long long M, int N, long long K, int act,
+                        int accumulate);
+
 /* ================= tuning and measurement =================
  * Stencil kernel classes: 0 = plain step (rtmhip_step, backward source
  * reconstruction, RBC), 1 = receiver step fused with receiver injection and

@@ -7,6 +7,7 @@
 #include "rtm_internal.hpp"
 
 #include <algorithm>
+#include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -284,14 +285,14 @@ int rtmhip_init(int device)
 int rtmhip_finalize(void)
 {
     if (!g_state.init) return RTMHIP_OK;
-    hipStreamSynchronize(g_state.stream);
+    (void)hipStreamSynchronize(g_state.stream);
     for (auto &t : g_state.tm) {
-        for (auto e : t.start) hipEventDestroy(e);
-        for (auto e : t.stop) hipEventDestroy(e);
+        for (auto e : t.start) (void)hipEventDestroy(e);
+        for (auto e : t.stop) (void)hipEventDestroy(e);
         t = Timing();
     }
-    hipFree(g_state.flag);
-    hipStreamDestroy(g_state.stream);
+    (void)hipFree(g_state.flag);
+    (void)hipStreamDestroy(g_state.stream);
     g_state = State();
     return RTMHIP_OK;
 }
@@ -1202,6 +1203,415 @@ int rtmhip_batch_backward_steps(rtmhip_batch *b, int it_hi, int nsteps)
         using T = decltype(t);
         return batch_backward<T>(b, it_hi, nsteps);
     });
+}
+
+}  // extern "C"
+
+// ===========================================================================
+// Snapshot autoencoder: a 4-layer patch-convolution network that compresses
+// 32^3 FP16 bricks of a source-wavefield snapshot 16x (2048 FP16 codes per
+// brick plus one FP32 scale), trained on the device by its own forward and
+// backward kernels (matrix-core GEMMs) with Adam.
+// ===========================================================================
+
+namespace {
+
+struct AeLayerDef {
+    int K, N, act;      // weights [K][N] row-major, then the bias [N]
+    long long rows;     // GEMM rows per brick
+};
+constexpr AeLayerDef kAe[4] = {{8, 8, 1, 4096}, {512, 32, 0, 64}, {32, 512, 1, 64},
+                               {8, 8, 0, 4096}};
+constexpr long long kBrickCells = 32768, kCodes = 2048;
+
+long long ae_param_offset(int layer)
+{
+    long long o = 0;
+    for (int i = 0; i < layer; ++i) o += (long long)kAe[i].K * kAe[i].N + kAe[i].N;
+    return o;
+}
+constexpr long long kAeParams = 8 * 8 + 8 + 512 * 32 + 32 + 32 * 512 + 512 + 8 * 8 + 8;
+
+}  // namespace
+
+struct rtmhip_ae {
+    int maxb = 0;
+    float *w = nullptr, *g = nullptr, *m = nullptr, *v = nullptr;  // kAeParams each
+    void *x0 = nullptr;     // FP16 input bricks [maxb][32768]
+    float *h1 = nullptr, *z = nullptr, *h3 = nullptr, *y = nullptr;
+    float *d1 = nullptr, *dz = nullptr, *d3 = nullptr, *dy = nullptr;
+    float *loss = nullptr;  // device accumulator
+    long long steps = 0;    // Adam steps taken
+};
+
+namespace {
+
+float *ae_w(const rtmhip_ae *a, int l) { return a->w + ae_param_offset(l); }
+float *ae_b(const rtmhip_ae *a, int l) { return ae_w(a, l) + (long long)kAe[l].K * kAe[l].N; }
+float *ae_gw(const rtmhip_ae *a, int l) { return a->g + ae_param_offset(l); }
+float *ae_gb(const rtmhip_ae *a, int l) { return ae_gw(a, l) + (long long)kAe[l].K * kAe[l].N; }
+
+// Row-major GEMM of one layer: out[rows][N] = act(in[rows][K] W + b).
+hipError_t ae_layer(const rtmhip_ae *a, int l, const void *in, int in_half, float *out, int n)
+{
+    AeGemm p{};
+    p.A = in;
+    p.sam = kAe[l].K;
+    p.sak = 1;
+    p.a_half = in_half;
+    p.B = ae_w(a, l);
+    p.sbk = kAe[l].N;
+    p.sbn = 1;
+    p.C = out;
+    p.ldc = kAe[l].N;
+    p.bias = ae_b(a, l);
+    p.M = kAe[l].rows * n;
+    p.N = kAe[l].N;
+    p.K = kAe[l].K;
+    p.act = kAe[l].act;
+    return launch_ae_gemm(p, g_state.stream);
+}
+
+hipError_t ae_forward(const rtmhip_ae *a, const void *x0, int n)
+{
+    hipError_t e;
+    if ((e = ae_layer(a, 0, x0, 1, a->h1, n)) != hipSuccess) return e;
+    if ((e = ae_layer(a, 1, a->h1, 0, a->z, n)) != hipSuccess) return e;
+    if ((e = ae_layer(a, 2, a->z, 0, a->h3, n)) != hipSuccess) return e;
+    return ae_layer(a, 3, a->h3, 0, a->y, n);
+}
+
+// Weight gradient of layer l: gW[K][N] += in^T (rows x K) * d (rows x N);
+// input gradient din[rows][K] = d * W^T (skipped when din is null).
+hipError_t ae_layer_back(const rtmhip_ae *a, int l, const void *in, int in_half, const float *d,
+                         float *din, int n)
+{
+    hipStream_t s = g_state.stream;
+    const long long rows = kAe[l].rows * n;
+    const int K = kAe[l].K, N = kAe[l].N;
+    AeGemm p{};
+    p.A = in;
+    p.sam = 1;
+    p.sak = K;
+    p.a_half = in_half;
+    p.B = d;
+    p.sbk = N;
+    p.sbn = 1;
+    p.C = ae_gw(a, l);
+    p.ldc = N;
+    p.M = K;
+    p.N = N;
+    p.K = rows;
+    p.accumulate = 1;
+    p.kchunk = 4096;
+    hipError_t e = launch_ae_gemm(p, s);
+    if (e != hipSuccess) return e;
+    if ((e = launch_colsum(d, rows, N, ae_gb(a, l), s)) != hipSuccess) return e;
+    if (!din) return hipSuccess;
+    AeGemm q{};
+    q.A = d;
+    q.sam = N;
+    q.sak = 1;
+    q.B = ae_w(a, l);  // B(k, n) = W[n][k]
+    q.sbk = 1;
+    q.sbn = N;
+    q.C = din;
+    q.ldc = K;
+    q.M = rows;
+    q.N = K;
+    q.K = N;
+    return launch_ae_gemm(q, s);
+}
+
+// Gradient of 0.5 * sum (Y - X)^2 over n bricks into a->g (zeroed first);
+// the sum of squares is added to *loss (device).
+hipError_t ae_backward(const rtmhip_ae *a, const void *x0, int n, float *loss)
+{
+    hipStream_t s = g_state.stream;
+    const long long cells = kBrickCells * n;
+    hipError_t e;
+    if ((e = launch_fill(a->g, (size_t)kAeParams, 0.f, s)) != hipSuccess) return e;
+    if ((e = launch_ae_residual(a->y, x0, a->dy, cells, loss, s)) != hipSuccess) return e;
+    if ((e = ae_layer_back(a, 3, a->h3, 0, a->dy, a->d3, n)) != hipSuccess) return e;
+    if ((e = launch_tanh_back(a->d3, a->h3, cells, s)) != hipSuccess) return e;
+    if ((e = ae_layer_back(a, 2, a->z, 0, a->d3, a->dz, n)) != hipSuccess) return e;
+    if ((e = ae_layer_back(a, 1, a->h1, 0, a->dz, a->d1, n)) != hipSuccess) return e;
+    if ((e = launch_tanh_back(a->d1, a->h1, cells, s)) != hipSuccess) return e;
+    return ae_layer_back(a, 0, x0, 1, a->d1, nullptr, n);
+}
+
+uint64_t splitmix64(uint64_t &x)
+{
+    uint64_t z = (x += 0x9E3779B97F4A7C15ULL);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+
+int ae_check_region(int nxe, int nye, int nze, int x0, int y0, int z0, int x1, int y1, int z1,
+                    AeRegion *r, Grid *g)
+{
+    int rc = check_grid(nxe, nye, nze);
+    if (rc) return rc;
+    if (x0 < 0 || y0 < 0 || z0 < 0 || x1 > nxe || y1 > nye || z1 > nze || x0 >= x1 || y0 >= y1 ||
+        z0 >= z1)
+        return fail(RTMHIP_ERR_INVALID, "region outside the grid or empty");
+    r->x0 = x0;
+    r->y0 = y0;
+    r->z0 = z0;
+    r->x1 = x1;
+    r->y1 = y1;
+    r->z1 = z1;
+    r->nbx = (x1 - x0 + 31) / 32;
+    r->nby = (y1 - y0 + 31) / 32;
+    r->nbz = (z1 - z0 + 31) / 32;
+    *g = dense_grid(nxe, nye, nze);
+    return RTMHIP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int rtmhip_ae_param_count(void) { return (int)kAeParams; }
+
+int rtmhip_ae_create(rtmhip_ae **out, int max_bricks, unsigned long long seed)
+{
+    NEED_INIT();
+    if (!out) return fail(RTMHIP_ERR_INVALID, "NULL pointer");
+    *out = nullptr;
+    if (max_bricks <= 0 || max_bricks > 4096)
+        return fail(RTMHIP_ERR_INVALID, "max_bricks %d not in 1..4096", max_bricks);
+    auto a = std::make_unique<rtmhip_ae>();
+    a->maxb = max_bricks;
+    const long long cells = kBrickCells * max_bricks, codes = kCodes * max_bricks;
+    struct Alloc {
+        void **p;
+        size_t bytes;
+    } allocs[] = {
+        {(void **)&a->w, kAeParams * 4}, {(void **)&a->g, kAeParams * 4},
+        {(void **)&a->m, kAeParams * 4}, {(void **)&a->v, kAeParams * 4},
+        {&a->x0, (size_t)cells * 2},     {(void **)&a->h1, (size_t)cells * 4},
+        {(void **)&a->z, (size_t)codes * 4}, {(void **)&a->h3, (size_t)cells * 4},
+        {(void **)&a->y, (size_t)cells * 4}, {(void **)&a->d1, (size_t)cells * 4},
+        {(void **)&a->dz, (size_t)codes * 4}, {(void **)&a->d3, (size_t)cells * 4},
+        {(void **)&a->dy, (size_t)cells * 4}, {(void **)&a->loss, 4},
+    };
+    for (auto &al : allocs) {
+        hipError_t e = hipMalloc(al.p, al.bytes);
+        if (e != hipSuccess) {
+            rtmhip_ae_destroy(a.release());
+            return hip_fail(e, "hipMalloc");
+        }
+    }
+    // Glorot-uniform weights from the seed, zero biases, zero Adam moments
+    std::vector<float> w((size_t)kAeParams, 0.f);
+    uint64_t st = seed;
+    for (int l = 0; l < 4; ++l) {
+        const float lim = std::sqrt(6.0f / (float)(kAe[l].K + kAe[l].N));
+        float *p = w.data() + ae_param_offset(l);
+        for (long long i = 0; i < (long long)kAe[l].K * kAe[l].N; ++i) {
+            const double u = (double)(splitmix64(st) >> 11) * (1.0 / 9007199254740992.0);
+            p[i] = (float)((2.0 * u - 1.0) * lim);
+        }
+    }
+    hipError_t e = hipMemcpy(a->w, w.data(), kAeParams * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = launch_fill(a->m, (size_t)kAeParams, 0.f, g_state.stream);
+    if (e == hipSuccess) e = launch_fill(a->v, (size_t)kAeParams, 0.f, g_state.stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(g_state.stream);
+    if (e != hipSuccess) {
+        rtmhip_ae_destroy(a.release());
+        return hip_fail(e, "autoencoder set-up");
+    }
+    *out = a.release();
+    return RTMHIP_OK;
+}
+
+int rtmhip_ae_destroy(rtmhip_ae *a)
+{
+    if (!a) return RTMHIP_OK;
+    if (g_state.init) (void)hipStreamSynchronize(g_state.stream);
+    void *ps[] = {a->w, a->g, a->m, a->v, a->x0, a->h1, a->z, a->h3,
+                  a->y, a->d1, a->dz, a->d3, a->dy, a->loss};
+    for (void *p : ps)
+        if (p) (void)hipFree(p);
+    delete a;
+    return RTMHIP_OK;
+}
+
+int rtmhip_ae_get_params(const rtmhip_ae *a, float *host, int which)
+{
+    NEED_INIT();
+    if (!a || !host || which < 0 || which > 1) return fail(RTMHIP_ERR_INVALID, "bad arguments");
+    HIPCHK(hipMemcpyAsync(host, which ? a->g : a->w, kAeParams * 4, hipMemcpyDeviceToHost,
+                          g_state.stream),
+           "hipMemcpyAsync");
+    HIPCHK(hipStreamSynchronize(g_state.stream), "hipStreamSynchronize");
+    return RTMHIP_OK;
+}
+
+int rtmhip_ae_set_params(rtmhip_ae *a, const float *host)
+{
+    NEED_INIT();
+    if (!a || !host) return fail(RTMHIP_ERR_INVALID, "bad arguments");
+    HIPCHK(hipMemcpyAsync(a->w, host, kAeParams * 4, hipMemcpyHostToDevice, g_state.stream),
+           "hipMemcpyAsync");
+    HIPCHK(hipStreamSynchronize(g_state.stream), "hipStreamSynchronize");
+    return RTMHIP_OK;
+}
+
+int rtmhip_ae_gradient(rtmhip_ae *a, const void *bricks, int n, float *loss)
+{
+    NEED_INIT();
+    if (!a || !bricks || !loss) return fail(RTMHIP_ERR_INVALID, "NULL pointer");
+    if (n <= 0 || n > a->maxb) return fail(RTMHIP_ERR_INVALID, "n %d not in 1..%d", n, a->maxb);
+    HIPCHK(launch_fill(a->loss, 1, 0.f, g_state.stream), "fill");
+    HIPCHK(ae_forward(a, bricks, n), "autoencoder forward");
+    HIPCHK(ae_backward(a, bricks, n, a->loss), "autoencoder backward");
+    HIPCHK(hipMemcpyAsync(loss, a->loss, 4, hipMemcpyDeviceToHost, g_state.stream), "copy");
+    HIPCHK(hipStreamSynchronize(g_state.stream), "hipStreamSynchronize");
+    *loss *= 0.5f;
+    return RTMHIP_OK;
+}
+
+int rtmhip_ae_train(rtmhip_ae *a, const void *bricks, int nbricks, int iters, int batch, float lr,
+                    unsigned long long seed, float *losses)
+{
+    NEED_INIT();
+    if (!a || !bricks) return fail(RTMHIP_ERR_INVALID, "NULL pointer");
+    if (nbricks <= 0 || iters <= 0 || batch <= 0 || batch > a->maxb || batch > nbricks)
+        return fail(RTMHIP_ERR_INVALID, "bad sizes");
+    if ((long long)iters * batch > (1LL << 28)) return fail(RTMHIP_ERR_INVALID, "too many iterations");
+    hipStream_t s = g_state.stream;
+    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    // every iteration's batch, drawn without replacement from reshuffled
+    // passes over the bricks (Fisher-Yates on a seeded splitmix64 stream)
+    uint64_t st = seed;
+    std::vector<int> perm(nbricks), picks((size_t)iters * batch);
+    for (int i = 0; i < nbricks; ++i) perm[i] = i;
+    size_t pos = (size_t)nbricks;
+    for (auto &p : picks) {
+        if (pos >= (size_t)nbricks) {
+            for (int i = nbricks - 1; i > 0; --i) {
+                const int k = (int)(splitmix64(st) % (uint64_t)(i + 1));
+                std::swap(perm[i], perm[k]);
+            }
+            pos = 0;
+        }
+        p = perm[pos++];
+    }
+    int *didx = nullptr;
+    float *dl = nullptr;
+    HIPCHK(hipMalloc((void **)&didx, sizeof(int) * picks.size()), "hipMalloc");
+    hipError_t e = hipMalloc((void **)&dl, sizeof(float) * iters);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(didx, picks.data(), sizeof(int) * picks.size(), hipMemcpyHostToDevice,
+                           s);
+    if (e == hipSuccess) e = launch_fill(dl, (size_t)iters, 0.f, s);
+    for (int it = 0; it < iters && e == hipSuccess; ++it) {
+        e = launch_gather_bricks(bricks, didx + (size_t)it * batch, a->x0, batch, s);
+        if (e == hipSuccess) e = ae_forward(a, a->x0, batch);
+        if (e == hipSuccess) e = ae_backward(a, a->x0, batch, dl + it);
+        a->steps++;
+        const double c1 = 1.0 - std::pow((double)b1, (double)a->steps);
+        const double c2 = 1.0 - std::pow((double)b2, (double)a->steps);
+        const float lr_t = (float)(lr * std::sqrt(c2) / c1);
+        if (e == hipSuccess)
+            e = launch_adam(a->w, a->g, a->m, a->v, (int)kAeParams, lr_t, b1, b2, eps, s);
+    }
+    std::vector<float> lhost(iters);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(lhost.data(), dl, sizeof(float) * iters, hipMemcpyDeviceToHost, s);
+    hipError_t es = hipStreamSynchronize(s);
+    if (e == hipSuccess) e = es;
+    (void)hipFree(didx);
+    if (dl) (void)hipFree(dl);
+    if (e != hipSuccess) return hip_fail(e, "autoencoder training");
+    if (losses)
+        for (int i = 0; i < iters; ++i)
+            losses[i] = lhost[i] / (float)(kBrickCells * batch);  // mean squared error
+    return RTMHIP_OK;
+}
+
+int rtmhip_ae_bricks(const float *field, int nxe, int nye, int nze, int x0, int y0, int z0,
+                     int x1, int y1, int z1, void *bricks, float *scales)
+{
+    NEED_INIT();
+    if (!field || !bricks || !scales) return fail(RTMHIP_ERR_INVALID, "NULL pointer");
+    AeRegion r;
+    Grid g;
+    int rc = ae_check_region(nxe, nye, nze, x0, y0, z0, x1, y1, z1, &r, &g);
+    if (rc) return rc;
+    HIPCHK(launch_bricks(field, g, r, bricks, scales, g_state.stream), "bricks");
+    return RTMHIP_OK;
+}
+
+int rtmhip_ae_encode(rtmhip_ae *a, const void *bricks, int n, void *codes)
+{
+    NEED_INIT();
+    if (!a || !bricks || !codes) return fail(RTMHIP_ERR_INVALID, "NULL pointer");
+    for (int b0 = 0; b0 < n; b0 += a->maxb) {
+        const int nb = std::min(a->maxb, n - b0);
+        const char *in = static_cast<const char *>(bricks) + (size_t)b0 * kBrickCells * 2;
+        HIPCHK(ae_layer(a, 0, in, 1, a->h1, nb), "encode");
+        HIPCHK(ae_layer(a, 1, a->h1, 0, a->z, nb), "encode");
+        HIPCHK(launch_f32_to_f16(a->z, static_cast<char *>(codes) + (size_t)b0 * kCodes * 2,
+                                 kCodes * nb, g_state.stream),
+               "encode");
+    }
+    return RTMHIP_OK;
+}
+
+int rtmhip_ae_decode(rtmhip_ae *a, const void *codes, const float *scales, int n, float *field,
+                     int nxe, int nye, int nze, int x0, int y0, int z0, int x1, int y1, int z1)
+{
+    NEED_INIT();
+    if (!a || !codes || !scales || !field) return fail(RTMHIP_ERR_INVALID, "NULL pointer");
+    AeRegion r;
+    Grid g;
+    int rc = ae_check_region(nxe, nye, nze, x0, y0, z0, x1, y1, z1, &r, &g);
+    if (rc) return rc;
+    if ((long long)n != (long long)r.nbx * r.nby * r.nbz)
+        return fail(RTMHIP_ERR_INVALID, "%d codes for a region of %d bricks", n,
+                    r.nbx * r.nby * r.nbz);
+    for (int b0 = 0; b0 < n; b0 += a->maxb) {
+        const int nb = std::min(a->maxb, n - b0);
+        HIPCHK(launch_f16_to_f32(static_cast<const char *>(codes) + (size_t)b0 * kCodes * 2, a->z,
+                                 kCodes * nb, g_state.stream),
+               "decode");
+        HIPCHK(ae_layer(a, 2, a->z, 0, a->h3, nb), "decode");
+        HIPCHK(ae_layer(a, 3, a->h3, 0, a->y, nb), "decode");
+        HIPCHK(launch_unbricks(a->y, scales, g, r, field, b0, nb, g_state.stream), "decode");
+    }
+    return RTMHIP_OK;
+}
+
+int rtmhip_ae_gemm_test(const void *A, long long sam, long long sak, int a_half, const void *B,
+                        long long sbk, long long sbn, int b_half, float *C, long long ldc,
+                        const float *bias, long long M, int N, long long K, int act,
+                        int accumulate)
+{
+    NEED_INIT();
+    AeGemm p{};
+    p.A = A;
+    p.sam = sam;
+    p.sak = sak;
+    p.a_half = a_half;
+    p.B = B;
+    p.sbk = sbk;
+    p.sbn = sbn;
+    p.b_half = b_half;
+    p.C = C;
+    p.ldc = ldc;
+    p.bias = bias;
+    p.M = M;
+    p.N = N;
+    p.K = K;
+    p.act = act;
+    p.accumulate = accumulate;
+    p.kchunk = 64;
+    HIPCHK(launch_ae_gemm(p, g_state.stream), "gemm");
+    return RTMHIP_OK;
 }
 
 }  // extern "C"

@@ -182,4 +182,46 @@ hipError_t launch_fill(float *p, size_t n, float v, hipStream_t s);
 hipError_t launch_fill_u16(uint16_t *p, size_t n, uint16_t bits, hipStream_t s);
 hipError_t launch_stack(float *dst, const float *src, size_t n, hipStream_t s);
 
+// ---- snapshot autoencoder (rtm_ae.hip) ----
+// One GEMM of the autoencoder on the matrix cores: C(m, n) (+)= sum_k
+// A(m, k) B(k, n) with A(m, k) = A[m*sam + k*sak], B(k, n) = B[k*sbk +
+// n*sbn] (FP32 or FP16 in memory, FP16 operands, FP32 accumulation).
+struct AeGemm {
+    const void *A;
+    long long sam, sak;
+    int a_half;
+    const void *B;
+    long long sbk, sbn;
+    int b_half;
+    float *C;
+    long long ldc;
+    const float *bias;  // per column, or null
+    long long M;
+    int N;
+    long long K;
+    int act;         // 0 none, 1 tanh
+    int accumulate;  // atomic split-K accumulation into C (no bias / act)
+    int kchunk;      // K per split (accumulate only; 0 = 4096)
+};
+// Bricks tile [x0, x0 + 32 nbx) x [y0, ..) x [z0, ..); cells at or beyond
+// (x1, y1, z1) are outside the compressed region (zero / not written).
+struct AeRegion {
+    int x0, y0, z0, x1, y1, z1;
+    int nbx, nby, nbz;
+};
+hipError_t launch_ae_gemm(const AeGemm &p, hipStream_t s);
+hipError_t launch_ae_residual(const float *Y, const void *X, float *dY, long long n, float *loss,
+                              hipStream_t s);
+hipError_t launch_tanh_back(float *dH, const float *H, long long n, hipStream_t s);
+hipError_t launch_colsum(const float *D, long long rows, int cols, float *db, hipStream_t s);
+hipError_t launch_adam(float *w, const float *g, float *m, float *v, int n, float lr_t, float b1,
+                       float b2, float eps, hipStream_t s);
+hipError_t launch_gather_bricks(const void *src, const int *idx, void *dst, int n, hipStream_t s);
+hipError_t launch_f32_to_f16(const float *a, void *b, long long n, hipStream_t s);
+hipError_t launch_f16_to_f32(const void *a, float *b, long long n, hipStream_t s);
+hipError_t launch_bricks(const float *F, const Grid &g, const AeRegion &r, void *out,
+                         float *scales, hipStream_t s);
+hipError_t launch_unbricks(const float *Y, const float *scales, const Grid &g, const AeRegion &r,
+                           float *F, int b0, int nb, hipStream_t s);
+
 }  // namespace rtmhip

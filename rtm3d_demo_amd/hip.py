@@ -25,6 +25,9 @@ EXPORTED = [
     "rtmhip_batch_create", "rtmhip_batch_destroy", "rtmhip_batch_forward_steps",
     "rtmhip_batch_backward_steps", "rtmhip_batch_fields", "rtmhip_batch_set_stencil",
     "rtmhip_batch_set_graph",
+    "rtmhip_ae_param_count", "rtmhip_ae_create", "rtmhip_ae_destroy", "rtmhip_ae_get_params",
+    "rtmhip_ae_set_params", "rtmhip_ae_gradient", "rtmhip_ae_train", "rtmhip_ae_bricks",
+    "rtmhip_ae_encode", "rtmhip_ae_decode", "rtmhip_ae_gemm_test",
 ]
 
 
@@ -101,6 +104,19 @@ def load():
         "rtmhip_batch_fields": [vp, i, pp, pp, pp, pp],
         "rtmhip_batch_set_stencil": [vp, i, i],
         "rtmhip_batch_set_graph": [vp, i],
+        "rtmhip_ae_param_count": [],
+        "rtmhip_ae_create": [pp, i, C.c_ulonglong],
+        "rtmhip_ae_destroy": [vp],
+        "rtmhip_ae_get_params": [vp, fp, i],
+        "rtmhip_ae_set_params": [vp, fp],
+        "rtmhip_ae_gradient": [vp, vp, i, C.POINTER(C.c_float)],
+        "rtmhip_ae_train": [vp, vp, i, i, i, C.c_float, C.c_ulonglong, fp],
+        "rtmhip_ae_bricks": [fp, i, i, i, i, i, i, i, i, i, vp, fp],
+        "rtmhip_ae_encode": [vp, vp, i, vp],
+        "rtmhip_ae_decode": [vp, vp, fp, i, fp, i, i, i, i, i, i, i, i, i],
+        "rtmhip_ae_gemm_test": [vp, C.c_longlong, C.c_longlong, i, vp, C.c_longlong,
+                                C.c_longlong, i, fp, C.c_longlong, fp, C.c_longlong, i,
+                                C.c_longlong, i, i],
     }
     for name, args in sig.items():
         f = getattr(L, name)
