@@ -240,8 +240,8 @@ int rtmhip_batch_fields(const rtmhip_batch *batch, int i, float **P, float **PP,
 /* Stencil launches of one kernel class (0 backward source step, 1 receiver
  * step with imaging, 2 forward tapered step): mode 0 = one launch per shot
  * with the class tiling of rtmhip_set_tiling; 6 or 8 = one launch for every
- * shot (barrier-free 64-lane waves with 6 or 8 rows each).  Defaults: class
- * 0 mode 8, classes 1 and 2 mode 0. */
+ * shot (barrier-free 64-lane waves with 6 or 8 rows each).  Default: mode 0
+ * for every class. */
 int rtmhip_batch_set_stencil(rtmhip_batch *batch, int kernel_class, int mode);
 /* Steps per captured graph: 0 = launch every step (no graph), else an even
  * count (default 8). */

@@ -798,7 +798,7 @@ struct rtmhip_batch {
     int fwd_blocks = 0, bwd_blocks = 0;
     int *ctr = nullptr;             // device time index of graph replays
     int cs = 0, cr = 0;             // P = f[c], PP = f[c ^ 1]
-    int mode[3] = {8, 0, 0};        // per stencil class: 0 per-shot launches, 6/8 batched
+    int mode[3] = {0, 0, 0};        // per stencil class: 0 per-shot launches, 6/8 batched
     int graph_steps = 8;            // steps per captured graph (0: no graphs)
     std::map<int, hipGraphExec_t> graphs;
 };

@@ -139,7 +139,7 @@ BWD = [(NT - 1, 12), (NT - 13, 13)]  # the two seeding steps, then batched steps
     ((8, 8, 8), 4),   # one launch for all shots, 8 rows per wave
     ((6, 6, 6), 2),
     ((8, 0, 6), 0),   # mixed, no graph
-    ((8, 0, 0), 8),   # the defaults
+    ((0, 0, 0), 8),   # the defaults
 ], ids=["pershot-g4", "ry8-g4", "ry6-g2", "mixed-eager", "default-g8"])
 @pytest.mark.parametrize("fmt", ["f32", "f16"])
 def test_batch_matches_single_shots(gpu, fmt, modes, graph_steps):
