@@ -18,6 +18,8 @@
 // backward, split-K weight gradients).
 #include "rtm_internal.hpp"
 
+#include <algorithm>
+
 namespace rtmhip {
 namespace {
 
@@ -139,17 +141,35 @@ __global__ void __launch_bounds__(256)
         dH[i] = dH[i] * (1.0f - H[i] * H[i]);
 }
 
-// db[c] += sum over rows of D[row][c] (rows x cols, row-major).
+// db[c] += sum over rows of D[row][c] (rows x cols, row-major; cols divides
+// 256 or is a multiple of 256).  A block walks its row range in row-major
+// order, so every load is coalesced; thread t keeps the partial sums of the
+// columns t, t + 256, ... (or of column t % cols), then one block reduction.
 __global__ void __launch_bounds__(256)
     k_colsum(const float *__restrict__ D, long long rows, int cols, float *__restrict__ db,
              long long rows_per_block)
 {
-    const long long r0 = (long long)blockIdx.y * rows_per_block;
+    __shared__ float red[256];
+    const long long r0 = (long long)blockIdx.x * rows_per_block;
     const long long r1 = min(rows, r0 + rows_per_block);
-    for (int c = blockIdx.x * 256 + threadIdx.x; c < cols; c += gridDim.x * 256) {
-        float s = 0.f;
-        for (long long r = r0; r < r1; ++r) s += D[r * cols + c];
-        atomicAdd(&db[c], s);
+    const long long e0 = r0 * cols, e1 = r1 * cols;
+    const int t = threadIdx.x;
+    if (cols <= 256) {
+        float acc = 0.f;
+        for (long long e = e0 + t; e < e1; e += 256) acc += D[e];
+        red[t] = acc;
+        __syncthreads();
+        if (t < cols) {
+            float s = 0.f;
+            for (int k = t; k < 256; k += cols) s += red[k];
+            atomicAdd(&db[t], s);
+        }
+    } else {
+        for (int c = t; c < cols; c += 256) {
+            float acc = 0.f;
+            for (long long e = e0 + c; e < e1; e += cols) acc += D[e];
+            atomicAdd(&db[c], acc);
+        }
     }
 }
 
@@ -287,7 +307,12 @@ hipError_t launch_ae_gemm(const AeGemm &p, hipStream_t s)
     AeGemm q = p;
     long long splits = 1;
     if (q.accumulate) {
-        if (q.kchunk <= 0) q.kchunk = 4096;
+        if (q.kchunk <= 0) {
+            // split K so that the launch has about 1024 blocks, 256..65536 per split
+            const long long tiles = ((q.M + 63) / 64) * ((q.N + 63) / 64);
+            const long long want = std::max(1LL, 1024 / std::max(1LL, tiles));
+            q.kchunk = (int)std::min(65536LL, std::max(256LL, (q.K + want - 1) / want));
+        }
         q.kchunk = (q.kchunk + 15) / 16 * 16;
         splits = (q.K + q.kchunk - 1) / q.kchunk;
     } else {
@@ -303,7 +328,8 @@ hipError_t launch_ae_gemm(const AeGemm &p, hipStream_t s)
 hipError_t launch_ae_residual(const float *Y, const void *X, float *dY, long long n, float *loss,
                               hipStream_t s)
 {
-    hipLaunchKernelGGL(k_ae_residual, dim3(grid_for(n)), dim3(256), 0, s, Y,
+    // at most 1024 blocks: one atomic per block on the single loss word
+    hipLaunchKernelGGL(k_ae_residual, dim3(std::min(1024u, grid_for(n))), dim3(256), 0, s, Y,
                        static_cast<const _Float16 *>(X), dY, n, loss);
     return hipGetLastError();
 }
@@ -316,11 +342,13 @@ hipError_t launch_tanh_back(float *dH, const float *H, long long n, hipStream_t 
 
 hipError_t launch_colsum(const float *D, long long rows, int cols, float *db, hipStream_t s)
 {
-    const long long rpb = 4096;
-    const long long gy = (rows + rpb - 1) / rpb;
-    if (gy > 65535) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_colsum, dim3((unsigned)((cols + 255) / 256), (unsigned)gy), dim3(256), 0,
-                       s, D, rows, cols, db, rpb);
+    if (cols <= 0 || rows <= 0) return hipSuccess;
+    if (!(cols <= 256 ? 256 % cols == 0 : cols % 256 == 0)) return hipErrorInvalidValue;
+    // about 16K elements per block
+    const long long rpb = std::max(1LL, 16384LL / cols);
+    const long long nb = (rows + rpb - 1) / rpb;
+    if (nb > 0x7fffffffLL) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_colsum, dim3((unsigned)nb), dim3(256), 0, s, D, rows, cols, db, rpb);
     return hipGetLastError();
 }
 

@@ -1303,7 +1303,7 @@ hipError_t ae_layer_back(const rtmhip_ae *a, int l, const void *in, int in_half,
     p.N = N;
     p.K = rows;
     p.accumulate = 1;
-    p.kchunk = 4096;
+    p.kchunk = 0;  // sized by the launcher
     hipError_t e = launch_ae_gemm(p, s);
     if (e != hipSuccess) return e;
     if ((e = launch_colsum(d, rows, N, ae_gb(a, l), s)) != hipSuccess) return e;

@@ -201,7 +201,7 @@ struct AeGemm {
     long long K;
     int act;         // 0 none, 1 tanh
     int accumulate;  // atomic split-K accumulation into C (no bias / act)
-    int kchunk;      // K per split (accumulate only; 0 = 4096)
+    int kchunk;      // K per split (accumulate only; 0 = sized for ~1024 blocks)
 };
 // Bricks tile [x0, x0 + 32 nbx) x [y0, ..) x [z0, ..); cells at or beyond
 // (x1, y1, z1) are outside the compressed region (zero / not written).
