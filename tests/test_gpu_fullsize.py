@@ -1,8 +1,9 @@
 """Parity at the benchmark's full grid sizes: one complete backward step of
 HBC migration (source reconstruction with upper-border restore, receiver
 tapers, fused receiver step + injection + imaging) on the 512x512x256+2*40
-grid, and one stencil step on the 800x800x400+2*40 grid, bit-exact against
-the oracle on the same random fields."""
+grid, one stencil step on the 800x800x400+2*40 grid, and two fused HBC
+forward steps with FP16 and with BF16 wavefields on that grid, bit-exact
+against the oracle on the same random fields."""
 import numpy as np
 import pytest
 
@@ -78,3 +79,41 @@ def test_stencil_step_800_grid(gpu, oracle):
     assert _bits_equal(got, PP)
     for x in (dP, dPP, dV):
         x.free()
+
+
+@pytest.mark.parametrize("fmt", ["f16", "bf16"])
+def test_16bit_forward_steps_800_grid(gpu, oracle, fmt):
+    """Two fused HBC forward steps with 16-bit wavefields on the
+    800x800x400 + 2*40 grid (the mixed-precision configuration), bit-exact
+    against the 16-bit oracle's operator sequence."""
+    shape, b, order, nt = (880, 880, 480), 40, 8, 2
+    print("16-bit forward steps", fmt, shape, flush=True)
+    P0 = oracle.to_format(_rand(shape, 21), fmt)
+    PP0 = oracle.to_format(_rand(shape, 22), fmt)
+    v2 = _rand(shape, 23, 0.0, 0.2)
+    g = host.taper_table(b, 0.9)
+    coefs = host.stencil_coefs(order, 12.5, 12.5, 12.5)
+    src = np.array([0.75, -0.5], np.float32)
+    F = gpu.FieldPair(shape, np.uint16)
+    F.a.upload(P0)
+    F.b.upload(PP0)
+    V, dg, ds = (gpu.DeviceArray.from_host(a) for a in (v2, g, src))
+    dupb = gpu.DeviceArray((nt, shape[0], shape[1], 4), np.uint16).zero()
+    shot = gpu.make_shot(*shape, b, order, nt, nt, (440, 440, b + 2),
+                         gpu.RcvGeom(0, 0, 0, 1, 1, 1, 1), fmt)
+    gpu.forward_steps(shot, True, V, coefs, dg, ds, F, dupb, 0, nt)
+    got_P, got_PP, got_upb = F.P.download(), F.PP.download(), dupb.download()
+    for d in (F, V, dg, ds, dupb):
+        d.free()
+    P, PP = P0, PP0
+    upb = np.zeros((nt, shape[0], shape[1], 4), np.uint16)
+    for it in range(nt):
+        P, PP = PP, P
+        oracle.taper(P, g, b, True, fmt)
+        oracle.taper(PP, g, b, True, fmt)
+        oracle.step(P, PP, v2, coefs, order, fmt)
+        oracle.apply_source(PP, 440, 440, b + 2, src[it], fmt)
+        oracle.upb(PP, upb, b, order, it, False, fmt)
+    assert np.array_equal(got_PP, PP)
+    assert np.array_equal(got_P, P)
+    assert np.array_equal(got_upb, upb)
