@@ -145,6 +145,20 @@ int rtmhip_upper_border(float *PP, float *upb, int nxe, int nye, int nze, int bl
 int rtmhip_image_condition(float *img, const float *PS, const float *PR, int nxe, int nye,
                            int nze, int blen);
 
+/* rtmUpdateFreqContributions followed by rtmFreqDomainImageCondition for
+ * every frequency of a group (RTMPlatform.hpp:93-102; CPU
+ * RTMCPUPlatform.cpp:20-96; used by RTMFreqDomainKernel,
+ * RTMFiniteDifferencesKernel_FreqDomain.cpp:301-315): for iw = iw0 + lw,
+ * lw < nwstep, iw < nw, at time index st, on the interior cells
+ *   sRe += kre[iw][st]*PS; sIm += kim[iw][st]*PS; rRe += kre[iw][st]*PR;
+ *   rIm += kim[iw][st]*PR; img += w2[iw]*(sRe*rRe - sIm*rIm).
+ * sre/sim/rre/rim: [nwstep][nx][ny][nz] (interior sizes); kre, kim:
+ * [nw][nt]; w2: [nw] (squared angular steps). */
+int rtmhip_freq_image(float *img, const float *PS, const float *PR, float *sre, float *sim,
+                      float *rre, float *rim, const float *kre, const float *kim,
+                      const float *w2, int nxe, int nye, int nze, int blen, int nwstep, int iw0,
+                      int nw, int nt, int st);
+
 /* Image post-processing of the controller (RTMAcousticController.cpp:244-249,
  * RTMGrid.cpp:129-166 and :226-300): order-2 Laplacian filter with the
  * [3][3] plane fcoefs (host) followed by border removal; out is the

@@ -83,6 +83,12 @@ def lib():
             getattr(L, "orc_migrate_shot_" + s).argtypes = [p, p, p, p, i, i, i, i, i, i, i, i,
                                                             i, i, C.POINTER(RcvGeom), p, i, p,
                                                             vp]
+            getattr(L, "orc_freq_image_" + s).argtypes = [p, w, w, p, p, p, p, p, p, p, i, i, i,
+                                                          i, i, i, i, i, i]
+            getattr(L, "orc_fmig_shot_" + s).argtypes = [p, p, p, p, i, i, i, i, i, i, i, i, i,
+                                                         i, C.POINTER(RcvGeom), p, p, p, p, i, i,
+                                                         p]
+        L.orc_freq_tables.argtypes = [i, f, f, f, vp, vp, vp, i]
         _lib = L
     return _lib
 
@@ -260,6 +266,42 @@ def migrate_shot(v2, coefs, g, src, blen, order, nt, ntstep, shot, geom, rcv, hb
     return img
 
 
+def freq_tables(nt, dt, fmin, fmax):
+    """(w2[nw], kre[nw][nt], kim[nw][nt]) of the frequency-domain migration."""
+    nw = lib().orc_freq_tables(nt, dt, fmin, fmax, None, None, None, 0)
+    w2 = np.zeros(max(nw, 1), np.float32)
+    kre = np.zeros((max(nw, 1), nt), np.float32)
+    kim = np.zeros((max(nw, 1), nt), np.float32)
+    lib().orc_freq_tables(nt, dt, fmin, fmax, w2.ctypes.data_as(C.c_void_p),
+                          kre.ctypes.data_as(C.c_void_p), kim.ctypes.data_as(C.c_void_p), nw)
+    return w2[:nw], kre[:nw], kim[:nw]
+
+
+def freq_image(img, PS, PR, acc, kre, kim, w2, blen, nwstep, iw0, nt, st, fmt=None):
+    """One time step of frequency contributions + imaging, in place
+    (acc: four [nwstep][nx][ny][nz] arrays sRe, sIm, rRe, rIm)."""
+    getattr(lib(), "orc_freq_image_" + _sfx(img.dtype, fmt))(
+        img, PS, PR, *acc, np.ascontiguousarray(kre), np.ascontiguousarray(kim), w2,
+        *img.shape, blen, nwstep, iw0, len(w2), nt, st)
+
+
+def fmig_shot(v2, coefs, g, src, blen, order, nt, ntstep, shot, geom, rcv, tables, nwstep,
+              img=None):
+    """Frequency-domain migration of one shot, accumulated into img (a new
+    zero image when None) and returned."""
+    w2, kre, kim = tables
+    if img is None:
+        img = np.zeros(v2.shape, v2.dtype)
+    rcv = np.ascontiguousarray(rcv, v2.dtype)
+    rc = getattr(lib(), "orc_fmig_shot_" + _sfx(v2.dtype))(
+        v2, coefs, g, src, *v2.shape, blen, order, nt, ntstep, *shot, C.byref(geom), rcv,
+        np.ascontiguousarray(kre, v2.dtype), np.ascontiguousarray(kim, v2.dtype),
+        np.ascontiguousarray(w2, v2.dtype), len(w2), nwstep, img)
+    if rc != 0:
+        raise RuntimeError("oracle frequency-domain migration failed (%d)" % rc)
+    return img
+
+
 # ---------------- controller-level pipelines ----------------
 def _setup(p, ve, dtype):
     order, blen = p["stencil_order"], p["blen"]
@@ -295,9 +337,22 @@ def pipeline_migration(p, vpe, records, dtype=np.float32, fmt=None):
     bc = p["boundary_condition"]
     hbc = bc != "rbc"
     total = np.zeros(vpe.shape, dtype)
+    if p.get("fmig"):
+        # the CPU reference processes every frequency in one group
+        tables = freq_tables(p["nt"], np.float32(p["dt"]), np.float32(p["fmig_min_freq"]),
+                             np.float32(p["fmig_max_freq"]))
+        # RTMFreqDomainKernel never clears its image grid between shots
+        # (no imgGrid->reset(), unlike RTMFiniteDifferencesKernel_HBC.cpp:257):
+        # every shot image it hands to the stack is the running sum so far
+        running = np.zeros(vpe.shape, dtype)
     for k, s in enumerate(shots_from_params(p)):
-        shot_img = migrate_shot(v2, coefs, g, src, p["blen"], p["stencil_order"], p["nt"],
-                                p["ntstep"], s, geom, records[k], hbc, fmt)
+        if p.get("fmig"):
+            fmig_shot(v2, coefs, g, src, p["blen"], p["stencil_order"], p["nt"], p["ntstep"], s,
+                      geom, records[k], tables, len(tables[0]), running)
+            shot_img = running.copy()
+        else:
+            shot_img = migrate_shot(v2, coefs, g, src, p["blen"], p["stencil_order"], p["nt"],
+                                    p["ntstep"], s, geom, records[k], hbc, fmt)
         total = total + shot_img
     return image_finish(total, p["blen"], float(np.float32(p["dx"])), float(np.float32(p["dy"])),
                         float(np.float32(p["dz"])))

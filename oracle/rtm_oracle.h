@@ -71,6 +71,10 @@ void orc_stencil_coefs_f32(int order, float dx, float dy, float dz, float *out);
 void orc_ricker_f32(int nt, float dt, float fpeak, float *out);
 /* Gaussian taper g[i], i<blen, rtm-3d/include/RTMSeismic.hpp:294-305. */
 void orc_taper_table_f32(int blen, float factor, float *out);
+/* Frequency-domain migration tables (w^2, cos / sin kernels [nw][nt]),
+ * RTMFiniteDifferencesKernel_FreqDomain.cpp:17-52, 125-152; returns nw. */
+int orc_freq_tables(int nt, float dt, float fmin, float fmax, float *w2, float *kre, float *kim,
+                    int cap);
 
 /* glibc TYPE_3 additive-feedback generator (the algorithm behind rand()). */
 typedef struct { int32_t r[34]; int32_t tbl[31]; int f, b; } orc_rand_state;
@@ -134,7 +138,16 @@ int  orc_model_shot_##S(const R *v2, const R *coefs, const R *g, const R *src,  
 int  orc_migrate_shot_##S(const R *v2, const R *coefs, const R *g, const R *src,       \
                           int nxe, int nye, int nze, int blen, int order, int nt,      \
                           int ntstep, int sx, int sy, int sz, const orc_rcv_geom *rg,  \
-                          const R *rcv, int hbc, R *img, W *fwd_last);
+                          const R *rcv, int hbc, R *img, W *fwd_last);                  \
+void orc_freq_image_##S(R *img, const W *PS, const W *PR, R *sre, R *sim, R *rre,       \
+                        R *rim, const R *kre, const R *kim, const R *w2, int nxe,      \
+                        int nye, int nze, int blen, int nwstep, int iw0, int nw,       \
+                        int nt, int st);                                               \
+int  orc_fmig_shot_##S(const R *v2, const R *coefs, const R *g, const R *src,          \
+                       int nxe, int nye, int nze, int blen, int order, int nt,         \
+                       int ntstep, int sx, int sy, int sz, const orc_rcv_geom *rg,     \
+                       const R *rcv, const R *kre, const R *kim, const R *w2, int nw,  \
+                       int nwstep, R *img);
 
 ORC_DECLARE_OPS(float, float, f32)
 ORC_DECLARE_OPS(double, double, f64)

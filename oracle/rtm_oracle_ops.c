@@ -272,6 +272,41 @@ void FN(orc_image_finish)(const R *img, R *out, int nxe, int nye, int nze, int b
             }
 }
 
+/* rtmUpdateFreqContributions then rtmFreqDomainImageCondition
+ * (rtm-3d/src/platform/host/RTMCPUPlatform.cpp:20-96) for the frequencies
+ * iw = iw0 + lw (lw < nwstep, iw < nw) at time index st: on the interior
+ * (x < nx, y < ny, all nz, extended offset +blen), in frequency order,
+ *   sRe += kre*ps; sIm += kim*ps; rRe += kre*pr; rIm += kim*pr;
+ *   img += w2 * (sRe*rRe - sIm*rIm).
+ * Accumulators are [nwstep][nx][ny][nz].  The update of one frequency over
+ * the whole grid precedes its imaging in the reference; both are per cell,
+ * so doing them cell by cell gives the same values. */
+void FN(orc_freq_image)(R *img, const W *PS, const W *PR, R *sre, R *sim, R *rre, R *rim,
+                        const R *kre, const R *kim, const R *w2, int nxe, int nye, int nze,
+                        int blen, int nwstep, int iw0, int nw, int nt, int st)
+{
+    const int nx = nxe - 2 * blen, ny = nye - 2 * blen, nz = nze - 2 * blen;
+    const size_t vol = (size_t)nx * ny * nz;
+    for (int lw = 0, iw = iw0; lw < nwstep && iw < nw; lw++, iw++) {
+        const R kr = kre[(size_t)iw * nt + st], ki = kim[(size_t)iw * nt + st], w = w2[iw];
+        R *a = sre + lw * vol, *b = sim + lw * vol, *c = rre + lw * vol, *d = rim + lw * vol;
+#pragma omp parallel for collapse(2) schedule(static)
+        for (int ix = 0; ix < nx; ix++)
+            for (int iy = 0; iy < ny; iy++) {
+                size_t f = ((size_t)ix * ny + iy) * nz;
+                size_t o = IDX(ix + blen, iy + blen, blen);
+                for (int iz = 0; iz < nz; iz++, f++, o++) {
+                    const R ps = LD(PS[o]), pr = LD(PR[o]);
+                    a[f] = a[f] + kr * ps;
+                    b[f] = b[f] + ki * ps;
+                    c[f] = c[f] + kr * pr;
+                    d[f] = d[f] + ki * pr;
+                    img[o] = img[o] + w * (a[f] * c[f] - b[f] * d[f]);
+                }
+            }
+    }
+}
+
 static W *zalloc(size_t n)
 {
     return (W *)calloc(n, sizeof(W));
@@ -369,5 +404,66 @@ int FN(orc_migrate_shot)(const R *v2, const R *coefs, const R *g, const R *src, 
         { W *t = Pr; Pr = PPr; PPr = t; }
     }
     free(A); free(B); free(C); free(D); free(upb);
+    return rc;
+}
+
+/* RTMFreqDomainKernel::rtmFreqDomainPropagation
+ * (rtm-3d/src/kernel/RTMFiniteDifferencesKernel_FreqDomain.cpp:220-357,
+ * :358-369): for every group of nwstep frequencies, from zero fields: per
+ * receiver chunk (last first) and per step lt descending with rt = itstep +
+ * lt, st = nt-1-rt: step of both fields (rtmStepMultipleWave), taper all
+ * borders of the four grids, source at st into PPs, receivers restored
+ * (set) into PPr at lt, frequency contributions and image at st, swaps.
+ * The image accumulates over the groups. */
+int FN(orc_fmig_shot)(const R *v2, const R *coefs, const R *g, const R *src, int nxe, int nye,
+                      int nze, int blen, int order, int nt, int ntstep, int sx, int sy, int sz,
+                      const orc_rcv_geom *rg, const R *rcv, const R *kre, const R *kim,
+                      const R *w2, int nw, int nwstep, R *img)
+{
+    const size_t E = (size_t)nxe * nye * nze;
+    const size_t vol = (size_t)(nxe - 2 * blen) * (nye - 2 * blen) * (nze - 2 * blen);
+    const size_t chunk = (size_t)rg->count_x * rg->count_y * ntstep;
+    if (nwstep < 1) return -2;
+    W *f[4];
+    R *acc[4];
+    for (int i = 0; i < 4; i++) {
+        f[i] = zalloc(E);
+        acc[i] = (R *)calloc(vol * (size_t)nwstep, sizeof(R));
+    }
+    int rc = 0;
+    for (int gw = 0; gw < nw && rc == 0; gw += nwstep) {
+        for (int i = 0; i < 4; i++) {
+            memset(f[i], 0, E * sizeof(W));
+            memset(acc[i], 0, vol * (size_t)nwstep * sizeof(R));
+        }
+        W *Ps = f[0], *PPs = f[1], *Pr = f[2], *PPr = f[3];
+        for (int itstep = nt - ntstep; itstep >= 0 && rc == 0; itstep -= ntstep) {
+            for (int lt = ntstep - 1; lt >= 0; lt--) {
+                const int rt = itstep + lt, st = nt - 1 - rt;
+                rc = FN(orc_step)(Ps, PPs, v2, coefs, nxe, nye, nze, order);
+                if (FN(orc_step)(Pr, PPr, v2, coefs, nxe, nye, nze, order)) rc = -1;
+                if (rc) break;
+                FN(orc_taper_all)(Ps, g, nxe, nye, nze, blen);
+                FN(orc_taper_all)(PPs, g, nxe, nye, nze, blen);
+                FN(orc_taper_all)(Pr, g, nxe, nye, nze, blen);
+                FN(orc_taper_all)(PPr, g, nxe, nye, nze, blen);
+                FN(orc_apply_source)(PPs, nxe, nye, nze, sx + blen, sy + blen, sz + blen, src[st]);
+                FN(orc_receivers)(PPr, (R *)rcv + (size_t)(itstep / ntstep) * chunk, nxe, nye,
+                                  nze, blen, rg, ntstep, lt, 1);
+                FN(orc_freq_image)(img, PPs, PPr, acc[0], acc[1], acc[2], acc[3], kre, kim, w2,
+                                   nxe, nye, nze, blen, nwstep, gw, nw, nt, st);
+                W *t = Ps;
+                Ps = PPs;
+                PPs = t;
+                t = Pr;
+                Pr = PPr;
+                PPr = t;
+            }
+        }
+    }
+    for (int i = 0; i < 4; i++) {
+        free(f[i]);
+        free(acc[i]);
+    }
     return rc;
 }

@@ -258,6 +258,36 @@ void orc_v2dt2_f32(float *v, size_t n, float dt)
     }
 }
 
+/* Frequency tables of RTMFreqDomainKernel::initKernel
+ * (rtm-3d/src/kernel/RTMFiniteDifferencesKernel_FreqDomain.cpp:125-152),
+ * rtmInitWList (:17-33) and rtmInitFreqKernels (:35-52), with the
+ * reference's float members (freq_min, freq_max, df) and its float/double
+ * promotions: df = 1/(double(nt)*dt) stored as float; nw = size_t of the
+ * float (fmax - fmin)/df; w[iw] = float(2*double(pi_f)*((float(iw)*df) +
+ * fmin)*double(dt)); kre/kim[iw][it] = cosf/sinf(float(double(w)*it));
+ * w2 = w*w in float.  Returns nw; the tables are written when nw <= cap. */
+int orc_freq_tables(int nt, float dt, float fmin, float fmax, float *w2, float *kre, float *kim,
+                    int cap)
+{
+    const float freq_max = 1.0 * fmax;
+    const float freq_min = 1.0 * fmin;
+    const float df = 1. / ((nt * 1.0) * dt);
+    const float nf = ((freq_max - freq_min) / df);
+    const int nw = (int)(size_t)nf;
+    if (nw > cap) return nw;
+    const float pi = acosf(-1.);
+    for (int iw = 0; iw < nw; iw++) {
+        const float lw = (float)iw * 1.0;
+        const float w = (float)(2. * pi * ((lw * df) + freq_min) * dt);
+        for (int it = 0; it < nt; it++) {
+            kre[(size_t)iw * nt + it] = cosf(w * (it * 1.0));
+            kim[(size_t)iw * nt + it] = sinf(w * (it * 1.0));
+        }
+        w2[iw] = w * w;
+    }
+    return nw;
+}
+
 /* ---------------- double-precision set-up (ideal values) -------------- */
 void orc_stencil_coefs_f64(int order, double dx, double dy, double dz, double *out)
 {

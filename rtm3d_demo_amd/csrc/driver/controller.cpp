@@ -72,8 +72,6 @@ Cube prepare_velocity(const RTMParam &p)
 
 void run(RTMParam &p, int device)
 {
-    if (p.fmig)
-        throw RTMException("frequency-domain migration (fmig) is not provided by this build");
     if (p.nx <= 1) throw RTMException("2D models (nx == 1) are not provided by this build");
     if (p.receiver_count_x > p.nx || p.receiver_count_y > p.ny)
         throw RTMException("[ Invalid Receiver Grid Size: (" + std::to_string(p.receiver_count_x) +

@@ -273,4 +273,34 @@ std::string image_name(const std::string &outdir, std::string mname, int ns, int
            std::to_string(nt) + ".bin.NP" + std::to_string(np);
 }
 
+// RTMFreqDomainKernel::initKernel, rtmInitWList and rtmInitFreqKernels
+// (rtm-3d/src/kernel/RTMFiniteDifferencesKernel_FreqDomain.cpp:17-52,
+// 125-152) with the reference's float members and promotions: df is the
+// float of 1/(double(nt)*dt); nw truncates the float (fmax - fmin)/df; the
+// angular step w = float(2*double(pi_f)*(float(iw)*df + fmin)*double(dt));
+// the kernels are cosf / sinf of float(double(w)*it); w2 = w*w in float.
+FreqTables freq_tables(int nt, float dt, float fmin, float fmax)
+{
+    FreqTables t;
+    const float freq_max = 1.0 * fmax;
+    const float freq_min = 1.0 * fmin;
+    const float df = 1. / ((nt * 1.0) * dt);
+    const float nf = ((freq_max - freq_min) / df);
+    t.nw = nf > 0 ? (int)(size_t)nf : 0;
+    const float pi = acosf(-1.);
+    t.w2.resize(t.nw);
+    t.kre.resize((size_t)t.nw * nt);
+    t.kim.resize((size_t)t.nw * nt);
+    for (int iw = 0; iw < t.nw; ++iw) {
+        const float lw = (float)iw * 1.0;
+        const float w = (float)(2. * pi * ((lw * df) + freq_min) * dt);
+        for (int it = 0; it < nt; ++it) {
+            t.kre[(size_t)iw * nt + it] = cosf(w * (it * 1.0));
+            t.kim[(size_t)iw * nt + it] = sinf(w * (it * 1.0));
+        }
+        t.w2[iw] = w * w;
+    }
+    return t;
+}
+
 }  // namespace rtm3d

@@ -46,6 +46,13 @@ std::vector<float> laplacian_taps(int order);
 std::vector<float> stencil_coefs(int order, float dx, float dy, float dz);
 std::vector<float> ricker(int nt, float dt, float fpeak);
 std::vector<float> taper_table(int blen, float factor);
+// Frequency-domain migration tables: the number of frequencies nw, the
+// squared angular steps w2[nw] and the DFT kernels kre / kim [nw][nt].
+struct FreqTables {
+    int nw = 0;
+    std::vector<float> w2, kre, kim;
+};
+FreqTables freq_tables(int nt, float dt, float fmin, float fmax);
 
 std::string seismogram_name(const std::string &datdir, int sx, int sy, int sz, int nrx, int nry,
                             int itstep, int ntstep);

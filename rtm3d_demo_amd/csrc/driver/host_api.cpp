@@ -51,6 +51,20 @@ void rtm3d_v2dt2(float *v, size_t n, float dt)
     }
 }
 
+// Frequency-domain migration tables; returns nw and writes w2[nw] and
+// kre / kim [nw][nt] when nw <= cap.
+int rtm3d_freq_tables(int nt, float dt, float fmin, float fmax, float *w2, float *kre, float *kim,
+                      int cap)
+{
+    const FreqTables t = freq_tables(nt, dt, fmin, fmax);
+    if (t.nw <= cap && t.nw > 0) {
+        std::memcpy(w2, t.w2.data(), t.w2.size() * sizeof(float));
+        std::memcpy(kre, t.kre.data(), t.kre.size() * sizeof(float));
+        std::memcpy(kim, t.kim.data(), t.kim.size() * sizeof(float));
+    }
+    return t.nw;
+}
+
 int rtm3d_run_json(const char *path, int device) { return run_json(path, device); }
 
 }  // extern "C"

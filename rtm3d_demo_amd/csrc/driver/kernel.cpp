@@ -1,6 +1,7 @@
 // kernel.cpp -- per-shot time loops (see kernel.hpp).
 #include "kernel.hpp"
 
+#include <algorithm>
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -9,6 +10,7 @@
 #include <iostream>
 #include <sstream>
 #include <stdexcept>
+#include <utility>
 
 namespace rtm3d {
 
@@ -89,8 +91,36 @@ void RTMKernel::initKernel()
     host_rcv_.assign(nrcv ? nrcv : 1, 0.f);
     if (!p_.modeling) {
         img_.createDeviceBuffer(E);
-        if (hbc_)
+        img_.devMemSet(0.f);
+        if (p_.fmig) {
+            if (wave_format_ != RTMHIP_WAVE_F32)
+                throw std::runtime_error("frequency-domain migration runs FP32 wavefields only");
+            freq_ = freq_tables((int)p_.nt, p_.dt, p_.fmig_min_freq, p_.fmig_max_freq);
+            if (freq_.nw <= 0)
+                throw std::runtime_error("frequency-domain migration: no frequency in the range");
+            kre_.createDeviceBuffer(freq_.kre.size());
+            kre_.moveToDevice(freq_.kre.data());
+            kim_.createDeviceBuffer(freq_.kim.size());
+            kim_.moveToDevice(freq_.kim.data());
+            w2_.createDeviceBuffer(freq_.w2.size());
+            w2_.moveToDevice(freq_.w2.data());
+            // every frequency in one group (the reference CPU platform's
+            // choice, RTMFiniteDifferencesKernel_FreqDomain.cpp:133-142) when
+            // the four accumulators can be allocated; otherwise groups of
+            // fmig_nwstep, as the reference's accelerator builds do
+            const size_t vol = (size_t)p_.nx * p_.ny * p_.nz;
+            nwstep_ = freq_.nw;
+            try {
+                for (auto &a : acc_) a.createDeviceBuffer(vol * (size_t)nwstep_);
+            } catch (const RTMException &) {
+                for (auto &a : acc_) a.removeDeviceBuffer();
+                if (p_.fmig_nwstep == 0) throw std::runtime_error("fmig_nwstep must be positive");
+                nwstep_ = (int)std::min<unsigned>(p_.fmig_nwstep, (unsigned)freq_.nw);
+                for (auto &a : acc_) a.createDeviceBuffer(vol * (size_t)nwstep_);
+            }
+        } else if (hbc_) {
             upb_.createDeviceBuffer(wave_slots((size_t)p_.nt * d_.nxe * d_.nye * (d_.order / 2)));
+        }
     }
 }
 
@@ -200,6 +230,7 @@ void RTMKernel::rtmModel(const ShotPos &s, const DeviceGrid &v2dt2)
 
 void RTMKernel::rtmMigrate(const ShotPos &s, const DeviceGrid &v2dt2)
 {
+    if (p_.fmig) return rtmMigrateFreq(s, v2dt2);
     const double t0 = now_s();
     const rtmhip_shot sh = make_shot(s);
     const std::vector<float> src = ricker((int)p_.nt, p_.dt, p_.fpeak);
@@ -283,6 +314,78 @@ void RTMKernel::rtmMigrate(const ShotPos &s, const DeviceGrid &v2dt2)
     report_.migration_n++;
     const double cells = (double)(d_.nxe - d_.order) * (d_.nye - d_.order) * (d_.nze - d_.order);
     report_.cell_updates += cells * (3.0 * nt - 2.0);
+}
+
+// Frequency-domain migration of one shot
+// (RTMFreqDomainKernel::rtmFreqDomainPropagation and rtmUpdatePressureGrids,
+// rtm-3d/src/kernel/RTMFiniteDifferencesKernel_FreqDomain.cpp:220-369).  Per
+// group of frequencies, from zero wavefields and accumulators: per receiver
+// chunk (last first), steps lt descending with rt = itstep + lt and
+// st = nt-1-rt: both fields stepped (rtmStepMultipleWave: two independent
+// steps), all borders of the four grids tapered, the source at st added to
+// PPs, the receivers of lt set into PPr, the contributions and the image of
+// the group's frequencies at st, then the swaps.  The image grid is not
+// cleared between shots, as in the reference: each shot image handed to the
+// stack is the running sum of the shots so far.
+void RTMKernel::rtmMigrateFreq(const ShotPos &s, const DeviceGrid &v2dt2)
+{
+    const double t0 = now_s();
+    const int nt = (int)p_.nt, ntstep = (int)p_.ntstep, b = d_.blen;
+    const int nxe = d_.nxe, nye = d_.nye, nze = d_.nze;
+    const std::vector<float> src = ricker(nt, p_.dt, p_.fpeak);
+    const rtmhip_rcv_geom g = geom();
+    std::string snapfile;
+    if (p_.save_snapshots) {
+        snapfile = p_.tmpdir + "/RTMSNAPSHOT_S" + std::to_string(s.sx) + "x" +
+                   std::to_string(s.sy) + "_" + std::to_string(nt / (int)p_.snapshot_step) +
+                   "x" + std::to_string(nye) + "x" + std::to_string(nze) + ".snapshot.FREQIMG";
+        std::ofstream(snapfile, std::ios::out | std::ios::binary);
+    }
+    for (int gw = 0; gw < freq_.nw; gw += nwstep_) {
+        for (auto &f : f_) f.devMemSet(0.f);
+        for (auto &a : acc_) a.devMemSet(0.f);
+        float *Ps = f_[0].ptr(), *PPs = f_[1].ptr(), *Pr = f_[2].ptr(), *PPr = f_[3].ptr();
+        for (int itstep = nt - ntstep; itstep >= 0; itstep -= ntstep) {
+            load_raw(seismogram_name(p_.datdir, s.sx, s.sy, s.sz, (int)p_.receiver_count_x,
+                                     (int)p_.receiver_count_y, itstep, ntstep),
+                     host_rcv_.data(), host_rcv_.size());
+            hipcheck(rtmhip_copy_to_device(rcv_.ptr(), host_rcv_.data(),
+                                           host_rcv_.size() * sizeof(float)),
+                     "receiver chunk to device");
+            for (int lt = ntstep - 1; lt >= 0; --lt) {
+                const int rt = itstep + lt, st = nt - 1 - rt;
+                hipcheck(rtmhip_step(Ps, PPs, v2dt2.ptr(), coefs_.data(), nxe, nye, nze,
+                                     d_.order),
+                         "source step");
+                hipcheck(rtmhip_step(Pr, PPr, v2dt2.ptr(), coefs_.data(), nxe, nye, nze,
+                                     d_.order),
+                         "receiver step");
+                for (float *f : {Ps, PPs, Pr, PPr})
+                    hipcheck(rtmhip_taper(f, taper_.ptr(), nxe, nye, nze, b, 0), "taper");
+                hipcheck(rtmhip_apply_source(PPs, nxe, nye, nze, s.sx + b, s.sy + b, s.sz + b,
+                                             src[st]),
+                         "source");
+                hipcheck(rtmhip_receivers(PPr, rcv_.ptr(), nxe, nye, nze, b, &g, ntstep, lt, 1),
+                         "receivers");
+                hipcheck(rtmhip_freq_image(img_.ptr(), PPs, PPr, acc_[0].ptr(), acc_[1].ptr(),
+                                           acc_[2].ptr(), acc_[3].ptr(), kre_.ptr(), kim_.ptr(),
+                                           w2_.ptr(), nxe, nye, nze, b, nwstep_, gw, freq_.nw,
+                                           nt, st),
+                         "frequency contributions");
+                if (p_.save_snapshots && st % (int)p_.snapshot_step == 0)
+                    snapshot(img_.ptr(), s.sx, snapfile);
+                std::swap(Ps, PPs);
+                std::swap(Pr, PPr);
+            }
+            progress("+ FMIG", s, nt - itstep, t0);
+        }
+    }
+    plat_.synchronize();
+    const double t1 = now_s();
+    report_.migration_s += t1 - t0;
+    report_.migration_n++;
+    const double cells = (double)(nxe - d_.order) * (nye - d_.order) * (nze - d_.order);
+    report_.cell_updates += cells * 2.0 * nt * (double)((freq_.nw + nwstep_ - 1) / nwstep_);
 }
 
 }  // namespace rtm3d

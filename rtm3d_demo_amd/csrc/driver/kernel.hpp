@@ -3,7 +3,9 @@
 // Mirrors RTMFiniteDifferencesKernel::rtmModel (rtm-3d/src/kernel/
 // RTMFiniteDifferencesKernel.cpp:103-246) and RTMHBCKernel / RTMRBCKernel
 // ::rtmMigrate (rtm-3d/src/kernel/RTMFiniteDifferencesKernel_HBC.cpp:64-365,
-// ..._RBC.cpp:36-274): same operator order, same chunked seismogram files,
+// ..._RBC.cpp:36-274) and RTMFreqDomainKernel::rtmMigrate
+// (rtm-3d/src/kernel/RTMFiniteDifferencesKernel_FreqDomain.cpp:207-369):
+// same operator order, same chunked seismogram files,
 // same outputs.  Differences by design: wavefields, the upper-border history
 // and the receiver chunks stay in device memory for the whole shot (no per
 // operator synchronisation, no UPB spill files), and the receiver step of the
@@ -52,6 +54,11 @@ private:
     DeviceGrid f_[4];  // two source and two receiver wavefields
     KernelReport report_;
     std::vector<float> host_rcv_;
+    // frequency-domain migration (fmig)
+    FreqTables freq_;
+    int nwstep_ = 0;
+    DeviceGrid kre_, kim_, w2_, acc_[4];  // accumulators sRe, sIm, rRe, rIm
+    void rtmMigrateFreq(const ShotPos &s, const DeviceGrid &v2dt2);
 
     void read_options();
     rtmhip_shot make_shot(const ShotPos &s) const;

@@ -453,6 +453,27 @@ int rtmhip_image_condition(float *img, const float *PS, const float *PR, int nxe
     return RTMHIP_OK;
 }
 
+int rtmhip_freq_image(float *img, const float *PS, const float *PR, float *sre, float *sim,
+                      float *rre, float *rim, const float *kre, const float *kim,
+                      const float *w2, int nxe, int nye, int nze, int blen, int nwstep, int iw0,
+                      int nw, int nt, int st)
+{
+    NEED_INIT();
+    int rc;
+    if ((rc = check_grid(nxe, nye, nze))) return rc;
+    if (!img || !PS || !PR || !sre || !sim || !rre || !rim || !kre || !kim || !w2)
+        return fail(RTMHIP_ERR_INVALID, "NULL pointer");
+    if (blen < 0 || 2 * blen >= nxe || 2 * blen >= nye || 2 * blen >= nze)
+        return fail(RTMHIP_ERR_INVALID, "blen %d leaves no interior", blen);
+    if (nwstep < 1 || nw < 0 || iw0 < 0 || nt <= 0 || st < 0 || st >= nt)
+        return fail(RTMHIP_ERR_INVALID, "bad frequency group or time index");
+    HIPCHK(launch_freq_image(img, PS, PR, sre, sim, rre, rim, kre, kim, w2,
+                             dense_grid(nxe, nye, nze), blen, nwstep, iw0, nw, nt, st,
+                             g_state.stream),
+           "frequency contributions");
+    return RTMHIP_OK;
+}
+
 int rtmhip_image_finish(const float *img, float *out, int nxe, int nye, int nze, int blen,
                         const float *fcoefs)
 {

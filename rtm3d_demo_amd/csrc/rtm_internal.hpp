@@ -178,6 +178,12 @@ int tail_blocks(long long items);
 // *p += d, or *p = d with set (one thread): the device-resident time index
 // of a captured step sequence.
 hipError_t launch_tick(int *p, int d, bool set, hipStream_t s);
+// Frequency-domain migration: contributions of the frequencies iw0 ..
+// iw0+nwstep-1 (< nw) at time index st and the imaging condition, FP32.
+hipError_t launch_freq_image(float *img, const float *PS, const float *PR, float *sre,
+                             float *sim, float *rre, float *rim, const float *kre,
+                             const float *kim, const float *w2, const Grid &g, int b, int nwstep,
+                             int iw0, int nw, int nt, int st, hipStream_t s);
 hipError_t launch_fill(float *p, size_t n, float v, hipStream_t s);
 hipError_t launch_fill_u16(uint16_t *p, size_t n, uint16_t bits, hipStream_t s);
 hipError_t launch_stack(float *dst, const float *src, size_t n, hipStream_t s);

@@ -401,6 +401,44 @@ __global__ void __launch_bounds__(kThreads)
     out[i] = ax + ay + az;
 }
 
+// rtmUpdateFreqContributions then rtmFreqDomainImageCondition
+// (RTMCPUPlatform.cpp:20-96) for the frequencies iw = iw0 + lw (lw < nwstep,
+// iw < nw) at time index st, one thread per interior cell (extended offset
+// +b), frequencies in order: sRe += kre*ps, sIm += kim*ps, rRe += kre*pr,
+// rIm += kim*pr, img += w2*(sRe*rRe - sIm*rIm).  Accumulators are
+// [nwstep][nx][ny][nz]; the image stays in a register across the group.
+__global__ void __launch_bounds__(kThreads)
+    k_freq_image(float *__restrict__ img, const float *__restrict__ PS,
+                 const float *__restrict__ PR, float *__restrict__ sre, float *__restrict__ sim,
+                 float *__restrict__ rre, float *__restrict__ rim, const float *__restrict__ kre,
+                 const float *__restrict__ kim, const float *__restrict__ w2, Grid g, int b,
+                 int nx, int ny, int nz, int nwstep, int iw0, int nw, int nt, int st)
+{
+    const long long vol = (long long)nx * ny * nz;
+    const long long i = (long long)blockIdx.x * kThreads + threadIdx.x;
+    if (i >= vol) return;
+    const int iz = (int)(i % nz);
+    const long long r = i / nz;
+    const int iy = (int)(r % ny), ix = (int)(r / ny);
+    const long long o = goff(g, ix + b, iy + b, iz + b);
+    const float ps = PS[o], pr = PR[o];
+    float im = img[o];
+    for (int lw = 0, iw = iw0; lw < nwstep && iw < nw; ++lw, ++iw) {
+        const float kr = kre[(long long)iw * nt + st], ki = kim[(long long)iw * nt + st];
+        const long long f = lw * vol + i;
+        const float a = sre[f] + kr * ps;
+        const float c2 = sim[f] + ki * ps;
+        const float c = rre[f] + kr * pr;
+        const float d = rim[f] + ki * pr;
+        sre[f] = a;
+        sim[f] = c2;
+        rre[f] = c;
+        rim[f] = d;
+        im = im + w2[iw] * (a * c - c2 * d);
+    }
+    img[o] = im;
+}
+
 __global__ void __launch_bounds__(kThreads) k_fill(float *__restrict__ p, size_t n, float v)
 {
     for (size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x; i < n;
@@ -661,6 +699,19 @@ hipError_t launch_image_finish(const float *img, float *out, const Grid &g, int 
     const long long n = (long long)(g.nx - 2 * b) * (g.ny - 2 * b) * (g.nz - 2 * b);
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_image_finish, dim3(nblocks(n)), dim3(kThreads), 0, s, img, out, g, b, fc);
+    return hipGetLastError();
+}
+
+hipError_t launch_freq_image(float *img, const float *PS, const float *PR, float *sre,
+                             float *sim, float *rre, float *rim, const float *kre,
+                             const float *kim, const float *w2, const Grid &g, int b, int nwstep,
+                             int iw0, int nw, int nt, int st, hipStream_t s)
+{
+    const int nx = g.nx - 2 * b, ny = g.ny - 2 * b, nz = g.nz - 2 * b;
+    const long long vol = (long long)nx * ny * nz;
+    if (vol <= 0 || nwstep <= 0 || iw0 >= nw) return hipSuccess;
+    hipLaunchKernelGGL(k_freq_image, dim3(nblocks(vol)), dim3(kThreads), 0, s, img, PS, PR, sre,
+                       sim, rre, rim, kre, kim, w2, g, b, nx, ny, nz, nwstep, iw0, nw, nt, st);
     return hipGetLastError();
 }
 

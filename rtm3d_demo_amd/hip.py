@@ -27,7 +27,7 @@ EXPORTED = [
     "rtmhip_batch_set_graph",
     "rtmhip_ae_param_count", "rtmhip_ae_create", "rtmhip_ae_destroy", "rtmhip_ae_get_params",
     "rtmhip_ae_set_params", "rtmhip_ae_gradient", "rtmhip_ae_train", "rtmhip_ae_bricks",
-    "rtmhip_ae_encode", "rtmhip_ae_decode", "rtmhip_ae_gemm_test",
+    "rtmhip_ae_encode", "rtmhip_ae_decode", "rtmhip_ae_gemm_test", "rtmhip_freq_image",
 ]
 
 
@@ -104,6 +104,7 @@ def load():
         "rtmhip_batch_fields": [vp, i, pp, pp, pp, pp],
         "rtmhip_batch_set_stencil": [vp, i, i],
         "rtmhip_batch_set_graph": [vp, i],
+        "rtmhip_freq_image": [fp] * 10 + [i] * 9,
         "rtmhip_ae_param_count": [],
         "rtmhip_ae_create": [pp, i, C.c_ulonglong],
         "rtmhip_ae_destroy": [vp],
@@ -233,6 +234,13 @@ def image_condition(img, PS, PR, blen):
 def image_finish(img, out, blen, fcoefs):
     c = _f32(fcoefs)
     check(load().rtmhip_image_finish(img.ptr, out.ptr, *img.shape, blen, _host_ptr(c)))
+
+
+def freq_image(img, PS, PR, acc, kre, kim, w2, blen, nwstep, iw0, nw, nt, st):
+    """Frequency contributions of iw0 .. iw0+nwstep-1 at time st and the
+    imaging condition (acc: four DeviceArrays sRe, sIm, rRe, rIm)."""
+    check(load().rtmhip_freq_image(img.ptr, PS.ptr, PR.ptr, *[a.ptr for a in acc], kre.ptr,
+                                   kim.ptr, w2.ptr, *img.shape, blen, nwstep, iw0, nw, nt, st))
 
 
 def stack(dst, src):

@@ -25,6 +25,8 @@ def load():
         L.rtm3d_taper_table.argtypes = [i, f, _f32p]
         L.rtm3d_extend_velocity.argtypes = [_f32p, i, i, i, i, i, _f32p]
         L.rtm3d_v2dt2.argtypes = [_f32p, C.c_size_t, f]
+        L.rtm3d_freq_tables.argtypes = [i, f, f, f, C.c_void_p, C.c_void_p, C.c_void_p, i]
+        L.rtm3d_freq_tables.restype = i
         L.rtm3d_run_json.argtypes = [C.c_char_p, i]
         L.rtm3d_run_json.restype = i
         _lib = L
@@ -61,6 +63,18 @@ def v2dt2(ve, dt):
     out = np.array(ve, np.float32, copy=True)
     load().rtm3d_v2dt2(out, out.size, dt)
     return out
+
+
+def freq_tables(nt, dt, fmin, fmax):
+    """Frequency-domain migration tables: (w2[nw], kre[nw][nt], kim[nw][nt])."""
+    L = load()
+    nw = L.rtm3d_freq_tables(nt, dt, fmin, fmax, None, None, None, 0)
+    w2 = np.zeros(max(nw, 1), np.float32)
+    kre = np.zeros((max(nw, 1), nt), np.float32)
+    kim = np.zeros((max(nw, 1), nt), np.float32)
+    L.rtm3d_freq_tables(nt, dt, fmin, fmax, w2.ctypes.data_as(C.c_void_p),
+                        kre.ctypes.data_as(C.c_void_p), kim.ctypes.data_as(C.c_void_p), nw)
+    return w2[:nw], kre[:nw], kim[:nw]
 
 
 def run_json(path, device=0):

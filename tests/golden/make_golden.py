@@ -14,6 +14,11 @@ Cases (all grids non-cubic with distinct spacings, so a transposed axis fails):
   small_hbc   HBC migration of small_mod's records (nthreads = 1, vpe saved)
   small_rbc   RBC migration of the same records (nthreads = 1, vpe saved)
   o4_mod      order-4 modeling, blen 6, odd sizes
+  small_fmig  frequency-domain migration (fmig) of small_mod's records,
+              1-30 Hz (7 frequencies), nthreads = 1, vpe saved
+
+``make_golden.py fmig`` regenerates only small_fmig (from small_mod.npz's
+records), leaving the other fixtures untouched.
 """
 import json
 import os
@@ -109,21 +114,69 @@ def image_name(p):
         p["nz"], p["nt"])
 
 
+def small_params():
+    nx, ny, nz, blen = 36, 44, 40, 10
+    p = base_params("small", nx, ny, nz, blen, 300, 150, 10.0, 12.0, 8.0, 8)
+    p.update({"source_count_x": 2, "source_distance_x": 5, "source_start_x": 17,
+              "source_start_y": 21, "source_depth_z": 3,
+              "receiver_start_x": 1, "receiver_start_y": 2, "receiver_depth_z": 1,
+              "receiver_distance_x": 2, "receiver_distance_y": 3,
+              "receiver_count_x": 16, "receiver_count_y": 13})
+    return p
+
+
+def fmig_case(wd, p, recs):
+    """Frequency-domain migration of the small case's records."""
+    for k, s in enumerate(shots(p)):
+        for c, itstep in enumerate(range(0, p["nt"], p["ntstep"])):
+            recs[k][c].astype(np.float32).tofile(os.path.join(wd, "dat", seism_name(p, s, itstep)))
+    q = dict(p)
+    q.update({"modeling": False, "migration": True, "boundary_condition": "hbc", "fmig": True,
+              "fmig_min_freq": 1.0, "fmig_max_freq": 30.0, "fmig_nwstep": 1,
+              "nthreads": 1, "save_vpe_file": True, "mname": "small_fmig"})
+    run(wd, q)
+    img = np.fromfile(os.path.join(wd, "out", image_name(q)), np.float32)
+    img = img.reshape(p["nx"], p["ny"], p["nz"])
+    vpe = np.fromfile(os.path.join(wd, "vel", "model_ext.bin"), np.float32)
+    b = p["blen"]
+    vpe = vpe.reshape(p["nx"] + 2 * b, p["ny"] + 2 * b, p["nz"] + 2 * b)
+    np.savez_compressed(os.path.join(HERE, "small_fmig.npz"), image=img, vpe=vpe)
+    return q
+
+
+def main_fmig():
+    tmp = tempfile.mkdtemp(prefix="rtmgold_")
+    try:
+        p = small_params()
+        fx = np.load(os.path.join(HERE, "small_mod.npz"), allow_pickle=False)
+        wd = os.path.join(tmp, "small")
+        for d in ("vel", "out", "tmp", "dat"):
+            os.makedirs(os.path.join(wd, d), exist_ok=True)
+        fx["vel"].tofile(os.path.join(wd, "vel", "model.bin"))
+        q = fmig_case(wd, p, fx["records"])
+    finally:
+        shutil.rmtree(tmp)
+    path = os.path.join(HERE, "cases.json")
+    with open(path) as f:
+        cases = json.load(f)
+    cases["small_fmig"] = q
+    with open(path, "w") as f:
+        json.dump(cases, f, indent=1, sort_keys=True)
+    print("fixture written: small_fmig")
+
+
 def main():
     if not os.path.exists(REFBIN):
         raise SystemExit("build the reference first: make -C oracle ref")
+    if sys.argv[1:] == ["fmig"]:
+        return main_fmig()
     cases = {}
     tmp = tempfile.mkdtemp(prefix="rtmgold_")
     try:
         # ---- small: modeling, then HBC and RBC migration of its records ----
         nx, ny, nz, blen = 36, 44, 40, 10
         vel = three_layer(nx, ny, nz, 13, 27)
-        p = base_params("small", nx, ny, nz, blen, 300, 150, 10.0, 12.0, 8.0, 8)
-        p.update({"source_count_x": 2, "source_distance_x": 5, "source_start_x": 17,
-                  "source_start_y": 21, "source_depth_z": 3,
-                  "receiver_start_x": 1, "receiver_start_y": 2, "receiver_depth_z": 1,
-                  "receiver_distance_x": 2, "receiver_distance_y": 3,
-                  "receiver_count_x": 16, "receiver_count_y": 13})
+        p = small_params()
         wd = os.path.join(tmp, "small")
         os.makedirs(os.path.join(wd, "vel"))
         vel.tofile(os.path.join(wd, "vel", "model.bin"))
@@ -142,6 +195,7 @@ def main():
             vpe = vpe.reshape(nx + 2 * blen, ny + 2 * blen, nz + 2 * blen)
             np.savez_compressed(os.path.join(HERE, "small_%s.npz" % bc), image=img, vpe=vpe)
             cases["small_" + bc] = q
+        cases["small_fmig"] = fmig_case(wd, p, recs)
 
         # ---- order-4 modeling, odd sizes, one chunk ----
         nx, ny, nz, blen = 21, 27, 25, 6

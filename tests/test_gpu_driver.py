@@ -119,3 +119,20 @@ def test_driver_wave_format_and_backward_mode(workdir, cases, golden, oracle, fm
     img = np.fromfile(os.path.join(workdir, "out", name), np.float32)
     want = oracle.pipeline_migration(q, vpe, recs, np.float32, ofmt)
     assert np.array_equal(img.view(np.uint32), want.ravel().view(np.uint32))
+
+
+def test_driver_fmig(workdir, cases, golden):
+    """Frequency-domain migration (fmig) through the driver: the reference's
+    image bit for bit, with its running image across the two shots."""
+    q = dict(cases["small_fmig"])
+    recs = golden("small_mod")["records"]
+    for k, s in enumerate(_shots(q)):
+        for c, itstep in enumerate(range(0, q["nt"], q["ntstep"])):
+            recs[k][c].astype(np.float32).tofile(os.path.join(workdir, "dat", _seism(q, s, itstep)))
+    _run(workdir, q)
+    vpe = np.fromfile(os.path.join(workdir, "vel", "model_ext.bin"), np.float32)
+    assert np.array_equal(vpe, golden("small_fmig")["vpe"].ravel())
+    name = "RTMIMG_%s_NS2_NXNYNZ_%dx%dx%d_NT%d.bin.NP1" % (
+        q["mname"].upper(), q["nx"], q["ny"], q["nz"], q["nt"])
+    img = np.fromfile(os.path.join(workdir, "out", name), np.float32)
+    assert np.array_equal(img.view(np.uint32), golden("small_fmig")["image"].ravel().view(np.uint32))

@@ -148,3 +148,28 @@ def test_image_stack_finish(gpu, oracle):
     assert _bits_equal(out.download(), oracle.image_finish(want, b, 10.0, 12.0, 8.0))
     for x in d + [out]:
         x.free()
+
+
+@pytest.mark.parametrize("nwstep, iw0", [(5, 0), (3, 2), (4, 3)])
+def test_freq_image_matches_oracle(gpu, oracle, nwstep, iw0):
+    """Frequency contributions + imaging (rtmUpdateFreqContributions and
+    rtmFreqDomainImageCondition) for a group of frequencies, bit-exact
+    against the oracle on non-zero accumulators; the last group is cut at
+    nw."""
+    rng = np.random.default_rng(40 + iw0)
+    shape, b, nt, st = (30, 34, 28), 6, 40, 17
+    nw = 5
+    inner = tuple(n - 2 * b for n in shape)
+    PS, PR, img = (rng.uniform(-1, 1, shape).astype(np.float32) for _ in range(3))
+    acc = [rng.uniform(-2, 2, (nwstep,) + inner).astype(np.float32) for _ in range(4)]
+    w2, kre, kim = oracle.freq_tables(nt, np.float32(0.004), np.float32(2.0), np.float32(40.0))
+    assert len(w2) >= nw  # the kernel reads kre / kim rows iw0 .. nw-1
+    w2, kre, kim = (np.ascontiguousarray(a[:nw]) for a in (w2, kre, kim))
+    d = [gpu.DeviceArray.from_host(a) for a in [img, PS, PR] + acc + [kre, kim, w2]]
+    gpu.freq_image(d[0], d[1], d[2], d[3:7], d[7], d[8], d[9], b, nwstep, iw0, nw, nt, st)
+    oracle.freq_image(img, PS, PR, acc, kre, kim, w2, b, nwstep, iw0, nt, st)
+    assert np.array_equal(d[0].download().view(np.uint32), img.view(np.uint32))
+    for k in range(4):
+        assert np.array_equal(d[3 + k].download().view(np.uint32), acc[k].view(np.uint32))
+    for x in d:
+        x.free()

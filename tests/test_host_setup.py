@@ -34,3 +34,16 @@ def test_abc_and_v2dt2(oracle):
     b = oracle.extend_velocity(vel, 5, "abc")
     assert np.array_equal(a, b)
     assert np.array_equal(host.v2dt2(a, 1.1e-3), oracle.v2dt2(b, np.float32(1.1e-3)))
+
+
+def test_freq_tables_match_oracle(oracle):
+    """Frequency-domain migration tables of the driver (w^2, cos / sin
+    kernels) bit-equal to the oracle's restatement, which the reference's
+    fmig image pins."""
+    from rtm3d_demo_amd import host
+    for nt, dt, fmin, fmax in ((300, 9.142857e-4, 1.0, 30.0), (1000, 1.1e-3, 2.0, 60.0)):
+        a = host.freq_tables(nt, np.float32(dt), np.float32(fmin), np.float32(fmax))
+        b = oracle.freq_tables(nt, np.float32(dt), np.float32(fmin), np.float32(fmax))
+        assert len(a[0]) == len(b[0]) > 0
+        for x, y in zip(a, b):
+            assert np.array_equal(x.view(np.uint32), y.view(np.uint32))

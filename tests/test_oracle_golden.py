@@ -47,3 +47,30 @@ def test_oracle_fixture_differs_from_f64_only_by_rounding(oracle, cases, golden)
     r32 = fx["records"].astype(np.float64)
     rel = np.linalg.norm(r64 - r32) / np.linalg.norm(r64)
     assert rel < 1e-4, rel
+
+
+def test_fmig_image_bit_exact(oracle, cases, golden):
+    """Frequency-domain migration (fmig, 7 frequencies from 1 to 30 Hz, two
+    shots) against the reference program's image, including its running image
+    across shots (its frequency-domain kernel never clears the image grid)."""
+    p = cases["small_fmig"]
+    fx = golden("small_fmig")
+    img = oracle.pipeline_migration(p, fx["vpe"], golden("small_mod")["records"])
+    ref = fx["image"]
+    assert np.abs(ref).max() > 0.1
+    assert np.array_equal(img.view(np.uint32), ref.view(np.uint32))
+
+
+def test_freq_tables_closed_form(oracle):
+    """w^2 and the DFT kernels against the closed form in double precision:
+    cos and sin of w*it, w = 2 pi (fmin + iw df) dt, df = 1/(nt dt),
+    nw = floor((fmax - fmin)/df)."""
+    nt, dt, fmin, fmax = 300, np.float32(9.142857e-4), np.float32(1.0), np.float32(30.0)
+    w2, kre, kim = oracle.freq_tables(nt, dt, fmin, fmax)
+    df = 1.0 / (nt * float(dt))
+    assert len(w2) == int((30.0 - 1.0) / df)
+    w = 2 * np.pi * (1.0 + np.arange(len(w2)) * df) * float(dt)
+    assert np.allclose(w2, w * w, rtol=1e-5)
+    t = np.arange(nt)
+    assert np.allclose(kre, np.cos(np.outer(w, t)), atol=2e-5)
+    assert np.allclose(kim, np.sin(np.outer(w, t)), atol=2e-5)
