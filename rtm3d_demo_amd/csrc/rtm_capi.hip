@@ -1116,13 +1116,14 @@ int rtmhip_batch_create(rtmhip_batch **out, const rtmhip_batch_shot *shots, int 
     const size_t bytes = sizeof(BatchShot) * (size_t)nshots;
     for (int k = 0; k < 2; ++k) {
         void *p = nullptr;
-        HIPCHK(hipMalloc(&p, bytes), "hipMalloc");
-        b->ds[k] = (BatchShot *)p;
-        hipError_t e = hipMemcpyAsync(p, b->hs[k].data(), bytes, hipMemcpyHostToDevice,
-                                      g_state.stream);
+        hipError_t e = hipMalloc(&p, bytes);
+        if (e == hipSuccess) {
+            b->ds[k] = (BatchShot *)p;
+            e = hipMemcpyAsync(p, b->hs[k].data(), bytes, hipMemcpyHostToDevice, g_state.stream);
+        }
         if (e != hipSuccess) {
             rtmhip_batch_destroy(b.release());
-            return hip_fail(e, "hipMemcpyAsync");
+            return hip_fail(e, "batch tables");
         }
     }
     void *p = nullptr;
