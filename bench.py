@@ -164,21 +164,24 @@ def cpu_reference_sample(workloads, steps, cores):
         shutil.rmtree(wd, ignore_errors=True)
 
 
-def accuracy_crop(hip, host, workloads, formats=("f32",)):
+def accuracy_crop(hip, host, workloads, formats=("f32",), crop=(80, 80, 64), nt=400):
     """Device results vs the FP64 oracle on a crop of the salt model
-    (80x80x64 + border 16, 400 steps): relative L2 of the receiver traces and
-    of the migrated HBC image, for each wavefield storage format (each
-    format models its own record and migrates it)."""
+    (default 80x80x64 + border 16, 400 steps): relative L2 of the receiver
+    traces and of the migrated HBC image, for each wavefield storage format
+    (each format models its own record and migrates it)."""
     sys.path.insert(0, os.path.join(REPO, "oracle"))
     import pyoracle as po
 
+    cx, cy, cz = crop
     full = workloads.salt_model()
-    vel = np.ascontiguousarray(full[216:296, 216:296, :64])
+    x0, y0 = 256 - cx // 2, 256 - cy // 2
+    vel = np.ascontiguousarray(full[x0:x0 + cx, y0:y0 + cy, :cz])
     del full
-    d, b, order, nt = 12.5, 16, 8, 400
-    p = workloads.params("crop", 80, 80, 64, d, b, nt, workloads.cfl_dt(4500.0, d), 25.0,
-                         (40, 40, 2), (0, 0, 0, 4, 4, 20, 20))
-    geom = hip.RcvGeom(0, 0, 0, 4, 4, 20, 20)
+    d, b, order = 12.5, 16, 8
+    nr = (cx // 4, cy // 4)
+    p = workloads.params("crop", cx, cy, cz, d, b, nt, workloads.cfl_dt(4500.0, d), 25.0,
+                         (cx // 2, cy // 2, 2), (0, 0, 0, 4, 4) + nr)
+    geom = hip.RcvGeom(0, 0, 0, 4, 4, *nr)
     coefs = host.stencil_coefs(order, d, d, d)
     src = host.ricker(nt, p["dt"], p["fpeak"])
     g = host.taper_table(b, p["taper_factor"])
@@ -191,9 +194,10 @@ def accuracy_crop(hip, host, workloads, formats=("f32",)):
     results = {}
     for fmt in formats:
         wdt = hip.WAVE_FORMATS[fmt][1]
-        shot = hip.make_shot(*shape, b, order, nt, nt, (40 + b, 40 + b, 2 + b), geom, fmt)
+        shot = hip.make_shot(*shape, b, order, nt, nt, (cx // 2 + b, cy // 2 + b, 2 + b), geom,
+                             fmt)
         F = hip.FieldPair(shape, wdt)
-        rcv = hip.DeviceArray((20, 20, nt)).zero()
+        rcv = hip.DeviceArray(nr + (nt,)).zero()
         hip.model_steps(shot, V, coefs, dg, dsrc, F, rcv, 0, nt)
         rec = rcv.download()
         S, R = hip.FieldPair(shape, wdt), hip.FieldPair(shape, wdt)
@@ -201,7 +205,7 @@ def accuracy_crop(hip, host, workloads, formats=("f32",)):
         img = hip.DeviceArray(shape).zero()
         hip.forward_steps(shot, True, Vh, coefs, dg, dsrc, S, upb, 0, nt)
         hip.backward_steps(shot, True, Vh, coefs, dg, S, R, upb, rcv, img, nt - 1, nt, True)
-        out = hip.DeviceArray((80, 80, 64))
+        out = hip.DeviceArray(crop)
         hip.image_finish(img, out, b, host.stencil_coefs(2, d, d, d))
         results[fmt] = (rec, out.download())
         for x in (F, rcv, S, R, upb, img, out):
@@ -215,7 +219,8 @@ def accuracy_crop(hip, host, workloads, formats=("f32",)):
     return {fmt: {"trace_rel_l2": rel(rec.astype(np.float64), rec64),
                   "image_rel_l2": rel(im.astype(np.float64), img64)}
             for fmt, (rec, im) in results.items()}, \
-        "80x80x64 crop of the salt model, border 16, 400 steps, 20x20 receivers"
+        "%dx%dx%d crop of the salt model, border 16, %d steps, %dx%d receivers" % (
+            cx, cy, cz, nt, nr[0], nr[1])
 
 
 class BatchShots:
@@ -483,7 +488,9 @@ def main():
         cores = min(16, os.cpu_count() or 1)
         cpu = None
         try:
-            acc, case = accuracy_crop(hip, host, workloads, tuple(formats))
+            # the mixed-precision configuration's accuracy case: a 128^3 crop,
+            # 4000 steps, against the FP64 oracle on the host cores
+            acc, case = accuracy_crop(hip, host, workloads, tuple(formats), (128, 128, 128), 4000)
             line["accuracy_vs_fp64"] = dict(acc["f32"], case=case)
             for fmt in formats[1:]:
                 line["wave_formats"][fmt].update(acc[fmt])
