@@ -453,6 +453,11 @@ def main():
     formats = ["f32"]
     if not args.no_precision:
         prec = {}
+        # 16-bit storage runs its own default shapes (the FP32 autotune's
+        # choices are for 4-byte lanes)
+        saved = [hip.get_tiling(c) for c in range(3)]
+        for c in range(3):
+            hip.set_tiling(c, 0)
         for fmt in ("f16", "bf16"):
             sh.migrate(fmt)
             hip.synchronize()
@@ -477,6 +482,8 @@ def main():
                          "algorithmic_bytes_per_launch": 10.0 * sh.U}
             log("%s shot: %.3f s (%.2fx FP32)" % (fmt, s16, shot_s / s16))
             formats.append(fmt)
+        for c, t in enumerate(saved):
+            hip.set_tiling(c, *t) if t[0] else hip.set_tiling(c, 0)
         line["wave_formats"] = prec
     for x in (sh.V, sh.src, sh.g, sh.F, sh.R, sh.rcv, sh.img, sh.upb):
         x.free()

@@ -629,9 +629,11 @@ __global__ void __launch_bounds__(64 * NWM * BY)
 // loads hit lines their neighbours fetch.
 // Minimum waves per SIMD asked of the compiler for the barrier-free kernel:
 // the register budget (512 / waves) it must fit.
-template <int RY, bool IMG, bool TAPER>
+template <typename T, int RY, bool IMG, bool TAPER>
 constexpr int wave_min_occupancy()
 {
+    // (every storage type: the tapered 8-row form spills a few bytes at 3
+    // waves and is still faster than at 2, FP32 and 16-bit alike)
     return RY == 8 ? (IMG ? 2 : 3) : (IMG || TAPER ? 3 : 4);
 }
 
@@ -875,7 +877,7 @@ __device__ __forceinline__ void st_at(char *base, unsigned off, U v)
 }
 
 template <typename T, int R, int RY, int WPB, bool IMG, bool TAPER>
-__global__ void __launch_bounds__(64 * WPB, (wave_min_occupancy<RY, IMG, TAPER>()))
+__global__ void __launch_bounds__(64 * WPB, (wave_min_occupancy<T, RY, IMG, TAPER>()))
     k_stencil_wave(const T *__restrict__ P, T *__restrict__ PP, const float *__restrict__ V,
                    const Coefs cf, const Grid g, const WaveGrid wg, const ImageFuse fz,
                    const TaperFuse tf, const float *__restrict__ gt, int *__restrict__ flag)
@@ -914,7 +916,7 @@ __global__ void __launch_bounds__(64 * WPB, (wave_min_occupancy<RY, IMG, TAPER>(
 // one taper table.  The block order is XCD-aware as above, so each XCD
 // marches a contiguous run of tiles.
 template <typename T, int R, int RY, int WPB, bool IMG, bool TAPER>
-__global__ void __launch_bounds__(64 * WPB, (wave_min_occupancy<RY, IMG, TAPER>()))
+__global__ void __launch_bounds__(64 * WPB, (wave_min_occupancy<T, RY, IMG, TAPER>()))
     k_stencil_batch(const BatchShot *__restrict__ shots, int nshots, long long nblocks, int par,
                     int pspar, const Coefs cf, const TaperFuse tf, int *__restrict__ flag)
 {
@@ -1772,9 +1774,10 @@ inline bool rows_fit(int nz, int by, int ry)
     return false;
 }
 
-// FP32 wavefields: every compiled shape.  16-bit wavefields: radius 4 with
-// the default shapes (full rows 4x4 and 2x6, tiles 64x4x4); any other
-// requested shape runs the default.
+// FP32 wavefields: every compiled shape.  16-bit wavefields: radius 4, the
+// barrier-free waves on request, else the defaults (imaging step on full
+// rows with a pair of z per lane where nz is even and at most 384, every
+// other step on 64x1x8 waves).
 template <typename T, int R>
 hipError_t launch_radius(const T *P, T *PP, const float *V, const Coefs &cf, const Grid &g,
                          const ImageFuse *fuse, const TaperFuse *taper, int *flag, hipStream_t s,
@@ -1788,8 +1791,8 @@ hipError_t launch_radius(const T *P, T *PP, const float *V, const Coefs &cf, con
             (g.nz + 127) / 128 <= 3 && (!taper || taper->b <= 4096))
             return launch_rows2<T, R, 4, 3, 3>(P, PP, V, cf, g, fuse, taper, flag, s, xchunk);
     }
-    if constexpr (R == 4 && std::is_same<T, float>::value) {
-        // barrier-free waves: 64 x 1 x ry
+    if constexpr (R == 4) {
+        // barrier-free waves: 64 x 1 x ry (every storage type)
         if (t && t->tz == 64 && t->by == 1 && (!taper || taper->b <= 8192)) {
             if (t->ry == 6)
                 return launch_wave<T, R, 6, 4>(P, PP, V, cf, g, fuse, taper, flag, s, xchunk);
@@ -1798,10 +1801,14 @@ hipError_t launch_radius(const T *P, T *PP, const float *V, const Coefs &cf, con
         }
     }
     if constexpr (R == 4 && !std::is_same<T, float>::value) {
-        // 16-bit wavefields: full rows with a pair of z per lane (32-bit
-        // accesses) whenever nz is even and at most 384
-        if ((g.nz & 1) == 0 && (g.nz + 127) / 128 <= 3 && (!taper || taper->b <= 4096))
+        // 16-bit wavefields by default: the imaging step on full rows with a
+        // pair of z per lane (32-bit accesses) whenever nz is even and at
+        // most 384; the plain and the tapered steps on 64 x 1 x 8 waves
+        // (salt grid, FP16: 0.317 / 0.540 ms against 0.517 / 0.563 on pairs)
+        if (fuse && (g.nz & 1) == 0 && (g.nz + 127) / 128 <= 3 && (!taper || taper->b <= 4096))
             return launch_rows2<T, R, 4, 3, 3>(P, PP, V, cf, g, fuse, taper, flag, s, xchunk);
+        if (!taper || taper->b <= 8192)
+            return launch_wave<T, R, 8, 4>(P, PP, V, cf, g, fuse, taper, flag, s, xchunk);
     }
     if constexpr (R == 4) {
         // full-row blocks by default whenever a compiled shape covers nz (and
@@ -1834,7 +1841,11 @@ hipError_t launch_radius(const T *P, T *PP, const float *V, const Coefs &cf, con
                 if (!taper || taper->b <= 8192)
                     return launch_wave<T, R, 8, 4>(P, PP, V, cf, g, fuse, taper, flag, s, xchunk);
             }
-            // 16-bit: a full-row shape that does not cover nz falls back to tiles
+            // a full-row shape that does not cover nz (16-bit): the same waves
+            if constexpr (!full) {
+                if (!taper || taper->b <= 8192)
+                    return launch_wave<T, R, 8, 4>(P, PP, V, cf, g, fuse, taper, flag, s, xchunk);
+            }
         }
     }
     if constexpr (full) {
