@@ -332,9 +332,9 @@ int rtmhip_ae_gemm_test(const void *A, long long sam, long long sak, int a_half,
  * tz = 0 restores the default: full-z-row blocks where they fit, else (FP32,
  * order 8) 64x1x8 waves, else 64x4x4; a full-row request that does not
  * cover nze takes the same fallback.  16-bit wavefields run the 64x1xry
- * waves on request and otherwise their defaults (the imaging step on full
- * rows with two z per lane where nze is even and at most 384, every other
- * step on 64x1x8 waves). */
+ * waves and the full rows 4x4 / 2x6 on request and otherwise their
+ * defaults (the imaging step on full rows 2x6, or two z per lane, where
+ * they cover nze; every other step on 64x1x8 waves). */
 int rtmhip_set_tiling(int kernel_class, int tz, int by, int ry, int xchunk);
 int rtmhip_get_tiling(int kernel_class, int *tz, int *by, int *ry, int *xchunk);
 

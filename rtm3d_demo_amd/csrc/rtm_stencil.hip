@@ -1775,9 +1775,9 @@ inline bool rows_fit(int nz, int by, int ry)
 }
 
 // FP32 wavefields: every compiled shape.  16-bit wavefields: radius 4, the
-// barrier-free waves on request, else the defaults (imaging step on full
-// rows with a pair of z per lane where nz is even and at most 384, every
-// other step on 64x1x8 waves).
+// barrier-free waves and the full rows 4x4 / 2x6 on request, else the
+// defaults (imaging step on full rows 2x6, or a pair of z per lane, where
+// they cover nz; every other step on 64x1x8 waves).
 template <typename T, int R>
 hipError_t launch_radius(const T *P, T *PP, const float *V, const Coefs &cf, const Grid &g,
                          const ImageFuse *fuse, const TaperFuse *taper, int *flag, hipStream_t s,
@@ -1801,12 +1801,26 @@ hipError_t launch_radius(const T *P, T *PP, const float *V, const Coefs &cf, con
         }
     }
     if constexpr (R == 4 && !std::is_same<T, float>::value) {
-        // 16-bit wavefields by default: the imaging step on full rows with a
-        // pair of z per lane (32-bit accesses) whenever nz is even and at
-        // most 384; the plain and the tapered steps on 64 x 1 x 8 waves
-        // (salt grid, FP16: 0.317 / 0.540 ms against 0.517 / 0.563 on pairs)
-        if (fuse && (g.nz & 1) == 0 && (g.nz + 127) / 128 <= 3 && (!taper || taper->b <= 4096))
-            return launch_rows2<T, R, 4, 3, 3>(P, PP, V, cf, g, fuse, taper, flag, s, xchunk);
+        // 16-bit wavefields: full rows (one z per lane) on request
+        if (t && t->tz == 1 && (!taper || taper->b <= 4096)) {
+            if (t->by == 4 && t->ry == 4 && rows_fit(g.nz, 4, 4))
+                return launch_rows<T, R, 4, 4, 4>(P, PP, V, cf, g, fuse, taper, flag, s, xchunk);
+            if (t->by == 2 && t->ry == 6 && rows_fit(g.nz, 2, 6))
+                return launch_rows<T, R, 2, 6, 6>(P, PP, V, cf, g, fuse, taper, flag, s, xchunk);
+        }
+        // 16-bit wavefields by default: the imaging step on full rows 2 x 6
+        // where they cover nz, else on full rows with a pair of z per lane
+        // (32-bit accesses) when nz is even and at most 384; the plain and
+        // the tapered steps on 64 x 1 x 8 waves (salt grid, FP16: imaging
+        // 0.743 ms on rows 2 x 6, 0.778 on pairs; plain / tapered 0.317 /
+        // 0.540 on waves, 0.517 / 0.563 on pairs)
+        if (fuse && (!taper || taper->b <= 4096)) {
+            if (rows_fit(g.nz, 2, 6))
+                return launch_rows<T, R, 2, 6, 6>(P, PP, V, cf, g, fuse, taper, flag, s, xchunk);
+            if ((g.nz & 1) == 0 && (g.nz + 127) / 128 <= 3)
+                return launch_rows2<T, R, 4, 3, 3>(P, PP, V, cf, g, fuse, taper, flag, s,
+                                                   xchunk);
+        }
         if (!taper || taper->b <= 8192)
             return launch_wave<T, R, 8, 4>(P, PP, V, cf, g, fuse, taper, flag, s, xchunk);
     }
