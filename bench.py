@@ -15,10 +15,14 @@ dominant kernel (the stencil step: 16 algorithmic bytes per updated cell,
 durations from HIP events around every launch in the timed region), and the
 reference CPU program timed on a bounded sample of the same workload on this
 host's cores.  Beside the FP32 line: the same shot with FP16 and with BF16
-wavefield storage (FP32 arithmetic; 10 algorithmic bytes per stencil cell),
-and every format's trace / image error against the FP64 oracle on a crop.
+wavefield storage (FP32 arithmetic; 10 algorithmic bytes per stencil cell);
+every format's trace / image error against the FP64 oracle on a 128^3 crop
+over 4000 steps; and a batched leg (8 shots with uneven apertures cut from a
+1024x1024x320 salt model, 2500 steps, batched against one at a time, images
+compared bit for bit).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+       [--no-cpu] [--no-precision] [--no-tune] [--batch-shots N] [--batch-nt T]
 """
 import argparse
 import json
