@@ -333,8 +333,7 @@ int rtmhip_ae_gemm_test(const void *A, long long sam, long long sak, int a_half,
  * order 8) 64x1x8 waves, else 64x4x4; a full-row request that does not
  * cover nze takes the same fallback.  16-bit wavefields run the 64x1xry
  * waves and the full rows 4x4 / 2x6 on request and otherwise their
- * defaults (the imaging step on full rows 2x6, or two z per lane, where
- * they cover nze; every other step on 64x1x8 waves). */
+ * defaults (64x1x6 waves for the imaging step, 64x1x8 waves otherwise). */
 int rtmhip_set_tiling(int kernel_class, int tz, int by, int ry, int xchunk);
 int rtmhip_get_tiling(int kernel_class, int *tz, int *by, int *ry, int *xchunk);
 

@@ -640,6 +640,7 @@ constexpr int wave_min_occupancy()
 struct WaveGrid {
     int nzt, nyt, nxc, xchunk;
     long long waves;  // nzt * nyt * nxc
+    int zt0;          // first z tile of the launch (its nzt tiles follow)
 };
 
 // One wave's tile of the barrier-free stencil: z tile zt (64 lanes), rows
@@ -901,7 +902,7 @@ __global__ void __launch_bounds__(64 * WPB, (wave_min_occupancy<T, RY, IMG, TAPE
         w = t * WPB + wid;
     }
     if (w >= wg.waves) return;
-    const int zt = (int)(w % wg.nzt);
+    const int zt = wg.zt0 + (int)(w % wg.nzt);
     const long long w2 = w / wg.nzt;
     const int yt = (int)(w2 % wg.nyt);
     const int xc = (int)(w2 / wg.nyt);
@@ -1300,7 +1301,7 @@ __global__ void __launch_bounds__(64 * WPB)
         w = t * WPB + wid;
     }
     if (w >= wg.waves) return;
-    const int zt = (int)(w % wg.nzt);
+    const int zt = wg.zt0 + (int)(w % wg.nzt);
     const long long w2 = w / wg.nzt;
     const int yt = (int)(w2 % wg.nyt);
     const int xc = (int)(w2 / wg.nyt);
@@ -1723,14 +1724,17 @@ hipError_t launch_rows2(const T *P, T *PP, const float *V, const Coefs &cf, cons
     });
 }
 
+// One launch over the z tiles [zt0, zt0 + nzt).
 template <typename T, int R, int RY, int WPB, bool IMG, bool TAPER>
 hipError_t launch_wave_cfg(const T *P, T *PP, const float *V, const Coefs &cf, const Grid &g,
                            const ImageFuse &fz, const TaperFuse &tf, int *flag, hipStream_t s,
-                           int xchunk)
+                           int xchunk, int zt0, int nzt)
 {
     auto kern = k_stencil_wave<T, R, RY, WPB, IMG, TAPER>;
     WaveGrid wg;
-    wg.nzt = (g.nz + 63) / 64;
+    wg.zt0 = zt0;
+    wg.nzt = nzt;
+    if (nzt <= 0) return hipSuccess;
     wg.nyt = (g.ny - 2 * R + RY - 1) / RY;
     const int nxi = g.nx - 2 * R;
     if (wg.nyt <= 0 || nxi <= 0 || g.nz - 2 * R <= 0) return hipSuccess;
@@ -1757,9 +1761,22 @@ hipError_t launch_wave(const T *P, T *PP, const float *V, const Coefs &cf, const
     ImageFuse fz;
     TaperFuse tf;
     fuse_args(fuse, taper, fz, tf);
-    return by_variant(fuse, taper, [&](auto img, auto tap) {
+    const int nzt = (g.nz + 63) / 64;
+    // The upper taper touches only z < b: the z tiles whose cells and z halo
+    // all lie at z >= b (zt * 64 - R >= b) run the untapered variant, with
+    // fewer registers, in a second launch over disjoint cells.  16-bit only
+    // (salt grid, FP16: tapered step 0.540 -> 0.398 ms, imaging step 0.83
+    // -> 0.69); for FP32 the second launch costs more than it saves.
+    const bool split = taper && !std::is_same<T, float>::value;
+    const int ntap = split ? std::min(nzt, (taper->b + R + 63) / 64) : nzt;
+    hipError_t e = by_variant(fuse, taper, [&](auto img, auto tap) {
         return launch_wave_cfg<T, R, RY, WPB, decltype(img)::value, decltype(tap)::value>(
-            P, PP, V, cf, g, fz, tf, flag, s, xchunk);
+            P, PP, V, cf, g, fz, tf, flag, s, xchunk, 0, ntap);
+    });
+    if (e != hipSuccess || ntap >= nzt) return e;
+    return by_variant(fuse, nullptr, [&](auto img, auto tap) {
+        return launch_wave_cfg<T, R, RY, WPB, decltype(img)::value, decltype(tap)::value>(
+            P, PP, V, cf, g, fz, tf, flag, s, xchunk, ntap, nzt - ntap);
     });
 }
 
@@ -1776,8 +1793,8 @@ inline bool rows_fit(int nz, int by, int ry)
 
 // FP32 wavefields: every compiled shape.  16-bit wavefields: radius 4, the
 // barrier-free waves and the full rows 4x4 / 2x6 on request, else the
-// defaults (imaging step on full rows 2x6, or a pair of z per lane, where
-// they cover nz; every other step on 64x1x8 waves).
+// defaults (barrier-free waves: 64x1x6 for the imaging step, 64x1x8
+// otherwise).
 template <typename T, int R>
 hipError_t launch_radius(const T *P, T *PP, const float *V, const Coefs &cf, const Grid &g,
                          const ImageFuse *fuse, const TaperFuse *taper, int *flag, hipStream_t s,
@@ -1808,21 +1825,14 @@ hipError_t launch_radius(const T *P, T *PP, const float *V, const Coefs &cf, con
             if (t->by == 2 && t->ry == 6 && rows_fit(g.nz, 2, 6))
                 return launch_rows<T, R, 2, 6, 6>(P, PP, V, cf, g, fuse, taper, flag, s, xchunk);
         }
-        // 16-bit wavefields by default: the imaging step on full rows 2 x 6
-        // where they cover nz, else on full rows with a pair of z per lane
-        // (32-bit accesses) when nz is even and at most 384; the plain and
-        // the tapered steps on 64 x 1 x 8 waves (salt grid, FP16: imaging
-        // 0.743 ms on rows 2 x 6, 0.778 on pairs; plain / tapered 0.317 /
-        // 0.540 on waves, 0.517 / 0.563 on pairs)
-        if (fuse && (!taper || taper->b <= 4096)) {
-            if (rows_fit(g.nz, 2, 6))
-                return launch_rows<T, R, 2, 6, 6>(P, PP, V, cf, g, fuse, taper, flag, s, xchunk);
-            if ((g.nz & 1) == 0 && (g.nz + 127) / 128 <= 3)
-                return launch_rows2<T, R, 4, 3, 3>(P, PP, V, cf, g, fuse, taper, flag, s,
-                                                   xchunk);
-        }
-        if (!taper || taper->b <= 8192)
+        // 16-bit wavefields by default: barrier-free waves, 64 x 1 x 6 for the
+        // imaging step and 64 x 1 x 8 otherwise (salt grid, FP16: imaging
+        // 0.672 ms, plain 0.317, tapered 0.398; full rows 2 x 6: 0.744 /
+        // 0.546 / 0.562; pairs: 0.778 / 0.517 / 0.563)
+        if (!taper || taper->b <= 8192) {
+            if (fuse) return launch_wave<T, R, 6, 4>(P, PP, V, cf, g, fuse, taper, flag, s, xchunk);
             return launch_wave<T, R, 8, 4>(P, PP, V, cf, g, fuse, taper, flag, s, xchunk);
+        }
     }
     if constexpr (R == 4) {
         // full-row blocks by default whenever a compiled shape covers nz (and
@@ -1905,6 +1915,7 @@ hipError_t launch_dual_cfg(const T *Ps, T *PPs, const T *Pr, T *PPr, const float
 {
     auto kern = k_stencil_dual<T, R, RY, WPB, TAPER>;
     WaveGrid wg;
+    wg.zt0 = 0;
     wg.nzt = (g.nz + 63) / 64;
     wg.nyt = (g.ny - 2 * R + RY - 1) / RY;
     const int nxi = g.nx - 2 * R;
