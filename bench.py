@@ -329,8 +329,42 @@ def batch_leg(hip, host, workloads, nshots, nt, tune=True):
     log("batch leg: %d shots batched: %.2f s" % (nshots, t_batch))
     same = all(np.array_equal(w.view(np.uint32), m["img"].download().view(np.uint32))
                for w, m in zip(want, bs.members))
-    del want
     batch.destroy()
+    # the same batch with FP16 wavefields and UPB history (FP32 arithmetic,
+    # velocity, records and images), default 16-bit shapes, image error
+    # against the FP32 images
+    saved = [hip.get_tiling(c) for c in range(3)]
+    for c in range(3):
+        hip.set_tiling(c, 0)
+    f16 = []
+    for m in bs.members:
+        shp = (m["shot"].nxe, m["shot"].nye, m["shot"].nze)
+        q = dict(m)
+        q["shot"] = hip.make_shot(*shp, bs.b, 8, nt, nt, (m["shot"].sxe, m["shot"].sye,
+                                                         m["shot"].sze), m["shot"].rcv, "f16")
+        q["S"], q["R"] = hip.FieldPair(shp, np.uint16), hip.FieldPair(shp, np.uint16)
+        q["upb"] = hip.DeviceArray(m["upb"].shape, np.uint16)
+        q["img"] = m["img"]
+        m["img"].zero()
+        f16.append(q)
+    batch = hip.Batch(f16, bs.coefs, bs.g)
+    hip.synchronize()
+    t = time.perf_counter()
+    batch.forward_steps(0, nt)
+    batch.backward_steps(nt - 1, nt)
+    hip.synchronize()
+    t_f16 = time.perf_counter() - t
+    batch.destroy()
+    log("batch leg: %d shots batched, FP16 wavefields: %.2f s" % (nshots, t_f16))
+    err16 = float(np.sqrt(sum(float(np.sum((q["img"].download().astype(np.float64) - w) ** 2))
+                              for q, w in zip(f16, want)) /
+                          sum(float(np.sum(w.astype(np.float64) ** 2)) for w in want)))
+    for q in f16:
+        for k in ("S", "R", "upb"):
+            q[k].free()
+    for c, tl in enumerate(saved):
+        hip.set_tiling(c, *tl) if tl[0] else hip.set_tiling(c, 0)
+    del want
     updates = bs.U * (3 * nt - 2)
     out = {
         "workload": "salt1024_batch%d_nt%d" % (nshots, nt),
@@ -345,6 +379,10 @@ def batch_leg(hip, host, workloads, nshots, nt, tune=True):
         "single_shots_per_hour": round(3600.0 * nshots / t_single, 1),
         "speedup": round(t_single / t_batch, 3),
         "images_bit_identical": bool(same),
+        "batched_f16_s": round(t_f16, 3),
+        "batched_f16_shots_per_hour": round(3600.0 * nshots / t_f16, 1),
+        "f16_speedup_vs_single_f32": round(t_single / t_f16, 3),
+        "f16_image_rel_l2_vs_f32": round(err16, 6),
         "autotune": {str(c): v for c, v in sorted(modes.items())} if modes else None,
         "graph_steps": 8,
     }
