@@ -7,7 +7,7 @@ reconstruction (rtm-3d/src/kernel/RTMFiniteDifferencesKernel_HBC.cpp:212-365)
 or as exact snapshots.  Here a snapshot of the imaging region (all x, y;
 z in [blen, nze - blen)) is cut into 32^3 bricks and each brick is stored as
 2048 FP16 codes plus one FP32 scale (32x smaller than the FP32 snapshot).
-Exact FP32 snapshots remain the fallback and the judge of the image error.
+Exact FP32 snapshots remain the fallback and the yardstick of the image error.
 """
 import ctypes as C
 

@@ -9,7 +9,7 @@
   held-out shot: the trained network against the same storage without
   learning (2x2x4 box decimation in FP16) and against the untrained
   network, for the snapshots and for the snapshot-RTM image, with exact
-  FP32 snapshots as the judge.
+  FP32 snapshots as the yardstick.
 """
 import numpy as np
 import pytest
