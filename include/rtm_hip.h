@@ -328,12 +328,15 @@ int rtmhip_ae_gemm_test(const void *A, long long sam, long long sak, int a_half,
  *   tz = 32: tiles 32x8x4, 32x8x2;
  *   tz = 1 (order 8): full-z-row blocks 4x4 (nze <= 256), 2x6 and 2x7
  *           (nze <= 384);
- *   tz = 2 (order 8, even nze <= 384): full rows, two z per lane, 4x3.
+ *   tz = 2 (order 8, even nze <= 384): full rows, two z per lane, 4x3;
+ *   tz = 128 (order 8): barrier-free waves, two z per lane, 128x1x4 (odd
+ *           nze or strides: 64x1x8 waves).
  * tz = 0 restores the default: full-z-row blocks where they fit, else (FP32,
  * order 8) 64x1x8 waves, else 64x4x4; a full-row request that does not
  * cover nze takes the same fallback.  16-bit wavefields run the 64x1xry
- * waves and the full rows 4x4 / 2x6 on request and otherwise their
- * defaults (64x1x6 waves for the imaging step, 64x1x8 waves otherwise). */
+ * and 128x1x4 waves and the full rows 4x4 / 2x6 on request and otherwise
+ * their defaults (64x1x6 waves for the imaging step, 128x1x4 waves for the
+ * plain step where nze and the strides are even, 64x1x8 waves otherwise). */
 int rtmhip_set_tiling(int kernel_class, int tz, int by, int ry, int xchunk);
 int rtmhip_get_tiling(int kernel_class, int *tz, int *by, int *ry, int *xchunk);
 

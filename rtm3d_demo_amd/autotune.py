@@ -47,7 +47,7 @@ def tune(hip, shot, V, coefs, g, src, F, R, upb, rcv, img, steps=6, rounds=2, lo
     return best
 
 
-BATCH_SHOT_TILINGS = [(1, 4, 4), (1, 2, 6), (1, 2, 7), (2, 4, 3), (64, 1, 8)]
+BATCH_SHOT_TILINGS = [(1, 4, 4), (1, 2, 6), (1, 2, 7), (2, 4, 3), (64, 1, 8), (128, 1, 4)]
 
 
 def tune_batch(hip, batch, nt, steps=8, log=None):

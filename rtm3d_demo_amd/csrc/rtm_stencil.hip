@@ -35,6 +35,7 @@
 #include "rtm_internal.hpp"
 
 #include <algorithm>
+#include <cstdint>
 #include <map>
 #include <mutex>
 #include <type_traits>
@@ -911,6 +912,339 @@ __global__ void __launch_bounds__(64 * WPB, (wave_min_occupancy<T, RY, IMG, TAPE
     wave_tile<T, R, RY, IMG, TAPER>(P, PP, V, cf, g, zt, yt, xb, xe, fz, tf, tab, tile[wid], flag);
 }
 
+// ---------------------------------------------------------------------------
+// Barrier-free waves with two z cells per lane.  A wave's tile is 128 z cells
+// x RY rows: lane l owns z0 + 2l and z0 + 2l + 1, so every row access moves
+// a pair (one 32-bit word of two 16-bit values, or 8 bytes for FP32) and the
+// z taps of both cells come from five 8-byte LDS reads.  Tap order, fusions
+// and rounding are those of wave_tile.  Needs even R, even nz and even
+// strides: a pair is then aligned and either updated whole or not at all.
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+
+template <typename T>
+__device__ __forceinline__ void ld_buf2(__amdgpu_buffer_rsrc_t rs, unsigned soff, unsigned voff,
+                                        float (&v)[2])
+{
+    if constexpr (sizeof(T) == 4) {
+        const u32x2 u = __builtin_amdgcn_raw_buffer_load_b64(rs, (int)voff, (int)soff, 0);
+        // Copy the elements out first: this compiler's __builtin_bit_cast of
+        // a vector-element expression (u.y) reads element 0.
+        const unsigned lo = u.x, hi = u.y;
+        v[0] = __builtin_bit_cast(float, lo);
+        v[1] = __builtin_bit_cast(float, hi);
+    } else {
+        const unsigned u = __builtin_amdgcn_raw_buffer_load_b32(rs, (int)voff, (int)soff, 0);
+        v[0] = wld(__builtin_bit_cast(T, (unsigned short)(u & 0xffffu)));
+        v[1] = wld(__builtin_bit_cast(T, (unsigned short)(u >> 16)));
+    }
+}
+template <typename T>
+__device__ __forceinline__ void st_buf2(__amdgpu_buffer_rsrc_t rs, unsigned soff, unsigned voff,
+                                        float a, float b)
+{
+    if constexpr (sizeof(T) == 4) {
+        u32x2 u;
+        u.x = __builtin_bit_cast(unsigned, a);
+        u.y = __builtin_bit_cast(unsigned, b);
+        __builtin_amdgcn_raw_buffer_store_b64(u, rs, (int)voff, (int)soff, 0);
+    } else {
+        const unsigned lo = __builtin_bit_cast(unsigned short, wst<T>(a));
+        const unsigned hi = __builtin_bit_cast(unsigned short, wst<T>(b));
+        __builtin_amdgcn_raw_buffer_store_b32(lo | (hi << 16), rs, (int)voff, (int)soff, 0);
+    }
+}
+
+template <bool IMG>
+constexpr int wave2_min_occupancy()
+{
+    return IMG ? 2 : 3;
+}
+
+template <typename T, int R, int RY, bool IMG, bool TAPER>
+__device__ __forceinline__ void wave_tile2(const T *__restrict__ P, T *__restrict__ PP,
+                                           const float *__restrict__ V, const Coefs &cf,
+                                           const Grid &g, int zt, int yt, int xb, int xe,
+                                           const ImageFuse &fz, const TaperFuse &tf,
+                                           const float *tab, float (*lt)[128 + 2 * R],
+                                           int *__restrict__ flag)
+{
+    constexpr int NR = 2 * R + 2;
+    constexpr int NZH = 2 * R;
+    static_assert(R % 2 == 0, "cell pairs are updated whole");
+    static_assert(NZH * RY <= 64, "one z-halo load per lane");
+    constexpr unsigned ES = sizeof(T);
+    constexpr unsigned VS = 4u / sizeof(T);
+    const T *__restrict__ PS = static_cast<const T *>(fz.ps);
+    const int lane = threadIdx.x;
+    const int z0 = zt * 128, y0 = R + yt * RY;
+    const int z = z0 + 2 * lane;  // the lane's first cell
+    // nz even: nz - 2 starts the last in-grid pair
+    const unsigned zc = (unsigned)min(z, g.nz - 2);
+    const unsigned zoff = zc * ES, zoffv = zc * 4u;
+    // with z, R and nz even this holds for both cells or for neither
+    const bool zcomp = (z >= R) && (z < g.nz - R);
+
+    auto lat = [&](int i, int n) -> float {
+        const int j = i < tf.b ? tf.b - 1 - i : (i >= n - tf.b ? i - (n - tf.b) : -1);
+        return j >= 0 ? tab[max(j, 0)] : 1.0f;
+    };
+    const bool wtap = TAPER && (z0 - R < tf.b);
+    bool ztap[2];
+    float fzl[2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a) {
+        ztap[a] = wtap && (z + a < tf.b);
+        fzl[a] = 1.0f;
+        if (ztap[a]) fzl[a] = tab[tf.b - 1 - (z + a)];
+    }
+
+    unsigned rowb[RY];
+    unsigned ymask = 0, rmask = 0;
+    float fyr[RY];
+#pragma unroll
+    for (int r = 0; r < RY; ++r) {
+        const int y = y0 + r;
+        const int yc = min(y, g.ny - 1);
+        rowb[r] = (unsigned)yc * (unsigned)g.sy * ES;
+        if (y < g.ny - R) ymask |= 1u << r;
+        if (IMG) {
+            const int dy = y - fz.ry0;
+            if (dy >= 0 && (dy % fz.rdy) == 0 && (dy / fz.rdy) < fz.rny) rmask |= 1u << r;
+        }
+        fyr[r] = 1.0f;
+        if (wtap) fyr[r] = __builtin_bit_cast(
+            float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, lat(yc, g.ny))));
+    }
+    unsigned yhb[2 * R];
+    float yhf[2 * R];
+#pragma unroll
+    for (int h = 0; h < 2 * R; ++h) {
+        const int gy = min(h < R ? y0 - R + h : y0 + RY + (h - R), g.ny - 1);
+        yhb[h] = (unsigned)gy * (unsigned)g.sy * ES;
+        yhf[h] = 1.0f;
+        if (wtap) yhf[h] = __builtin_bit_cast(
+            float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, lat(gy, g.ny))));
+    }
+    // this lane's z-halo cell (one 16- or 32-bit load): row zr, LDS column zcol
+    const bool zh_on = lane < NZH * RY;
+    const int zr = min(lane / NZH, RY - 1), zk = lane % NZH;
+    const int zhz = zk < R ? z0 - R + zk : z0 + 128 + (zk - R);
+    const int zhzc = min(max(zhz, 0), g.nz - 1);
+    const int zcol = zk < R ? zk : 128 + zk;
+    const int zhy = min(y0 + zr, g.ny - 1);
+    const unsigned zhoff = ((unsigned)zhy * (unsigned)g.sy + (unsigned)zhzc) * ES;
+    const bool zhtap = wtap && zhzc < tf.b;
+    float zhfy = 1.0f, zhfz = 1.0f;
+    if (zhtap) {
+        zhfy = lat(zhy, g.ny);
+        zhfz = tab[tf.b - 1 - zhzc];
+    }
+
+    float q[NR][RY][2];
+    float yh[2][2 * R][2], zh[2];
+
+    auto plane_fx = [&](int x) -> float {
+        return wtap ? __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(
+                                                    __builtin_bit_cast(int, lat(x, g.nx))))
+                    : 1.0f;
+    };
+    auto tap = [&](float v, int a, float fx, float fy) -> float {
+        return wrnd<T>(taper1(v, ztap[a] ? fx : 1.0f, ztap[a] ? fy : 1.0f, fzl[a]));
+    };
+    auto load_ring = [&](int x, float(&dst)[RY][2]) {
+        const auto rs = plane_rsrc(P + (size_t)x * (size_t)g.sx);
+#pragma unroll
+        for (int r = 0; r < RY; ++r) ld_buf2<T>(rs, rowb[r], zoff, dst[r]);
+    };
+    auto taper_ring = [&](int x, float(&dst)[RY][2]) {
+        if (TAPER && wtap) {
+            const float fx = plane_fx(x);
+#pragma unroll
+            for (int r = 0; r < RY; ++r)
+#pragma unroll
+                for (int a = 0; a < 2; ++a) dst[r][a] = tap(dst[r][a], a, fx, fyr[r]);
+        }
+    };
+    auto fetch_halo = [&](int x, int set) {
+        const auto rs = plane_rsrc(P + (size_t)x * (size_t)g.sx);
+#pragma unroll
+        for (int h = 0; h < 2 * R; ++h) ld_buf2<T>(rs, yhb[h], zoff, yh[set][h]);
+        zh[set] = ld_buf<T>(rs, 0u, zhoff);
+    };
+    auto taper_halo = [&](int x, int set) {
+        if (TAPER && wtap) {
+            const float fx = plane_fx(x);
+#pragma unroll
+            for (int h = 0; h < 2 * R; ++h)
+#pragma unroll
+                for (int a = 0; a < 2; ++a) yh[set][h][a] = tap(yh[set][h][a], a, fx, yhf[h]);
+            zh[set] = wrnd<T>(taper1(zh[set], zhtap ? fx : 1.0f, zhfy, zhfz));
+        }
+    };
+
+#pragma unroll
+    for (int s = 0; s <= 2 * R; ++s) load_ring(xb - R + s, q[s]);
+    fetch_halo(xb, 0);
+#pragma unroll
+    for (int s = 0; s < 2 * R; ++s) taper_ring(xb - R + s, q[s]);
+
+    bool bad = false;
+    int x = xb;
+    auto step = [&]<int K>(std::integral_constant<int, K>) -> bool {
+        if (x >= xe) return false;
+        constexpr int cur = K & 1;
+        constexpr int ctr = (K + R) % NR;
+        taper_ring(x + R, q[(K + 2 * R) % NR]);
+        taper_halo(x, cur);
+        const size_t pl = (size_t)x * (size_t)g.sx;
+        const auto rpp = plane_rsrc(PP + pl);
+        const auto rv = plane_rsrc(V + pl);
+        const auto rps = plane_rsrc(IMG ? (const void *)(PS + pl) : (const void *)PP);
+        const auto rimg = plane_rsrc(IMG ? (const void *)(fz.img + pl) : (const void *)PP);
+        float c_pp[RY][2], c_v[RY][2], c_ps[RY][2], c_img[RY][2];
+#pragma unroll
+        for (int r = 0; r < RY; ++r) {
+            ld_buf2<T>(rpp, rowb[r], zoff, c_pp[r]);
+            ld_buf2<float>(rv, rowb[r] * VS, zoffv, c_v[r]);
+        }
+        if (IMG) {
+#pragma unroll
+            for (int r = 0; r < RY; ++r) {
+                ld_buf2<T>(rps, rowb[r], zoff, c_ps[r]);
+                ld_buf2<float>(rimg, rowb[r] * VS, zoffv, c_img[r]);
+            }
+        }
+        // stage the centre plane's rows and z halo in this wave's LDS rows
+        __builtin_amdgcn_wave_barrier();
+#pragma unroll
+        for (int r = 0; r < RY; ++r)
+            *reinterpret_cast<float2 *>(&lt[r][R + 2 * lane]) =
+                make_float2(q[ctr][r][0], q[ctr][r][1]);
+        if (zh_on) lt[zr][zcol] = zh[cur];
+        __builtin_amdgcn_wave_barrier();
+        float yc[2 * R][2];
+#pragma unroll
+        for (int h = 0; h < 2 * R; ++h) {
+            yc[h][0] = yh[cur][h][0];
+            yc[h][1] = yh[cur][h][1];
+        }
+        if (x + 1 < xe) {
+            load_ring(x + 1 + R, q[(K + 2 * R + 1) % NR]);
+            fetch_halo(x + 1, cur ^ 1);
+        }
+        bool rcvx = false;
+        if (IMG) {
+            const int dx = x - fz.rx0;
+            rcvx = dx >= 0 && (dx % fz.rdx) == 0 && (dx / fz.rdx) < fz.rnx;
+        }
+        float fxc = 1.0f;
+        if (TAPER) fxc = plane_fx(x);
+        float lapl[RY][2];
+#pragma unroll
+        for (int r = 0; r < RY; ++r) {
+            // LDS row entries 2l .. 2l + 2R + 1: cells z0 - R + 2l .. z0 + R + 2l + 1
+            float w[2 * R + 2];
+            const float2 *t2 = reinterpret_cast<const float2 *>(&lt[r][2 * lane]);
+#pragma unroll
+            for (int k = 0; k <= R; ++k) {
+                const float2 u = t2[k];
+                w[2 * k] = u.x;
+                w[2 * k + 1] = u.y;
+            }
+#pragma unroll
+            for (int a = 0; a < 2; ++a) {
+                const float p = q[ctr][r][a];
+                float ax = 0.f, ay = 0.f, az = 0.f;
+#pragma unroll
+                for (int io = 0; io <= 2 * R; ++io) {
+                    const int ry = r + io - R;  // compile-time row of the y tap
+                    const float vy = ry < 0 ? yc[ry + R][a]
+                                            : (ry >= RY ? yc[R + ry - RY][a] : q[ctr][ry][a]);
+                    const float vz = (io == R) ? p : w[io + a];
+                    ax += q[(K + io) % NR][r][a] * cf.c[0][io];
+                    ay += vy * cf.c[1][io];
+                    az += vz * cf.c[2][io];
+                }
+                lapl[r][a] = ax + ay + az;
+                asm volatile("" : "+v"(lapl[r][a]));
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < RY; ++r) {
+            float npp[2];
+#pragma unroll
+            for (int a = 0; a < 2; ++a) {
+                float pp = c_pp[r][a];
+                if (TAPER && wtap) {
+                    pp = tap(pp, a, fxc, fyr[r]);
+                    if (tf.pp_taps > 1) pp = tap(pp, a, fxc, fyr[r]);
+                }
+                npp[a] = 2.f * q[ctr][r][a] - pp + c_v[r][a] * lapl[r][a];
+            }
+            if (((ymask >> r) & 1u) && zcomp) {
+                st_buf2<T>(rpp, rowb[r], zoff, npp[0], npp[1]);
+                bad |= (npp[0] != npp[0]) || (npp[1] != npp[1]);
+                if (IMG) {
+                    bool im[2];
+                    float iv[2];
+#pragma unroll
+                    for (int a = 0; a < 2; ++a) {
+                        const int za = z + a;
+                        im[a] = za >= fz.z0 && za < fz.z1 &&
+                                !(rcvx && ((rmask >> r) & 1u) && za == fz.rz);
+                        iv[a] = c_img[r][a] + c_ps[r][a] * wrnd<T>(npp[a]);
+                    }
+                    if (im[0] && im[1]) st_buf2<float>(rimg, rowb[r] * VS, zoffv, iv[0], iv[1]);
+                    else if (im[0]) st_buf<float>(rimg, rowb[r] * VS, zoffv, iv[0]);
+                    else if (im[1]) st_buf<float>(rimg, rowb[r] * VS, zoffv + 4u, iv[1]);
+                }
+            }
+        }
+        ++x;
+        return true;
+    };
+    while (x < xe) {
+        [&]<int... Ks>(std::integer_sequence<int, Ks...>) {
+            (step(std::integral_constant<int, Ks>{}) && ...);
+        }(std::make_integer_sequence<int, NR>{});
+    }
+    if (bad) atomicOr(flag, 1);
+}
+
+template <typename T, int R, int RY, int WPB, bool IMG, bool TAPER>
+__global__ void __launch_bounds__(64 * WPB, (wave2_min_occupancy<IMG>()))
+    k_stencil_wave2(const T *__restrict__ P, T *__restrict__ PP, const float *__restrict__ V,
+                    const Coefs cf, const Grid g, const WaveGrid wg, const ImageFuse fz,
+                    const TaperFuse tf, const float *__restrict__ gt, int *__restrict__ flag)
+{
+    constexpr int LW = 128 + 2 * R;  // LDS row: z0 - R .. z0 + 127 + R
+    __shared__ __align__(8) float tile[WPB][RY][LW];
+    extern __shared__ float tab[];
+    const int wid = __builtin_amdgcn_readfirstlane(threadIdx.y);
+    if (TAPER) {
+        const int tid = threadIdx.y * 64 + threadIdx.x;
+        for (int i = tid; i < tf.b; i += 64 * WPB) tab[i] = gt[i];
+        __syncthreads();
+    }
+    long long w;
+    {
+        const long long nb = (wg.waves + WPB - 1) / WPB;
+        const long long h = blockIdx.x;
+        const long long q = nb >> 3, rr = nb & 7, k = h & 7, j = h >> 3;
+        const long long t = k * q + (k < rr ? k : rr) + j;
+        w = t * WPB + wid;
+    }
+    if (w >= wg.waves) return;
+    const int zt = wg.zt0 + (int)(w % wg.nzt);
+    const long long w2 = w / wg.nzt;
+    const int yt = (int)(w2 % wg.nyt);
+    const int xc = (int)(w2 / wg.nyt);
+    const int xb = R + xc * wg.xchunk;
+    const int xe = min(xb + wg.xchunk, g.nx - R);
+    wave_tile2<T, R, RY, IMG, TAPER>(P, PP, V, cf, g, zt, yt, xb, xe, fz, tf, tab, tile[wid],
+                                     flag);
+}
+
 // Batched form: the waves of every shot of a batch in one launch.  Each
 // shot's waves fill whole blocks (blocks never straddle shots), so a block
 // finds its shot by a binary search over the shots' first blocks and keeps
@@ -1780,6 +2114,70 @@ hipError_t launch_wave(const T *P, T *PP, const float *V, const Coefs &cf, const
     });
 }
 
+// Two-cells-per-lane waves, one launch over the z tiles [zt0, zt0 + nzt) of
+// 128 cells.
+template <typename T, int R, int RY, int WPB, bool IMG, bool TAPER>
+hipError_t launch_wave2_cfg(const T *P, T *PP, const float *V, const Coefs &cf, const Grid &g,
+                            const ImageFuse &fz, const TaperFuse &tf, int *flag, hipStream_t s,
+                            int xchunk, int zt0, int nzt)
+{
+    auto kern = k_stencil_wave2<T, R, RY, WPB, IMG, TAPER>;
+    WaveGrid wg;
+    wg.zt0 = zt0;
+    wg.nzt = nzt;
+    if (nzt <= 0) return hipSuccess;
+    wg.nyt = (g.ny - 2 * R + RY - 1) / RY;
+    const int nxi = g.nx - 2 * R;
+    if (wg.nyt <= 0 || nxi <= 0 || g.nz - 2 * R <= 0) return hipSuccess;
+    const size_t lds = TAPER ? (size_t)tf.b * sizeof(float) : 0;
+    if (xchunk <= 0) {
+        const int cap = device_capacity((const void *)kern, 64 * WPB, lds) * WPB;
+        const int n = choose_chunks((long long)wg.nzt * wg.nyt, nxi, R, cap);
+        xchunk = (nxi + n - 1) / n;
+    }
+    wg.xchunk = xchunk;
+    wg.nxc = (nxi + xchunk - 1) / xchunk;
+    wg.waves = (long long)wg.nzt * wg.nyt * wg.nxc;
+    const long long nb = (wg.waves + WPB - 1) / WPB;
+    hipLaunchKernelGGL(kern, dim3((unsigned)nb), dim3(64, WPB), lds, s, P, PP, V, cf, g, wg, fz,
+                       tf, tf.t, flag);
+    return hipGetLastError();
+}
+
+// Pair accesses need even nz and strides and pair-aligned arrays.
+template <typename T>
+bool wave2_fits(const T *P, const T *PP, const float *V, const ImageFuse *fuse, const Grid &g)
+{
+    auto al = [](const void *p, size_t a) { return ((uintptr_t)p % a) == 0; };
+    if ((g.nz & 1) || (g.sy & 1) || (g.sx & 1)) return false;
+    if (!al(P, 2 * sizeof(T)) || !al(PP, 2 * sizeof(T)) || !al(V, 8)) return false;
+    if (fuse && (!al(fuse->ps, 2 * sizeof(T)) || !al(fuse->img, 8))) return false;
+    return true;
+}
+
+template <typename T, int R, int RY, int WPB>
+hipError_t launch_wave2(const T *P, T *PP, const float *V, const Coefs &cf, const Grid &g,
+                        const ImageFuse *fuse, const TaperFuse *taper, int *flag, hipStream_t s,
+                        int xchunk)
+{
+    ImageFuse fz;
+    TaperFuse tf;
+    fuse_args(fuse, taper, fz, tf);
+    const int nzt = (g.nz + 127) / 128;
+    // tapered z tiles first, the rest untapered (16-bit, as launch_wave)
+    const bool split = taper && !std::is_same<T, float>::value;
+    const int ntap = split ? std::min(nzt, (taper->b + R + 127) / 128) : nzt;
+    hipError_t e = by_variant(fuse, taper, [&](auto img, auto tap) {
+        return launch_wave2_cfg<T, R, RY, WPB, decltype(img)::value, decltype(tap)::value>(
+            P, PP, V, cf, g, fz, tf, flag, s, xchunk, 0, ntap);
+    });
+    if (e != hipSuccess || ntap >= nzt) return e;
+    return by_variant(fuse, nullptr, [&](auto img, auto tap) {
+        return launch_wave2_cfg<T, R, RY, WPB, decltype(img)::value, decltype(tap)::value>(
+            P, PP, V, cf, g, fz, tf, flag, s, xchunk, ntap, nzt - ntap);
+    });
+}
+
 // Full-row shapes compiled for radius 4 (by thread rows x ry rows each):
 // (4, 4) up to 4 waves per row (nz <= 256); (2, 6) and (2, 7) up to 6 waves
 // per row (nz <= 384).
@@ -1809,6 +2207,13 @@ hipError_t launch_radius(const T *P, T *PP, const float *V, const Coefs &cf, con
             return launch_rows2<T, R, 4, 3, 3>(P, PP, V, cf, g, fuse, taper, flag, s, xchunk);
     }
     if constexpr (R == 4) {
+        // barrier-free waves, two z cells per lane: 128 x 1 x 4 (every storage
+        // type); odd nz or strides run one cell per lane (64 x 1 x 8)
+        if (t && t->tz == 128 && t->by == 1 && t->ry == 4 && (!taper || taper->b <= 8192)) {
+            if (wave2_fits(P, PP, V, fuse, g))
+                return launch_wave2<T, R, 4, 4>(P, PP, V, cf, g, fuse, taper, flag, s, xchunk);
+            return launch_wave<T, R, 8, 4>(P, PP, V, cf, g, fuse, taper, flag, s, xchunk);
+        }
         // barrier-free waves: 64 x 1 x ry (every storage type)
         if (t && t->tz == 64 && t->by == 1 && (!taper || taper->b <= 8192)) {
             if (t->ry == 6)
@@ -1826,11 +2231,15 @@ hipError_t launch_radius(const T *P, T *PP, const float *V, const Coefs &cf, con
                 return launch_rows<T, R, 2, 6, 6>(P, PP, V, cf, g, fuse, taper, flag, s, xchunk);
         }
         // 16-bit wavefields by default: barrier-free waves, 64 x 1 x 6 for the
-        // imaging step and 64 x 1 x 8 otherwise (salt grid, FP16: imaging
-        // 0.672 ms, plain 0.317, tapered 0.398; full rows 2 x 6: 0.744 /
-        // 0.546 / 0.562; pairs: 0.778 / 0.517 / 0.563)
+        // imaging step, 128 x 1 x 4 (two z per lane) for the plain step and
+        // 64 x 1 x 8 for the tapered one (salt grid, FP16: imaging 0.672 ms,
+        // plain 0.305, tapered 0.398; 64 x 1 x 8 plain 0.317; 128 x 1 x 4
+        // imaging 0.737, tapered 0.483; full rows 2 x 6: 0.744 / 0.546 /
+        // 0.562; pairs: 0.778 / 0.517 / 0.563)
         if (!taper || taper->b <= 8192) {
             if (fuse) return launch_wave<T, R, 6, 4>(P, PP, V, cf, g, fuse, taper, flag, s, xchunk);
+            if (!taper && wave2_fits(P, PP, V, fuse, g))
+                return launch_wave2<T, R, 4, 4>(P, PP, V, cf, g, fuse, taper, flag, s, xchunk);
             return launch_wave<T, R, 8, 4>(P, PP, V, cf, g, fuse, taper, flag, s, xchunk);
         }
     }
@@ -1875,7 +2284,7 @@ hipError_t launch_radius(const T *P, T *PP, const float *V, const Coefs &cf, con
     if constexpr (full) {
         // the default tiles also serve the full-row, pair and wave shapes
         // where those do not apply (orders below 8, large tapers)
-        if (!t || t->tz == 1 || t->tz == 2 || (t->tz == 64 && t->by == 1) ||
+        if (!t || t->tz == 1 || t->tz == 2 || (t->tz == 64 && t->by == 1) || t->tz == 128 ||
             (t->tz == 64 && t->by == 4 && t->ry == 4))
             return launch_variant<T, R, 64, 4, 4>(P, PP, V, cf, g, fuse, taper, flag, s, xchunk);
         if (t->tz == 64 && t->by == 4 && t->ry == 2)
@@ -1964,7 +2373,8 @@ bool stencil_tiling_supported(int tz, int by, int ry)
     return (tz == 64 && by == 4 && (ry == 4 || ry == 2)) || (tz == 64 && by == 8 && ry == 2) ||
            (tz == 32 && by == 8 && (ry == 4 || ry == 2)) ||
            (tz == 1 && ((by == 4 && ry == 4) || (by == 2 && (ry == 6 || ry == 7)))) ||
-           (tz == 64 && by == 1 && (ry == 6 || ry == 8)) || (tz == 2 && by == 4 && ry == 3);
+           (tz == 64 && by == 1 && (ry == 6 || ry == 8)) || (tz == 2 && by == 4 && ry == 3) ||
+           (tz == 128 && by == 1 && ry == 4);
 }
 
 template <typename T>
