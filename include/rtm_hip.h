@@ -335,8 +335,9 @@ int rtmhip_ae_gemm_test(const void *A, long long sam, long long sak, int a_half,
  * order 8) 64x1x8 waves, else 64x4x4; a full-row request that does not
  * cover nze takes the same fallback.  16-bit wavefields run the 64x1xry
  * and 128x1x4 waves and the full rows 4x4 / 2x6 on request and otherwise
- * their defaults (64x1x6 waves for the imaging step, 128x1x4 waves for the
- * plain step where nze and the strides are even, 64x1x8 waves otherwise). */
+ * their defaults (128x1x4 waves for untapered steps where nze and the
+ * strides are even, else 64x1x6 waves for the imaging step and 64x1x8 waves
+ * otherwise). */
 int rtmhip_set_tiling(int kernel_class, int tz, int by, int ry, int xchunk);
 int rtmhip_get_tiling(int kernel_class, int *tz, int *by, int *ry, int *xchunk);
 

@@ -642,6 +642,7 @@ struct WaveGrid {
     int nzt, nyt, nxc, xchunk;
     long long waves;  // nzt * nyt * nxc
     int zt0;          // first z tile of the launch (its nzt tiles follow)
+    int zb;           // two-z waves: z of tile 0 (even); tile t starts at zb + 128 t
 };
 
 // One wave's tile of the barrier-free stencil: z tile zt (64 lanes), rows
@@ -963,7 +964,7 @@ constexpr int wave2_min_occupancy()
 template <typename T, int R, int RY, bool IMG, bool TAPER>
 __device__ __forceinline__ void wave_tile2(const T *__restrict__ P, T *__restrict__ PP,
                                            const float *__restrict__ V, const Coefs &cf,
-                                           const Grid &g, int zt, int yt, int xb, int xe,
+                                           const Grid &g, int z0, int yt, int xb, int xe,
                                            const ImageFuse &fz, const TaperFuse &tf,
                                            const float *tab, float (*lt)[128 + 2 * R],
                                            int *__restrict__ flag)
@@ -976,7 +977,7 @@ __device__ __forceinline__ void wave_tile2(const T *__restrict__ P, T *__restric
     constexpr unsigned VS = 4u / sizeof(T);
     const T *__restrict__ PS = static_cast<const T *>(fz.ps);
     const int lane = threadIdx.x;
-    const int z0 = zt * 128, y0 = R + yt * RY;
+    const int y0 = R + yt * RY;
     const int z = z0 + 2 * lane;  // the lane's first cell
     // nz even: nz - 2 starts the last in-grid pair
     const unsigned zc = (unsigned)min(z, g.nz - 2);
@@ -1235,13 +1236,13 @@ __global__ void __launch_bounds__(64 * WPB, (wave2_min_occupancy<IMG>()))
         w = t * WPB + wid;
     }
     if (w >= wg.waves) return;
-    const int zt = wg.zt0 + (int)(w % wg.nzt);
+    const int z0 = wg.zb + (wg.zt0 + (int)(w % wg.nzt)) * 128;
     const long long w2 = w / wg.nzt;
     const int yt = (int)(w2 % wg.nyt);
     const int xc = (int)(w2 / wg.nyt);
     const int xb = R + xc * wg.xchunk;
     const int xe = min(xb + wg.xchunk, g.nx - R);
-    wave_tile2<T, R, RY, IMG, TAPER>(P, PP, V, cf, g, zt, yt, xb, xe, fz, tf, tab, tile[wid],
+    wave_tile2<T, R, RY, IMG, TAPER>(P, PP, V, cf, g, z0, yt, xb, xe, fz, tf, tab, tile[wid],
                                      flag);
 }
 
@@ -2115,14 +2116,15 @@ hipError_t launch_wave(const T *P, T *PP, const float *V, const Coefs &cf, const
 }
 
 // Two-cells-per-lane waves, one launch over the z tiles [zt0, zt0 + nzt) of
-// 128 cells.
+// 128 cells, tile t starting at z = zb + 128 t (zb even).
 template <typename T, int R, int RY, int WPB, bool IMG, bool TAPER>
 hipError_t launch_wave2_cfg(const T *P, T *PP, const float *V, const Coefs &cf, const Grid &g,
                             const ImageFuse &fz, const TaperFuse &tf, int *flag, hipStream_t s,
-                            int xchunk, int zt0, int nzt)
+                            int xchunk, int zb, int zt0, int nzt)
 {
     auto kern = k_stencil_wave2<T, R, RY, WPB, IMG, TAPER>;
     WaveGrid wg;
+    wg.zb = zb;
     wg.zt0 = zt0;
     wg.nzt = nzt;
     if (nzt <= 0) return hipSuccess;
@@ -2169,12 +2171,44 @@ hipError_t launch_wave2(const T *P, T *PP, const float *V, const Coefs &cf, cons
     const int ntap = split ? std::min(nzt, (taper->b + R + 127) / 128) : nzt;
     hipError_t e = by_variant(fuse, taper, [&](auto img, auto tap) {
         return launch_wave2_cfg<T, R, RY, WPB, decltype(img)::value, decltype(tap)::value>(
-            P, PP, V, cf, g, fz, tf, flag, s, xchunk, 0, ntap);
+            P, PP, V, cf, g, fz, tf, flag, s, xchunk, 0, 0, ntap);
     });
     if (e != hipSuccess || ntap >= nzt) return e;
     return by_variant(fuse, nullptr, [&](auto img, auto tap) {
         return launch_wave2_cfg<T, R, RY, WPB, decltype(img)::value, decltype(tap)::value>(
-            P, PP, V, cf, g, fz, tf, flag, s, xchunk, ntap, nzt - ntap);
+            P, PP, V, cf, g, fz, tf, flag, s, xchunk, 0, ntap, nzt - ntap);
+    });
+}
+
+// The 64-cell z tiles that the upper taper reaches (zt * 64 - R < b) on
+// one-z waves of RY rows with the taper fused, every deeper cell on
+// untapered two-z waves starting at the first untapered tile (even z).
+// Without a taper: two-z waves throughout (the 16-bit default for untapered
+// steps; salt grid, FP16: imaging 0.605 ms on one-z 6-row waves, 0.545 on
+// two-z waves; plain 0.314 / 0.301).
+template <typename T, int R, int RY>
+hipError_t launch_wave_deep2(const T *P, T *PP, const float *V, const Coefs &cf, const Grid &g,
+                             const ImageFuse *fuse, const TaperFuse *taper, int *flag,
+                             hipStream_t s, int xchunk)
+{
+    ImageFuse fz;
+    TaperFuse tf;
+    fuse_args(fuse, taper, fz, tf);
+    const bool tapered = taper && taper->b > 0;
+    const int nzt = (g.nz + 63) / 64;
+    const int ntap = tapered ? std::min(nzt, (taper->b + R + 63) / 64) : 0;
+    if (ntap > 0) {
+        const hipError_t e = by_variant(fuse, taper, [&](auto img, auto tap) {
+            return launch_wave_cfg<T, R, RY, 4, decltype(img)::value, decltype(tap)::value>(
+                P, PP, V, cf, g, fz, tf, flag, s, xchunk, 0, ntap);
+        });
+        if (e != hipSuccess || ntap >= nzt) return e;
+    }
+    const int zb = 64 * ntap;
+    const int n2 = (g.nz - zb + 127) / 128;
+    return by_variant(fuse, nullptr, [&](auto img, auto tap) {
+        return launch_wave2_cfg<T, R, 4, 4, decltype(img)::value, decltype(tap)::value>(
+            P, PP, V, cf, g, fz, tf, flag, s, xchunk, zb, 0, n2);
     });
 }
 
@@ -2230,16 +2264,21 @@ hipError_t launch_radius(const T *P, T *PP, const float *V, const Coefs &cf, con
             if (t->by == 2 && t->ry == 6 && rows_fit(g.nz, 2, 6))
                 return launch_rows<T, R, 2, 6, 6>(P, PP, V, cf, g, fuse, taper, flag, s, xchunk);
         }
-        // 16-bit wavefields by default: barrier-free waves, 64 x 1 x 6 for the
-        // imaging step, 128 x 1 x 4 (two z per lane) for the plain step and
-        // 64 x 1 x 8 for the tapered one (salt grid, FP16: imaging 0.672 ms,
-        // plain 0.305, tapered 0.398; 64 x 1 x 8 plain 0.317; 128 x 1 x 4
-        // imaging 0.737, tapered 0.483; full rows 2 x 6: 0.744 / 0.546 /
-        // 0.562; pairs: 0.778 / 0.517 / 0.563)
+        // 16-bit wavefields by default: barrier-free waves.  Untapered steps
+        // (plain, RBC imaging) on two-z 128 x 1 x 4 waves where pairs fit;
+        // tapered ones on one-z waves (64 x 1 x 6 with imaging, 64 x 1 x 8
+        // otherwise) split into tapered and untapered tiles.  Salt grid,
+        // FP16: plain 0.305 ms (one-z 0.317), untapered imaging 0.545
+        // (0.605), tapered 0.398, tapered imaging 0.672; two-z waves behind
+        // the tapered tiles (launch_wave_deep2) made the HBC shot slower,
+        // 4.49 s against 4.40 (the 272 deep cells fill three 128-cell tiles);
+        // full rows 2 x 6: 0.546 / - / 0.562 / 0.744; pairs: 0.517 / - /
+        // 0.563 / 0.778.
         if (!taper || taper->b <= 8192) {
+            const bool tapered = taper && taper->b > 0;
+            if (!tapered && wave2_fits(P, PP, V, fuse, g))
+                return launch_wave_deep2<T, R, 8>(P, PP, V, cf, g, fuse, taper, flag, s, xchunk);
             if (fuse) return launch_wave<T, R, 6, 4>(P, PP, V, cf, g, fuse, taper, flag, s, xchunk);
-            if (!taper && wave2_fits(P, PP, V, fuse, g))
-                return launch_wave2<T, R, 4, 4>(P, PP, V, cf, g, fuse, taper, flag, s, xchunk);
             return launch_wave<T, R, 8, 4>(P, PP, V, cf, g, fuse, taper, flag, s, xchunk);
         }
     }

@@ -55,8 +55,9 @@ def shape16(request, gpu):
         gpu.set_tiling(cls, 0)
 
 
-# two and six waves per z row (full-row kernel), seven (z-tiled kernel)
-GRIDS16 = [(40, 44, 90), (32, 36, 340), (30, 32, 420)]
+# two and six waves per z row (full-row kernel), seven (z-tiled kernel);
+# odd nz: one z per lane for every wave shape
+GRIDS16 = [(40, 44, 90), (32, 36, 340), (30, 32, 420), (30, 34, 91)]
 
 
 @pytest.mark.parametrize("grid", GRIDS16)
