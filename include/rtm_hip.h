@@ -329,8 +329,10 @@ int rtmhip_ae_gemm_test(const void *A, long long sam, long long sak, int a_half,
  *   tz = 1 (order 8): full-z-row blocks 4x4 (nze <= 256), 2x6 and 2x7
  *           (nze <= 384);
  *   tz = 2 (order 8, even nze <= 384): full rows, two z per lane, 4x3;
- *   tz = 128 (order 8): barrier-free waves, two z per lane, 128x1x4 (odd
- *           nze or strides: 64x1x8 waves).
+ *   tz = 128 (order 8): barrier-free waves, two z per lane, 128x1x4;
+ *           128x1x6 and 128x1x8: one-z 64x1x6 / 64x1x8 waves on the z
+ *           tiles the upper taper reaches and two-z waves below (odd nze or
+ *           strides: one-z waves throughout).
  * tz = 0 restores the default: full-z-row blocks where they fit, else (FP32,
  * order 8) 64x1x8 waves, else 64x4x4; a full-row request that does not
  * cover nze takes the same fallback.  16-bit wavefields run the 64x1xry

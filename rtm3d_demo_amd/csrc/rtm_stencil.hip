@@ -2242,11 +2242,26 @@ hipError_t launch_radius(const T *P, T *PP, const float *V, const Coefs &cf, con
     }
     if constexpr (R == 4) {
         // barrier-free waves, two z cells per lane: 128 x 1 x 4 (every storage
-        // type); odd nz or strides run one cell per lane (64 x 1 x 8)
-        if (t && t->tz == 128 && t->by == 1 && t->ry == 4 && (!taper || taper->b <= 8192)) {
-            if (wave2_fits(P, PP, V, fuse, g))
-                return launch_wave2<T, R, 4, 4>(P, PP, V, cf, g, fuse, taper, flag, s, xchunk);
-            return launch_wave<T, R, 8, 4>(P, PP, V, cf, g, fuse, taper, flag, s, xchunk);
+        // type); 128 x 1 x ry (ry 6, 8): one-z 64 x 1 x ry waves on the z tiles
+        // the upper taper reaches, two-z waves below.  Odd nz or strides run
+        // one cell per lane (64 x 1 x 8, or 64 x 1 x ry).
+        if (t && t->tz == 128 && t->by == 1 && (!taper || taper->b <= 8192)) {
+            const bool fits = wave2_fits(P, PP, V, fuse, g);
+            if (t->ry == 4) {
+                if (fits)
+                    return launch_wave2<T, R, 4, 4>(P, PP, V, cf, g, fuse, taper, flag, s, xchunk);
+                return launch_wave<T, R, 8, 4>(P, PP, V, cf, g, fuse, taper, flag, s, xchunk);
+            }
+            if (t->ry == 6)
+                return fits ? launch_wave_deep2<T, R, 6>(P, PP, V, cf, g, fuse, taper, flag, s,
+                                                         xchunk)
+                            : launch_wave<T, R, 6, 4>(P, PP, V, cf, g, fuse, taper, flag, s,
+                                                      xchunk);
+            if (t->ry == 8)
+                return fits ? launch_wave_deep2<T, R, 8>(P, PP, V, cf, g, fuse, taper, flag, s,
+                                                         xchunk)
+                            : launch_wave<T, R, 8, 4>(P, PP, V, cf, g, fuse, taper, flag, s,
+                                                      xchunk);
         }
         // barrier-free waves: 64 x 1 x ry (every storage type)
         if (t && t->tz == 64 && t->by == 1 && (!taper || taper->b <= 8192)) {
@@ -2413,7 +2428,7 @@ bool stencil_tiling_supported(int tz, int by, int ry)
            (tz == 32 && by == 8 && (ry == 4 || ry == 2)) ||
            (tz == 1 && ((by == 4 && ry == 4) || (by == 2 && (ry == 6 || ry == 7)))) ||
            (tz == 64 && by == 1 && (ry == 6 || ry == 8)) || (tz == 2 && by == 4 && ry == 3) ||
-           (tz == 128 && by == 1 && ry == 4);
+           (tz == 128 && by == 1 && (ry == 4 || ry == 6 || ry == 8));
 }
 
 template <typename T>

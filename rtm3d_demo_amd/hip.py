@@ -316,7 +316,7 @@ def get_tiling(kernel_class):
 
 
 TILINGS = [(1, 4, 4), (1, 2, 6), (1, 2, 7), (2, 4, 3), (64, 1, 6), (64, 1, 8), (128, 1, 4),
-           (64, 4, 4), (64, 4, 2), (64, 8, 2), (32, 8, 4), (32, 8, 2)]
+           (128, 1, 6), (128, 1, 8), (64, 4, 4), (64, 4, 2), (64, 8, 2), (32, 8, 4), (32, 8, 2)]
 
 
 def timing_enable(on=True):

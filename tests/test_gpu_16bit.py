@@ -43,7 +43,8 @@ def test_migration_16bit_matches_oracle(gpu, oracle, cases, golden, fmt, bc, fus
     assert _same(img, oracle.pipeline_migration(p, vpe, records, np.float32, fmt))
 
 
-@pytest.fixture(params=[(0, 0, 0), (1, 4, 4), (1, 2, 6), (64, 1, 6), (128, 1, 4), (64, 4, 4)],
+@pytest.fixture(params=[(0, 0, 0), (1, 4, 4), (1, 2, 6), (64, 1, 6), (128, 1, 4), (128, 1, 8),
+                        (64, 4, 4)],
                 ids=lambda t: "t%d.%d.%d" % t)
 def shape16(request, gpu):
     """Default shapes, full rows requested (run where they cover nz), waves

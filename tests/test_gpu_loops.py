@@ -181,8 +181,8 @@ def _pair(gpu, A, B):
 
 @pytest.fixture(params=[((0, 0, 0), 0), ((1, 4, 4), 1), ((1, 4, 4), 5), ((1, 2, 6), 0),
                         ((1, 2, 7), 3), ((2, 4, 3), 0), ((2, 4, 3), 5), ((64, 1, 6), 0),
-                        ((64, 1, 8), 3), ((128, 1, 4), 0), ((128, 1, 4), 3), ((64, 4, 4), 0),
-                        ((64, 4, 4), 5)],
+                        ((64, 1, 8), 3), ((128, 1, 4), 0), ((128, 1, 4), 3), ((128, 1, 6), 0),
+                        ((128, 1, 8), 3), ((64, 4, 4), 0), ((64, 4, 4), 5)],
                 ids=lambda p: "t%d.%d.%d-x%d" % (*p[0], p[1]))
 def xchunk(request, gpu):
     """Stencil shape and planes marched per block: the built-in default, each
@@ -287,7 +287,7 @@ def test_fused_backward_matches_operator_sequence(gpu, oracle, splits, xchunk, s
 
 @pytest.mark.parametrize("tiling", [(64, 4, 4), (64, 4, 2), (64, 8, 2), (32, 8, 4), (32, 8, 2),
                                     (1, 4, 4), (1, 2, 6), (1, 2, 7), (2, 4, 3), (64, 1, 6),
-                                    (64, 1, 8), (128, 1, 4)])
+                                    (64, 1, 8), (128, 1, 4), (128, 1, 6), (128, 1, 8)])
 def test_tile_shapes_bit_exact(gpu, oracle, cases, golden, tiling):
     """Every compiled tile shape, for all three stencil classes, reproduces
     the reference HBC image bit for bit."""
