@@ -253,14 +253,21 @@ def model_shot(v2, coefs, g, src, blen, order, nt, ntstep, shot, geom, fmt=None)
 
 
 def migrate_shot(v2, coefs, g, src, blen, order, nt, ntstep, shot, geom, rcv, hbc=True,
-                 fmt=None):
+                 fmt=None, fwd_last=None):
+    """Image of one shot on the extended grid; fwd_last (optional, an array of
+    the wavefield storage type and grid shape) receives the forward pass's
+    last source wavefield."""
     dt = v2.dtype
     img = np.zeros(v2.shape, dt)
     rcv = np.ascontiguousarray(rcv, dt)
+    last = None
+    if fwd_last is not None:
+        assert fwd_last.shape == v2.shape and fwd_last.flags["C_CONTIGUOUS"]
+        last = fwd_last.ctypes.data_as(C.c_void_p)
     rc = getattr(lib(), "orc_migrate_shot_" + _sfx(dt, fmt))(v2, coefs, g, src, *v2.shape, blen,
                                                              order, nt, ntstep, *shot,
                                                              C.byref(geom), rcv, int(hbc), img,
-                                                             None)
+                                                             last)
     if rc != 0:
         raise RuntimeError("oracle migration failed (%d)" % rc)
     return img

@@ -50,12 +50,15 @@ def tune(hip, shot, V, coefs, g, src, F, R, upb, rcv, img, steps=6, rounds=2, lo
 BATCH_SHOT_TILINGS = [(1, 4, 4), (1, 2, 6), (1, 2, 7), (2, 4, 3), (64, 1, 8), (128, 1, 4)]
 
 
-def tune_batch(hip, batch, nt, steps=8, log=None):
+def tune_batch(hip, batch, nt, steps=17, log=None):
     """Autotunes a batch's stencil launches per kernel class: one launch per
     shot with each candidate tile shape (a full-row shape that does not cover
     a shot's nz runs that shot with the fallback), or one launch for every
     shot (barrier-free waves, 6 or 8 rows).  Each candidate is timed over
-    `steps` batched forward (class 2) or backward (classes 0, 1) steps.
+    `steps` batched forward (class 2) or backward (classes 0, 1) steps: the
+    first step of each call runs eagerly, the rest replay the batch's captured
+    graphs (steps = 17 is two replays of the default 8-step graph), so the
+    timing covers the graph form the batch runs.
 
     Installs the fastest form per class on the batch, and the fastest
     per-shot tile shape per class with set_tiling (so single-shot runs of the
@@ -73,7 +76,7 @@ def tune_batch(hip, batch, nt, steps=8, log=None):
                 batch.backward_steps(nt - 3, steps)
 
         def timed():
-            run()  # first pass captures the graphs
+            run()  # untimed: captures this candidate's graphs
             hip.synchronize()
             t = time.perf_counter()
             for _ in range(2):

@@ -48,14 +48,26 @@ def _run(cmd):
     return r
 
 
-def build_library(jobs=4):
+def _stale(obj, deps):
+    if not os.path.exists(obj):
+        return True
+    t = os.path.getmtime(obj)
+    return any(os.path.getmtime(d) > t for d in deps)
+
+
+def build_library(jobs=4, incremental=False):
+    """Compiles every HIP source (incremental=True: only those whose object
+    is older than the source or a shared header) and links the library."""
     os.makedirs(LIBDIR, exist_ok=True)
     os.makedirs(OBJDIR, exist_ok=True)
+    headers = [os.path.join(CSRC, "rtm_internal.hpp"), os.path.join(REPO, "include", "rtm_hip.h")]
     objs = []
     cmds = []
     for src in HIP_SOURCES:
         obj = os.path.join(OBJDIR, src.replace(".hip", ".o"))
         objs.append(obj)
+        if incremental and not _stale(obj, [os.path.join(CSRC, src)] + headers):
+            continue
         cmds.append([HIPCC] + HIP_FLAGS + ["-c", os.path.join(CSRC, src), "-o", obj])
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
         list(ex.map(_run, cmds))
@@ -82,12 +94,12 @@ def build_driver(jobs=4):
     return DRIVER
 
 
-def build_all(jobs=4):
-    build_library(jobs)
+def build_all(jobs=4, incremental=False):
+    build_library(jobs, incremental)
     if all(os.path.exists(os.path.join(CSRC, s)) for s in DRIVER_SOURCES):
         build_driver(jobs)
 
 
 if __name__ == "__main__":
-    build_all()
+    build_all(incremental="--incremental" in sys.argv)
     print("built", LIBNAME)

@@ -22,6 +22,7 @@ using namespace rtmhip;
 
 namespace {
 
+
 struct Timing {
     std::vector<hipEvent_t> start, stop;
     size_t used = 0;
@@ -35,6 +36,7 @@ struct State {
     bool timing = false;
     Timing tm[4];
     StencilTiling tiling[4] = {};  // per kernel class; tz == 0: built-in default
+    int tiling_gen = 0;            // bumped by every rtmhip_set_tiling call
 };
 
 State g_state;
@@ -510,8 +512,8 @@ int model_loop(const rtmhip_shot *shot, const ShotCtx &c, const float *V2DT2, co
         T *t = *P;
         *P = *PP;
         *PP = t;
-        HIPCHK(launch_taper(*P, taper, c.g, shot->blen, false, s), "taper");
-        HIPCHK(launch_taper(*PP, taper, c.g, shot->blen, false, s), "taper");
+        // taperAllBorders of P, then of PP: disjoint cells, one pass
+        HIPCHK(launch_taper_all2(*P, *PP, taper, c.g, shot->blen, s), "taper");
         HIPCHK(stencil(c.radius, *P, *PP, V2DT2, c.cf, c.g, nullptr), "stencil");
         HIPCHK(launch_source(*PP, c.src_off, ricker, it, 0.f, s), "source");
         HIPCHK(launch_receivers(*PP, rcv_chunk, c.g, c.rx0, c.ry0, c.rz, shot->rcv.dist_x,
@@ -746,6 +748,7 @@ int rtmhip_set_tiling(int kernel_class, int tz, int by, int ry, int xchunk)
         return fail(RTMHIP_ERR_INVALID, "kernel class %d not in 0..3", kernel_class);
     if (tz == 0) {
         g_state.tiling[kernel_class] = StencilTiling{};
+        ++g_state.tiling_gen;
         return RTMHIP_OK;
     }
     const bool ok = kernel_class == 3 ? (tz == 64 && by == 1 && ry == 4)
@@ -753,6 +756,7 @@ int rtmhip_set_tiling(int kernel_class, int tz, int by, int ry, int xchunk)
     if (!ok || xchunk < 0)
         return fail(RTMHIP_ERR_UNSUPPORTED, "tile shape %dx%dx%d not compiled", tz, by, ry);
     g_state.tiling[kernel_class] = StencilTiling{tz, by, ry, xchunk};
+    ++g_state.tiling_gen;
     return RTMHIP_OK;
 }
 
@@ -812,8 +816,10 @@ struct rtmhip_batch {
     int blen = 0, nt = 0, hf = 0, radius = 0;
     Coefs cf{};
     const float *taper = nullptr;
-    std::vector<rtmhip_shot> shots;
-    std::vector<BatchShot> hs[2];   // host tables: wave rows 6 and 8
+    std::vector<rtmhip_shot> shots_v;
+    std::vector<BatchShot> hs_v[2];  // host tables of a created batch: wave rows 6 and 8
+    const rtmhip_shot *shots = nullptr;        // views of the tables above, or of
+    BatchShot *hs[2] = {nullptr, nullptr};     // fixed storage (no-allocation runs)
     BatchShot *ds[2] = {nullptr, nullptr};
     long long nblk[2] = {0, 0};
     int fwd_blocks = 0, bwd_blocks = 0;
@@ -822,6 +828,7 @@ struct rtmhip_batch {
     int mode[3] = {0, 0, 0};        // per stencil class: 0 per-shot launches, 6/8 batched
     int graph_steps = 8;            // steps per captured graph (0: no graphs)
     std::map<int, hipGraphExec_t> graphs;
+    int tiling_gen = 0;             // g_state.tiling_gen when the graphs were captured
 };
 
 namespace {
@@ -839,7 +846,7 @@ int batch_fwd_step(rtmhip_batch *b, int taps, const int *itp, int itoff)
     const int cls = 2;
     if (b->mode[cls]) {
         const int k = batch_tab(b, cls);
-        HIPCHK(launch_stencil_batch<T>(b->mode[cls], b->ds[k], b->hs[k].data(), b->n, b->nblk[k],
+        HIPCHK(launch_stencil_batch<T>(b->mode[cls], b->ds[k], b->hs[k], b->n, b->nblk[k],
                                        false, b->cs, 0, b->cf, &tf, g_state.flag, s),
                "batched stencil");
     } else {
@@ -866,7 +873,7 @@ int batch_bwd_step(rtmhip_batch *b, int taps, const int *itp, int itoff)
     const TaperFuse tf{b->taper, b->blen, taps};
     if (b->mode[0]) {
         const int k = batch_tab(b, 0);
-        HIPCHK(launch_stencil_batch<T>(b->mode[0], b->ds[k], b->hs[k].data(), b->n, b->nblk[k],
+        HIPCHK(launch_stencil_batch<T>(b->mode[0], b->ds[k], b->hs[k], b->n, b->nblk[k],
                                        false, b->cs, 0, b->cf, nullptr, g_state.flag, s),
                "batched stencil");
     } else {
@@ -880,7 +887,7 @@ int batch_bwd_step(rtmhip_batch *b, int taps, const int *itp, int itoff)
     }
     if (b->mode[1]) {
         const int k = batch_tab(b, 1);
-        HIPCHK(launch_stencil_batch<T>(b->mode[1], b->ds[k], b->hs[k].data(), b->n, b->nblk[k],
+        HIPCHK(launch_stencil_batch<T>(b->mode[1], b->ds[k], b->hs[k], b->n, b->nblk[k],
                                        true, b->cr, b->cs ^ 1, b->cf, &tf, g_state.flag, s),
                "batched stencil");
     } else {
@@ -917,6 +924,14 @@ int batch_bwd_step(rtmhip_batch *b, int taps, const int *itp, int itoff)
 template <typename T>
 int batch_graph(rtmhip_batch *b, int dir, hipGraphExec_t *out)
 {
+    if (b->tiling_gen != g_state.tiling_gen) {
+        // per-shot launches in the captured graphs use the class tilings of
+        // their capture: recapture after rtmhip_set_tiling
+        HIPCHK(hipStreamSynchronize(g_state.stream), "hipStreamSynchronize");
+        for (auto &kv : b->graphs) (void)hipGraphExecDestroy(kv.second);
+        b->graphs.clear();
+        b->tiling_gen = g_state.tiling_gen;
+    }
     const int key = (dir > 0 ? 0 : 4) | (b->cs << 1) | b->cr;
     auto f = b->graphs.find(key);
     if (f != b->graphs.end()) {
@@ -925,12 +940,18 @@ int batch_graph(rtmhip_batch *b, int dir, hipGraphExec_t *out)
     }
     hipStream_t s = g_state.stream;
     const int cs = b->cs, cr = b->cr;
+    // Every launch shape of the captured steps was used by the eager first
+    // step of this call (batch_run), so the launchers' occupancy cache
+    // already holds it; the guard makes a miss an error instead of a runtime
+    // query inside the capture.
     HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+    capture_guard(true);
     int rc = RTMHIP_OK;
     for (int k = 0; k < b->graph_steps && rc == RTMHIP_OK; ++k)
         rc = dir > 0 ? batch_fwd_step<T>(b, 2, b->ctr, k) : batch_bwd_step<T>(b, 2, b->ctr, -k);
     hipError_t e = rc == RTMHIP_OK ? launch_tick(b->ctr, dir * b->graph_steps, false, s)
                                    : hipSuccess;
+    capture_guard(false);
     hipGraph_t graph = nullptr;
     hipError_t e2 = hipStreamEndCapture(s, &graph);
     b->cs = cs;
@@ -1026,34 +1047,28 @@ int batch_backward(rtmhip_batch *b, int it_hi, int nsteps)
     return batch_run<T>(b, -1, it, left);
 }
 
-}  // namespace
-
-extern "C" {
-
-int rtmhip_batch_create(rtmhip_batch **out, const rtmhip_batch_shot *shots, int nshots,
-                        const float *coefs, const float *taper)
+// Validates a batch's shots and fills the host table hs (n entries, the
+// 6-row plan not yet made), the shot array sh and the batch's shared
+// fields; allocates nothing.
+int batch_fill(rtmhip_batch *b, const rtmhip_batch_shot *shots, int nshots, const float *coefs,
+               const float *taper, BatchShot *hs, rtmhip_shot *sh)
 {
-    NEED_INIT();
-    if (!out || !shots || !coefs || !taper) return fail(RTMHIP_ERR_INVALID, "NULL pointer");
-    *out = nullptr;
-    if (nshots <= 0) return fail(RTMHIP_ERR_INVALID, "empty batch");
-    auto b = std::make_unique<rtmhip_batch>();
     b->n = nshots;
     const rtmhip_shot &s0 = shots[0].shot;
     for (int i = 0; i < nshots; ++i) {
         const rtmhip_batch_shot &bs = shots[i];
-        const rtmhip_shot &sh = bs.shot;
+        const rtmhip_shot &shot = bs.shot;
         ShotCtx c;
         int rc;
-        if ((rc = prepare_shot(&sh, coefs, &c))) return rc;
-        if ((rc = check_format(&sh))) return rc;
-        if (sh.order != 8) return fail(RTMHIP_ERR_UNSUPPORTED, "batches need stencil order 8");
-        if (sh.blen != s0.blen || sh.nt != s0.nt || sh.wave_format != s0.wave_format)
+        if ((rc = prepare_shot(&shot, coefs, &c))) return rc;
+        if ((rc = check_format(&shot))) return rc;
+        if (shot.order != 8) return fail(RTMHIP_ERR_UNSUPPORTED, "batches need stencil order 8");
+        if (shot.blen != s0.blen || shot.nt != s0.nt || shot.wave_format != s0.wave_format)
             return fail(RTMHIP_ERR_INVALID, "shot %d: blen, nt and wave_format must match shot 0",
                         i);
-        if (sh.blen > 8192)
-            return fail(RTMHIP_ERR_UNSUPPORTED, "shot %d: blen %d above 8192", i, sh.blen);
-        if (sh.ntstep != sh.nt)
+        if (shot.blen > 8192)
+            return fail(RTMHIP_ERR_UNSUPPORTED, "shot %d: blen %d above 8192", i, shot.blen);
+        if (shot.ntstep != shot.nt)
             return fail(RTMHIP_ERR_INVALID, "shot %d: batches keep whole records (ntstep == nt)",
                         i);
         if (!bs.V2DT2 || !bs.ricker || !bs.fs[0] || !bs.fs[1] || !bs.fr[0] || !bs.fr[1] ||
@@ -1074,12 +1089,12 @@ int rtmhip_batch_create(rtmhip_batch **out, const rtmhip_batch_shot *shots, int 
         h.rx0 = c.rx0;
         h.ry0 = c.ry0;
         h.rz = c.rz;
-        h.rdx = std::max(1, sh.rcv.dist_x);
-        h.rdy = std::max(1, sh.rcv.dist_y);
-        h.rnx = sh.rcv.count_x;
-        h.rny = sh.rcv.count_y;
-        b->hs[0].push_back(h);
-        b->shots.push_back(sh);
+        h.rdx = std::max(1, shot.rcv.dist_x);
+        h.rdy = std::max(1, shot.rcv.dist_y);
+        h.rnx = shot.rcv.count_x;
+        h.rny = shot.rcv.count_y;
+        hs[i] = h;
+        sh[i] = shot;
         if (i == 0) {
             b->radius = c.radius;
             b->hf = c.hf;
@@ -1091,7 +1106,8 @@ int rtmhip_batch_create(rtmhip_batch **out, const rtmhip_batch_shot *shots, int 
     b->nt = s0.nt;
     b->taper = taper;
     long long fw = 0, bw = 0;
-    for (auto &h : b->hs[0]) {
+    for (int i = 0; i < nshots; ++i) {
+        BatchShot &h = hs[i];
         const long long nf = tail_fwd_items(h.g, b->blen, b->radius, b->hf);
         const long long nb = tail_bwd_items(h.g, b->blen, b->radius, b->hf, h.rnx, h.rny);
         if (nf >= (1LL << 31) || nb >= (1LL << 31) ||
@@ -1108,9 +1124,33 @@ int rtmhip_batch_create(rtmhip_batch **out, const rtmhip_batch_shot *shots, int 
     }
     b->fwd_blocks = (int)fw;
     b->bwd_blocks = (int)bw;
-    b->hs[1] = b->hs[0];
-    b->nblk[0] = plan_stencil_batch(6, b->hs[0].data(), nshots, b->radius, b->blen);
-    b->nblk[1] = plan_stencil_batch(8, b->hs[1].data(), nshots, b->radius, b->blen);
+    return RTMHIP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int rtmhip_batch_create(rtmhip_batch **out, const rtmhip_batch_shot *shots, int nshots,
+                        const float *coefs, const float *taper)
+{
+    NEED_INIT();
+    if (!out || !shots || !coefs || !taper) return fail(RTMHIP_ERR_INVALID, "NULL pointer");
+    *out = nullptr;
+    if (nshots <= 0) return fail(RTMHIP_ERR_INVALID, "empty batch");
+    auto b = std::make_unique<rtmhip_batch>();
+    b->shots_v.resize(nshots);
+    b->hs_v[0].resize(nshots);
+    int rc = batch_fill(b.get(), shots, nshots, coefs, taper, b->hs_v[0].data(),
+                        b->shots_v.data());
+    if (rc) return rc;
+    b->hs_v[1] = b->hs_v[0];
+    b->shots = b->shots_v.data();
+    b->hs[0] = b->hs_v[0].data();
+    b->hs[1] = b->hs_v[1].data();
+    b->tiling_gen = g_state.tiling_gen;
+    b->nblk[0] = plan_stencil_batch(6, b->hs[0], nshots, b->radius, b->blen);
+    b->nblk[1] = plan_stencil_batch(8, b->hs[1], nshots, b->radius, b->blen);
     if (b->nblk[0] <= 0 || b->nblk[1] <= 0)
         return fail(RTMHIP_ERR_INVALID, "a shot's grid leaves no interior");
     const size_t bytes = sizeof(BatchShot) * (size_t)nshots;
@@ -1119,7 +1159,7 @@ int rtmhip_batch_create(rtmhip_batch **out, const rtmhip_batch_shot *shots, int 
         hipError_t e = hipMalloc(&p, bytes);
         if (e == hipSuccess) {
             b->ds[k] = (BatchShot *)p;
-            e = hipMemcpyAsync(p, b->hs[k].data(), bytes, hipMemcpyHostToDevice, g_state.stream);
+            e = hipMemcpyAsync(p, b->hs[k], bytes, hipMemcpyHostToDevice, g_state.stream);
         }
         if (e != hipSuccess) {
             rtmhip_batch_destroy(b.release());

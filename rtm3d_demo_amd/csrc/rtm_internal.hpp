@@ -122,6 +122,10 @@ hipError_t launch_stencil_dual(int radius, const float *Ps, float *PPs, const fl
 template <typename T>
 hipError_t launch_taper(T *P, const float *taper, const Grid &g, int blen, bool upper_only,
                         hipStream_t s);
+// taper3D of two grids of the same shape in one pass (Q may be NULL: one grid).
+template <typename T>
+hipError_t launch_taper_all2(T *P, T *Q, const float *taper, const Grid &g, int b,
+                             hipStream_t s);
 // Upper taper (applied `taps` times) of the slab cells the stencil never
 // updates (z < R or the x / y frame of width R), in place.
 template <typename T>
@@ -175,6 +179,10 @@ hipError_t launch_tail_bwd(const BatchShot *shots, int n, int blocks, int spar, 
 long long tail_fwd_items(const Grid &g, int b, int R, int hf);
 long long tail_bwd_items(const Grid &g, int b, int R, int hf, int rnx, int rny);
 int tail_blocks(long long items);
+// Brackets a stream capture: while on, stencil launchers use only launch
+// shapes already seen eagerly (no occupancy or attribute call inside the
+// capture) and fail with hipErrorStreamCaptureUnsupported on a new one.
+void capture_guard(bool on);
 // *p += d, or *p = d with set (one thread): the device-resident time index
 // of a captured step sequence.
 hipError_t launch_tick(int *p, int d, bool set, hipStream_t s);

@@ -146,8 +146,10 @@ __global__ void __launch_bounds__(kThreads)
 // face pass's product P*gy is the value it stored (rounded to T).
 template <typename T>
 __global__ void __launch_bounds__(kThreads)
-    k_taper_front_right(T *__restrict__ P, const float *__restrict__ t, Grid g, int b)
+    k_taper_front_right(T *__restrict__ P0, T *__restrict__ P1, const float *__restrict__ t,
+                        Grid g, int b)
 {
+    T *__restrict__ P = blockIdx.y ? P1 : P0;  // one grid per blockIdx.y
     const int lz = g.nz - 2 * b;
     const long long n = (long long)b * b * lz;
     const long long i = (long long)blockIdx.x * kThreads + threadIdx.x;
@@ -167,69 +169,54 @@ __global__ void __launch_bounds__(kThreads)
 }
 
 // taper3D (RTMCPUPlatform.cpp:710-920) for every border cell except the
-// front-right edge.  Work is enumerated as three disjoint slabs:
-//   A: z border (all x, y)      nx*ny*2b cells
-//   B: y border, z inner        nx*2b*lz
-//   C: x border, y and z inner  2b*ly*lz
+// front-right edge's interior-z cells, of one or two grids (blockIdx.y picks
+// the grid).  One wave per (x, y) row, z on the lanes: in a row with x or y in
+// a border every z is a border cell; otherwise, and in the front-right
+// column (x < b, y >= ny-b), only z < b and z >= nz-b are (the 2b cells are
+// packed onto consecutive lanes).  x, y and their factors are wave-uniform.
 // Per-cell factor order follows the reference loop nests.
 template <typename T>
 __global__ void __launch_bounds__(kThreads)
-    k_taper_all(T *__restrict__ P, const float *__restrict__ t, Grid g, int b)
+    k_taper_all(T *__restrict__ P0, T *__restrict__ P1, const float *__restrict__ t, Grid g,
+                int b)
 {
-    const int lz = g.nz - 2 * b, ly = g.ny - 2 * b;
-    const long long nA = (long long)g.nx * g.ny * 2 * b;
-    const long long nB = (long long)g.nx * 2 * b * lz;
-    const long long nC = (long long)2 * b * ly * lz;
-    long long i = (long long)blockIdx.x * kThreads + threadIdx.x;
-    int x, y, z;
-    if (i < nA) {
-        const int zz = (int)(i % (2 * b));
-        const long long r = i / (2 * b);
-        y = (int)(r % g.ny);
-        x = (int)(r / g.ny);
-        z = zz < b ? zz : g.nz - 2 * b + zz;
-    } else if ((i -= nA) < nB) {
-        const int zi = (int)(i % lz);
-        const long long r = i / lz;
-        const int yy = (int)(r % (2 * b));
-        x = (int)(r / (2 * b));
-        y = yy < b ? yy : g.ny - 2 * b + yy;
-        z = b + zi;
-    } else if ((i -= nB) < nC) {
-        const int zi = (int)(i % lz);
-        const long long r = i / lz;
-        const int yi = (int)(r % ly);
-        const int xx = (int)(r / ly);
-        x = xx < b ? xx : g.nx - 2 * b + xx;
-        y = b + yi;
-        z = b + zi;
-    } else {
-        return;
-    }
+    T *__restrict__ P = blockIdx.y ? P1 : P0;
+    const int lane = threadIdx.x & 63;
+    const int row = __builtin_amdgcn_readfirstlane((int)blockIdx.x * (kThreads / 64) +
+                                                   (int)(threadIdx.x >> 6));
+    if (row >= g.nx * g.ny) return;
+    const int x = row / g.ny;
+    const int y = row - x * g.ny;
     const bool bx = x < b || x >= g.nx - b;
     const bool by = y < b || y >= g.ny - b;
-    const bool bz = z < b || z >= g.nz - b;
-    if (bx && by && !bz && x < b && y >= g.ny - b) return;  // front-right edge
-    const long long o = goff(g, x, y, z);
-    float v = wld(P[o]);
-    if (bx && bz) {
-        if (by) v *= gfac(t, y, g.ny, b);
-        v *= gfac(t, x, g.nx, b);
-        v *= gfac(t, z, g.nz, b);
-    } else if (by && bz) {
-        v *= gfac(t, y, g.ny, b);
-        v *= gfac(t, z, g.nz, b);
-    } else if (bx && by) {
-        v *= gfac(t, x, g.nx, b);
-        v *= gfac(t, y, g.ny, b);
-    } else if (bz) {
-        v *= gfac(t, z, g.nz, b);
-    } else if (by) {
-        v *= gfac(t, y, g.ny, b);
-    } else {
-        v *= gfac(t, x, g.nx, b);
+    const float gx = bx ? gfac(t, x, g.nx, b) : 1.0f;
+    const float gy = by ? gfac(t, y, g.ny, b) : 1.0f;
+    T *__restrict__ r = P + ((long long)x * g.sx + (long long)y * g.sy);
+    const bool full = (bx || by) && !(x < b && y >= g.ny - b);
+    const int n = full ? g.nz : 2 * b;
+    for (int i = lane; i < n; i += 64) {
+        const int z = full || i < b ? i : g.nz - 2 * b + i;
+        const bool bz = z < b || z >= g.nz - b;
+        float v = wld(r[z]);
+        if (bx && bz) {
+            if (by) v *= gy;
+            v *= gx;
+            v *= gfac(t, z, g.nz, b);
+        } else if (by && bz) {
+            v *= gy;
+            v *= gfac(t, z, g.nz, b);
+        } else if (bx && by) {
+            v *= gx;
+            v *= gy;
+        } else if (bz) {
+            v *= gfac(t, z, g.nz, b);
+        } else if (by) {
+            v *= gy;
+        } else {
+            v *= gx;
+        }
+        r[z] = wst<T>(v);
     }
-    P[o] = wst<T>(v);
 }
 
 // rtmApplySource (RTMCPUPlatform.cpp:103-113): PP[s] += value (or table[it]).
@@ -612,15 +599,27 @@ hipError_t launch_taper(T *P, const float *taper, const Grid &g, int b, bool upp
         hipLaunchKernelGGL(k_taper_upper<T>, dim3(nblocks(n)), dim3(kThreads), 0, s, P, taper, g, b);
         return hipGetLastError();
     }
-    const long long lz = g.nz - 2 * b, ly = g.ny - 2 * b;
+    return launch_taper_all2(P, (T *)nullptr, taper, g, b, s);
+}
+
+template <typename T>
+hipError_t launch_taper_all2(T *P, T *Q, const float *taper, const Grid &g, int b,
+                             hipStream_t s)
+{
+    if (b <= 0) return hipSuccess;
+    if ((long long)g.nx * g.ny >= (1LL << 31)) return hipErrorInvalidValue;
+    const unsigned ng = Q ? 2 : 1;
+    // the front-right edge first: it reads (blen, y, z) before the face pass
+    // rescales that cell
+    const long long lz = g.nz - 2 * b;
     const long long nfr = (long long)b * b * lz;
     if (nfr > 0 && g.nx > 2 * b) {
-        hipLaunchKernelGGL(k_taper_front_right<T>, dim3(nblocks(nfr)), dim3(kThreads), 0, s, P,
-                           taper, g, b);
+        hipLaunchKernelGGL(k_taper_front_right<T>, dim3(nblocks(nfr), ng), dim3(kThreads), 0, s,
+                           P, Q, taper, g, b);
     }
-    const long long n = (long long)g.nx * g.ny * 2 * b + (long long)g.nx * 2 * b * lz +
-                        (long long)2 * b * ly * lz;
-    hipLaunchKernelGGL(k_taper_all<T>, dim3(nblocks(n)), dim3(kThreads), 0, s, P, taper, g, b);
+    const long long rows = (long long)g.nx * g.ny;
+    hipLaunchKernelGGL(k_taper_all<T>, dim3((unsigned)((rows + kThreads / 64 - 1) / (kThreads / 64)), ng),
+                       dim3(kThreads), 0, s, P, Q, taper, g, b);
     return hipGetLastError();
 }
 
@@ -785,6 +784,8 @@ hipError_t launch_tick(int *p, int d, bool set, hipStream_t s)
 #define RTM_INSTANTIATE_OPS(T)                                                                \
     template hipError_t launch_taper<T>(T *, const float *, const Grid &, int, bool,            \
                                         hipStream_t);                                           \
+    template hipError_t launch_taper_all2<T>(T *, T *, const float *, const Grid &, int,        \
+                                             hipStream_t);                                      \
     template hipError_t launch_taper_frame<T>(T *, const float *, const Grid &, int, int, int,  \
                                               hipStream_t);                                     \
     template hipError_t launch_source<T>(T *, long long, const float *, int, float, hipStream_t); \

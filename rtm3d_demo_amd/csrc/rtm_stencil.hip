@@ -35,6 +35,7 @@
 #include "rtm_internal.hpp"
 
 #include <algorithm>
+#include <atomic>
 #include <cstdint>
 #include <map>
 #include <mutex>
@@ -1866,9 +1867,16 @@ struct LaunchKey {
     }
 };
 
+// Depth of active stream captures (capture_guard).  While it is non-zero,
+// device_capacity answers from its cache only: hipFuncSetAttribute and the
+// occupancy query are never issued inside a capture.
+std::atomic<int> g_capture_depth{0};
+
 // Resident blocks on the whole device for one kernel (occupancy x CUs).
 // Kernels using more than 64 KiB of dynamic LDS get the attribute raised
-// first (gfx950 has 160 KiB per CU).
+// first (gfx950 has 160 KiB per CU).  Returns 0 for a launch shape first seen
+// during a capture; the launchers then fail with
+// hipErrorStreamCaptureUnsupported rather than query the runtime there.
 int device_capacity(const void *fn, int threads, size_t lds = 0)
 {
     static std::mutex mu;
@@ -1877,6 +1885,7 @@ int device_capacity(const void *fn, int threads, size_t lds = 0)
     const LaunchKey key{fn, threads, lds};
     auto it = cache.find(key);
     if (it != cache.end()) return it->second;
+    if (g_capture_depth.load() > 0) return 0;
     if (lds > 65536)
         (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     int dev = 0, ncu = 0, per_cu = 0;
@@ -1927,6 +1936,7 @@ hipError_t launch_cfg(const T *P, T *PP, const float *V, const Coefs &cf, const 
     if (tg.nty <= 0 || nxi <= 0 || g.nz - 2 * R <= 0) return hipSuccess;  // nothing to update
     if (xchunk <= 0) {
         const int cap = device_capacity((const void *)kern, TZ * BY);
+        if (cap <= 0) return hipErrorStreamCaptureUnsupported;
         const int n = choose_chunks((long long)tg.ntz * tg.nty, nxi, R, cap);
         xchunk = (nxi + n - 1) / n;
     }
@@ -1992,6 +2002,7 @@ hipError_t launch_rows_cfg(const T *P, T *PP, const float *V, const Coefs &cf, c
     const size_t lds = rows_lds_bytes<R, BY, RY, NWM>() + (TAPER ? (size_t)tf.b * sizeof(float) : 0);
     const int threads = 64 * nw * BY;
     const int cap = device_capacity((const void *)kern, threads, lds);
+    if (cap <= 0) return hipErrorStreamCaptureUnsupported;
     if (xchunk <= 0) {
         const int n = choose_chunks(rg.nty, nxi, R, cap);
         xchunk = (nxi + n - 1) / n;
@@ -2034,6 +2045,7 @@ hipError_t launch_rows2_cfg(const T *P, T *PP, const float *V, const Coefs &cf, 
                        (TAPER ? (size_t)tf.b * sizeof(float) : 0);
     const int threads = 64 * nw * BY;
     const int cap = device_capacity((const void *)kern, threads, lds);
+    if (cap <= 0) return hipErrorStreamCaptureUnsupported;
     if (xchunk <= 0) {
         const int n = choose_chunks(rg.nty, nxi, R, cap);
         xchunk = (nxi + n - 1) / n;
@@ -2076,6 +2088,7 @@ hipError_t launch_wave_cfg(const T *P, T *PP, const float *V, const Coefs &cf, c
     const size_t lds = TAPER ? (size_t)tf.b * sizeof(float) : 0;
     if (xchunk <= 0) {
         const int cap = device_capacity((const void *)kern, 64 * WPB, lds) * WPB;
+        if (cap <= 0) return hipErrorStreamCaptureUnsupported;
         const int n = choose_chunks((long long)wg.nzt * wg.nyt, nxi, R, cap);
         xchunk = (nxi + n - 1) / n;
     }
@@ -2134,6 +2147,7 @@ hipError_t launch_wave2_cfg(const T *P, T *PP, const float *V, const Coefs &cf, 
     const size_t lds = TAPER ? (size_t)tf.b * sizeof(float) : 0;
     if (xchunk <= 0) {
         const int cap = device_capacity((const void *)kern, 64 * WPB, lds) * WPB;
+        if (cap <= 0) return hipErrorStreamCaptureUnsupported;
         const int n = choose_chunks((long long)wg.nzt * wg.nyt, nxi, R, cap);
         xchunk = (nxi + n - 1) / n;
     }
@@ -2386,6 +2400,7 @@ hipError_t launch_dual_cfg(const T *Ps, T *PPs, const T *Pr, T *PPr, const float
     const size_t lds = TAPER ? (size_t)tf.b * sizeof(float) : 0;
     if (xchunk <= 0) {
         const int cap = device_capacity((const void *)kern, 64 * WPB, lds) * WPB;
+        if (cap <= 0) return hipErrorStreamCaptureUnsupported;
         const int n = choose_chunks((long long)wg.nzt * wg.nyt, nxi, R, cap);
         xchunk = (nxi + n - 1) / n;
     }
@@ -2450,6 +2465,8 @@ hipError_t launch_stencil(int radius, const T *P, T *PP, const float *V, const C
     }
 }
 
+void capture_guard(bool on) { g_capture_depth += on ? 1 : -1; }
+
 long long plan_stencil_batch(int ry, BatchShot *hs, int n, int R, int blen)
 {
     if (R != 4 || (ry != 6 && ry != 8) || n <= 0) return -1;
@@ -2458,6 +2475,7 @@ long long plan_stencil_batch(int ry, BatchShot *hs, int n, int R, int blen)
     const void *fn = ry == 6 ? (const void *)k_stencil_batch<float, 4, 6, kBatchWPB, true, true>
                              : (const void *)k_stencil_batch<float, 4, 8, kBatchWPB, true, true>;
     const int cap = device_capacity(fn, 64 * kBatchWPB, (size_t)blen * sizeof(float)) * kBatchWPB;
+    if (cap <= 0) return -1;
     long long tiles = 0;
     int nxmin = 1 << 30;
     for (int i = 0; i < n; ++i) {

@@ -139,6 +139,17 @@ def _lerp_matrix(start, n, step, ncoarse):
     return m
 
 
+def salt_1024_model(seed=2025, vmax=4500.0, d=12.5):
+    """The whole 1024 x 1024 x 320 model (1.3 GB, float32), cell for cell the
+    windows of salt_1024_window put side by side."""
+    nx, ny, nz = SALT1024
+    v = np.empty(SALT1024, np.float32)
+    step = 128
+    for x0 in range(0, nx, step):
+        v[x0:x0 + step] = salt_1024_window(x0, 0, step, ny, nz, seed, vmax, d)
+    return v
+
+
 def salt_1024_window(x0, y0, nx, ny, nz, seed=2025, vmax=4500.0, d=12.5):
     """Cells [x0, x0+nx) x [y0, y0+ny) x [0, nz) of the 1024^2 x 320 model."""
     noise, bodies = _salt1024_fields(seed)

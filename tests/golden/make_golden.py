@@ -19,6 +19,13 @@ Cases (all grids non-cubic with distinct spacings, so a transposed axis fails):
 
 ``make_golden.py fmig`` regenerates only small_fmig (from small_mod.npz's
 records), leaving the other fixtures untouched.
+
+``make_golden.py c0`` runs the 128^3 three-layer modeling case in full (1000
+steps, 64 x 64 receivers, border 32; about two minutes on 8 cores) and stores
+what pins the whole record without shipping its 16 MB: the SHA-256 of the
+reference's seismogram file, every trace's float64 L2 norm, and a decimated
+sub-record (every 8th receiver in x and y, every 4th sample) in
+``c0_modeling.npz``.
 """
 import json
 import os
@@ -165,11 +172,54 @@ def main_fmig():
     print("fixture written: small_fmig")
 
 
+def c0_params():
+    """The 128^3 three-layer modeling case (15 Hz, dt from CFL 0.4, taper 0.9)."""
+    nx = ny = nz = 128
+    d = 10.0
+    p = base_params("l3_128", nx, ny, nz, 32, 1000, 1000, d, d, d, 8)
+    p.update({"source_start_x": 64, "source_start_y": 64, "source_depth_z": 2,
+              "receiver_start_x": 0, "receiver_start_y": 0, "receiver_depth_z": 0,
+              "receiver_distance_x": 2, "receiver_distance_y": 2,
+              "receiver_count_x": 64, "receiver_count_y": 64,
+              "receiver_total": 4096, "fmax": 30.0})
+    return p
+
+
+def main_c0():
+    import hashlib
+
+    tmp = tempfile.mkdtemp(prefix="rtmgold_")
+    try:
+        p = c0_params()
+        wd = os.path.join(tmp, "c0")
+        os.makedirs(os.path.join(wd, "vel"))
+        three_layer(128, 128, 128, 40, 80).tofile(os.path.join(wd, "vel", "model.bin"))
+        run(wd, p)
+        path = os.path.join(wd, "dat", seism_name(p, shots(p)[0], 0))
+        raw = open(path, "rb").read()
+        rec = np.frombuffer(raw, np.float32).reshape(64, 64, 1000)
+        np.savez_compressed(os.path.join(HERE, "c0_modeling.npz"),
+                            sha256=np.frombuffer(hashlib.sha256(raw).digest(), np.uint8),
+                            trace_l2=np.sqrt((rec.astype(np.float64) ** 2).sum(axis=2)),
+                            sub=np.ascontiguousarray(rec[::8, ::8, ::4]))
+    finally:
+        shutil.rmtree(tmp)
+    path = os.path.join(HERE, "cases.json")
+    with open(path) as f:
+        cases = json.load(f)
+    cases["c0_modeling"] = p
+    with open(path, "w") as f:
+        json.dump(cases, f, indent=1, sort_keys=True)
+    print("fixture written: c0_modeling")
+
+
 def main():
     if not os.path.exists(REFBIN):
         raise SystemExit("build the reference first: make -C oracle ref")
     if sys.argv[1:] == ["fmig"]:
         return main_fmig()
+    if sys.argv[1:] == ["c0"]:
+        return main_c0()
     cases = {}
     tmp = tempfile.mkdtemp(prefix="rtmgold_")
     try:
