@@ -261,6 +261,23 @@ int rtmhip_batch_set_stencil(rtmhip_batch *batch, int kernel_class, int mode);
  * count (default 8). */
 int rtmhip_batch_set_graph(rtmhip_batch *batch, int steps);
 
+/* No reference counterpart.  For callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc): nsteps batched HBC steps of the shots described by `shots`
+ * alone, with no batch object, no graph and no device allocation -- forward
+ * (direction = +1, steps it .. it+nsteps-1, as rtmhip_batch_forward_steps)
+ * or backward (direction = -1, it down to it-nsteps+1, as
+ * rtmhip_batch_backward_steps).  Results are bit-identical to a batch
+ * created from the same shots and run over the same chunks.  On entry
+ * (fs[0], fs[1]) and (fr[0], fr[1]) are each pair's current (P, PP); on
+ * return they hold the new roles.  The shot table is staged in a fixed device
+ * array of the library holding at most 64 shots (one per GPU, kept until the
+ * process ends, holding the last call's table); more shots are rejected with
+ * RTMHIP_ERR_INVALID.  Calls run in order on the library stream; calls from
+ * several host threads are serialised. */
+int rtmhip_batch_steps_noalloc(rtmhip_batch_shot *shots, int nshots, const float *coefs,
+                               const float *taper, int direction, int it, int nsteps);
+
 /* ================= learned snapshot compression =================
  * No reference counterpart (the reference keeps the source field by HBC
  * reconstruction or by exact snapshots).  A 4-layer patch-convolution

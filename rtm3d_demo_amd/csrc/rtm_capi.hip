@@ -13,6 +13,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -20,8 +21,20 @@
 
 using namespace rtmhip;
 
+// Staging array of rtmhip_batch_steps_noalloc: the shot table of one call
+// (at most kNoAllocShots entries, 64 x sizeof(BatchShot) = about 12 KB).
+// Every GPU the library runs on holds its own copy until the process ends.
+// Host code never passes `scratch` itself to a kernel or a copy: its device
+// address on each GPU is looked up once, at rtmhip_init with that GPU
+// current, and kept in g_scratch.
+constexpr int kNoAllocShots = 64;
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) BatchShot scratch[kNoAllocShots];
+
 namespace {
 
+BatchShot *g_scratch[kMaxDevices] = {};  // device address of `scratch` per GPU
+std::mutex g_noalloc_mu;                  // guards the no-allocation host tables
 
 struct Timing {
     std::vector<hipEvent_t> start, stop;
@@ -275,6 +288,11 @@ int rtmhip_init(int device)
     if (device < 0 || device >= n)
         return fail(RTMHIP_ERR_INVALID, "device %d not present (%d devices)", device, n);
     HIPCHK(hipSetDevice(device), "hipSetDevice");
+    if (device < kMaxDevices && !g_scratch[device]) {
+        void *p = nullptr;
+        HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+        g_scratch[device] = (BatchShot *)p;
+    }
     HIPCHK(hipStreamCreateWithFlags(&g_state.stream, hipStreamNonBlocking), "hipStreamCreate");
     HIPCHK(hipMalloc((void **)&g_state.flag, sizeof(int)), "hipMalloc(flag)");
     HIPCHK(hipMemsetAsync(g_state.flag, 0, sizeof(int), g_state.stream), "hipMemset(flag)");
@@ -1265,6 +1283,50 @@ int rtmhip_batch_backward_steps(rtmhip_batch *b, int it_hi, int nsteps)
         using T = decltype(t);
         return batch_backward<T>(b, it_hi, nsteps);
     });
+}
+
+int rtmhip_batch_steps_noalloc(rtmhip_batch_shot *shots, int nshots, const float *coefs,
+                               const float *taper, int direction, int it, int nsteps)
+{
+    NEED_INIT();
+    if (!shots || !coefs || !taper) return fail(RTMHIP_ERR_INVALID, "NULL pointer");
+    if (nshots <= 0) return fail(RTMHIP_ERR_INVALID, "empty batch");
+    if (nshots > kNoAllocShots)
+        return fail(RTMHIP_ERR_INVALID,
+                    "rtmhip_batch_steps_noalloc: %d shots, the staging array holds at most %d",
+                    nshots, kNoAllocShots);
+    if (direction != 1 && direction != -1)
+        return fail(RTMHIP_ERR_INVALID, "direction must be +1 (forward) or -1 (backward)");
+    if (g_state.device < 0 || g_state.device >= kMaxDevices || !g_scratch[g_state.device])
+        return fail(RTMHIP_ERR_NOT_INIT, "no staging array on device %d", g_state.device);
+    std::lock_guard<std::mutex> lock(g_noalloc_mu);
+    static BatchShot hs[kNoAllocShots];
+    static rtmhip_shot sh[kNoAllocShots];
+    rtmhip_batch b;     // its containers stay empty: nothing is allocated
+    b.graph_steps = 0;  // eager steps (instantiating a graph allocates)
+    int rc = batch_fill(&b, shots, nshots, coefs, taper, hs, sh);
+    if (rc) return rc;
+    if (nsteps <= 0) return RTMHIP_OK;
+    if (direction > 0 ? (it < 0 || it + nsteps > b.nt) : (it >= b.nt || it - nsteps + 1 < 0))
+        return fail(RTMHIP_ERR_INVALID, "time range outside [0,%d)", b.nt);
+    BatchShot *d = g_scratch[g_state.device];
+    b.shots = sh;
+    b.hs[0] = b.hs[1] = hs;
+    b.ds[0] = b.ds[1] = d;
+    // stream-ordered after the previous call's kernels that read the array
+    HIPCHK(hipMemcpyAsync(d, hs, sizeof(BatchShot) * (size_t)nshots, hipMemcpyHostToDevice,
+                          g_state.stream),
+           "staging the shot table");
+    rc = with_format(&sh[0], [&](auto t) {
+        using T = decltype(t);
+        return direction > 0 ? batch_run<T>(&b, +1, it, nsteps) : batch_backward<T>(&b, it, nsteps);
+    });
+    if (rc) return rc;
+    for (int i = 0; i < nshots; ++i) {
+        if (b.cs) std::swap(shots[i].fs[0], shots[i].fs[1]);
+        if (b.cr) std::swap(shots[i].fr[0], shots[i].fr[1]);
+    }
+    return RTMHIP_OK;
 }
 
 }  // extern "C"

@@ -24,7 +24,7 @@ EXPORTED = [
     "rtmhip_set_tiling", "rtmhip_get_tiling", "rtmhip_timing_enable", "rtmhip_timing_read",
     "rtmhip_batch_create", "rtmhip_batch_destroy", "rtmhip_batch_forward_steps",
     "rtmhip_batch_backward_steps", "rtmhip_batch_fields", "rtmhip_batch_set_stencil",
-    "rtmhip_batch_set_graph",
+    "rtmhip_batch_set_graph", "rtmhip_batch_steps_noalloc",
     "rtmhip_ae_param_count", "rtmhip_ae_create", "rtmhip_ae_destroy", "rtmhip_ae_get_params",
     "rtmhip_ae_set_params", "rtmhip_ae_gradient", "rtmhip_ae_train", "rtmhip_ae_bricks",
     "rtmhip_ae_encode", "rtmhip_ae_decode", "rtmhip_ae_gemm_test", "rtmhip_freq_image",
@@ -104,6 +104,7 @@ def load():
         "rtmhip_batch_fields": [vp, i, pp, pp, pp, pp],
         "rtmhip_batch_set_stencil": [vp, i, i],
         "rtmhip_batch_set_graph": [vp, i],
+        "rtmhip_batch_steps_noalloc": [C.POINTER(BatchShotDesc), i, fp, fp, i, i, i],
         "rtmhip_freq_image": [fp] * 10 + [i] * 9,
         "rtmhip_ae_param_count": [],
         "rtmhip_ae_create": [pp, i, C.c_ulonglong],
@@ -380,3 +381,25 @@ class Batch:
         if self.h:
             check(load().rtmhip_batch_destroy(self.h))
             self.h = C.c_void_p()
+
+def batch_steps_noalloc(members, coefs, g, direction, it, nsteps):
+    """rtmhip_batch_steps_noalloc over `members` (the dicts of Batch): the
+    FieldPairs' current roles go in as (P, PP) and are updated from the
+    roles the call returns."""
+    n = len(members)
+    arr = (BatchShotDesc * n)()
+    for k, m in enumerate(members):
+        d = arr[k]
+        d.shot = m["shot"]
+        d.V2DT2 = m["V"].ptr.value
+        d.ricker = m["ricker"].ptr.value
+        d.fs[0], d.fs[1] = m["S"].p.value, m["S"].pp.value
+        d.fr[0], d.fr[1] = m["R"].p.value, m["R"].pp.value
+        d.upb = m["upb"].ptr.value
+        d.rcv = m["rcv"].ptr.value
+        d.img = m["img"].ptr.value
+    c = _f32(coefs)
+    check(load().rtmhip_batch_steps_noalloc(arr, n, _host_ptr(c), g.ptr, direction, it, nsteps))
+    for k, m in enumerate(members):
+        m["S"].p, m["S"].pp = C.c_void_p(arr[k].fs[0]), C.c_void_p(arr[k].fs[1])
+        m["R"].p, m["R"].pp = C.c_void_p(arr[k].fr[0]), C.c_void_p(arr[k].fr[1])
