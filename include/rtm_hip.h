@@ -207,6 +207,38 @@ int rtmhip_backward_steps(const rtmhip_shot *shot, int hbc, const float *V2DT2,
                           float **Pr, float **PPr, const float *upb, const float *rcv_chunk,
                           float *img, int it_hi, int nsteps, int fused);
 
+/* ================= snapshot RTM =================
+ * No reference counterpart as a loop: the reference recovers the source
+ * field for imaging by HBC reconstruction (rtmHBCBackward,
+ * RTMFiniteDifferencesKernel_HBC.cpp:261-354).  Here the forward pass stores
+ * the source wavefield of every k-th step -- the steps it = nt-1, nt-1-k,
+ * nt-1-2k, ... ("snapshot steps"; snapshot j is step nt-1-j*k) -- over the
+ * imaging range (all x, y; z in [blen, nze-blen)) as [j][nxe][nye][nze-2*blen]
+ * elements of snap_format, and the backward pass propagates only the
+ * receiver field and applies the imaging condition at the snapshot steps:
+ * img += k * (snapshot * PR).  With k = 1 and F32 snapshots this is the
+ * reference's imaging condition (RTMCPUPlatform.cpp:513-540) on the forward
+ * source field ("exact-snapshot RTM"); k > 1 images every k-th step, weighted
+ * so that the image approximates the every-step one.  The forward step
+ * sequence is rtmhip_forward_steps' with hbc = 1 minus the upper-border
+ * history; the receiver half of the backward pass is rtmhip_backward_steps'
+ * (hbc = 1).  Calls may split the time range into chunks like those loops.
+ * wave_format may be F32, F16 or BF16 (the snapshot format is independent). */
+#define RTMHIP_SNAP_F32 0  /* FP32 snapshots */
+#define RTMHIP_SNAP_F16 1  /* FP16 snapshots (nearest-even), FP32 imaging arithmetic */
+
+/* Size in bytes of the snapshot buffer of one shot. */
+int rtmhip_snap_buffer_bytes(const rtmhip_shot *shot, int snap_format, int k, size_t *bytes);
+int rtmhip_snap_forward_steps(const rtmhip_shot *shot, const float *V2DT2, const float *coefs,
+                              const float *taper, const float *ricker, float **P, float **PP,
+                              void *snaps, int snap_format, int k, int it0, int nsteps);
+/* it from it_hi down to it_hi-nsteps+1; *Pr, *PPr and img zero on the first
+ * call (it_hi = nt-1); rcv_chunk holds the chunk containing it_hi. */
+int rtmhip_snap_backward_steps(const rtmhip_shot *shot, const float *V2DT2, const float *coefs,
+                               const float *taper, float **Pr, float **PPr, const void *snaps,
+                               int snap_format, int k, const float *rcv_chunk, float *img,
+                               int it_hi, int nsteps);
+
 /* ================= batches of shots =================
  * No reference counterpart: the reference migrates one shot at a time per
  * process (rtm-3d/src/controller/RTMAcousticController.cpp:141-200).  A batch

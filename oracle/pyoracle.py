@@ -273,6 +273,57 @@ def migrate_shot(v2, coefs, g, src, blen, order, nt, ntstep, shot, geom, rcv, hb
     return img
 
 
+def snapshot_migrate_shot(v2, coefs, g, src, blen, order, nt, shot, geom, rcv, k,
+                          snap_fmt="f32", fmt=None):
+    """Snapshot RTM of one shot (no reference counterpart as a loop): the HBC
+    forward step sequence of orc_migrate_shot (upper tapers, step, source)
+    keeping the source wavefield of the steps nt-1, nt-1-k, ... over the
+    imaging range z in [blen, nze-blen) -- rounded to FP16 (nearest-even)
+    for snap_fmt='f16' -- instead of the upper-border history; then the
+    receiver half of its backward loop (upper tapers, step, receivers set)
+    with img += k * (snapshot * PR) at those steps, in the arithmetic type of
+    v2 (numpy, that operation order).  rcv: [count_x][count_y][nt].
+    fmt: wavefield storage format ('f16' / 'bf16' with float32 v2)."""
+    dt = v2.dtype
+    nxe, nye, nze = v2.shape
+    wdt = np.uint16 if fmt in ("f16", "bf16") else dt
+    sx, sy, sz = shot
+    z0, z1 = blen, nze - blen
+
+    def widen(a):
+        return from_format(a, fmt) if wdt == np.uint16 else np.array(a, dt)
+
+    P, PP = np.zeros(v2.shape, wdt), np.zeros(v2.shape, wdt)
+    snaps = {}
+    for it in range(nt):
+        P, PP = PP, P
+        taper(P, g, blen, True, fmt)
+        taper(PP, g, blen, True, fmt)
+        if step(P, PP, v2, coefs, order, fmt) != 0:
+            raise RuntimeError("oracle step produced a NaN")
+        apply_source(PP, sx + blen, sy + blen, sz + blen, src[it], fmt)
+        if (nt - 1 - it) % k == 0:
+            a = widen(PP[:, :, z0:z1])
+            if snap_fmt == "f16":
+                a = a.astype(np.float16).astype(dt)
+            snaps[it] = a
+    img = np.zeros(v2.shape, dt)
+    Pr, PPr = np.zeros(v2.shape, wdt), np.zeros(v2.shape, wdt)
+    rcv = np.ascontiguousarray(rcv, dt)
+    w = dt.type(k)
+    for it in range(nt - 1, -1, -1):
+        taper(Pr, g, blen, True, fmt)
+        taper(PPr, g, blen, True, fmt)
+        if step(Pr, PPr, v2, coefs, order, fmt) != 0:
+            raise RuntimeError("oracle step produced a NaN")
+        receivers(PPr, rcv, blen, geom, nt, it, True, fmt)
+        if it in snaps:
+            v = snaps[it] * widen(PPr[:, :, z0:z1])
+            img[:, :, z0:z1] = img[:, :, z0:z1] + w * v
+        Pr, PPr = PPr, Pr
+    return img
+
+
 def freq_tables(nt, dt, fmin, fmax):
     """(w2[nw], kre[nw][nt], kim[nw][nt]) of the frequency-domain migration."""
     nw = lib().orc_freq_tables(nt, dt, fmin, fmax, None, None, None, 0)

@@ -691,6 +691,91 @@ int backward_loop(const rtmhip_shot *shot, const ShotCtx &c, int hbc, const floa
     return RTMHIP_OK;
 }
 
+// Snapshot slot of time step it, or -1: the snapshot steps are nt-1,
+// nt-1-k, nt-1-2k, ... (slot 0 is the last step, imaged first).
+inline int snap_slot(int it, int nt, int k)
+{
+    const int d = nt - 1 - it;
+    return d % k == 0 ? d / k : -1;
+}
+
+inline size_t snap_cells(const rtmhip_shot *shot)
+{
+    return (size_t)shot->nxe * shot->nye * (size_t)(shot->nze - 2 * shot->blen);
+}
+
+// Forward propagation of snapshot RTM: the HBC forward step sequence of
+// forward_loop (upper tapers fused, frame taper, source) with the source
+// wavefield of every snapshot step stored over the imaging range instead of
+// the upper-border history.
+template <typename T, typename S>
+int snap_forward_loop(const rtmhip_shot *shot, const ShotCtx &c, const float *V2DT2,
+                      const float *taper, const float *ricker, T **P, T **PP, S *snaps, int k,
+                      int it0, int nsteps)
+{
+    hipStream_t s = g_state.stream;
+    const int b = shot->blen;
+    const size_t per = snap_cells(shot);
+    for (int it = it0; it < it0 + nsteps; ++it) {
+        T *t = *P;
+        *P = *PP;
+        *PP = t;
+        const TaperFuse tf{taper, b, it == it0 ? 1 : 2};
+        HIPCHK(stencil(c.radius, *P, *PP, V2DT2, c.cf, c.g, nullptr, &tf), "stencil");
+        HIPCHK(launch_taper_frame(*PP, taper, c.g, b, c.radius, tf.pp_taps, s), "frame taper");
+        HIPCHK(launch_source(*PP, c.src_off, ricker, it, 0.f, s), "source");
+        const int j = snap_slot(it, shot->nt, k);
+        if (j >= 0)
+            HIPCHK(launch_snap_store(*PP, snaps + (size_t)j * per, c.g, b, s), "snapshot");
+    }
+    HIPCHK(launch_taper(*P, taper, c.g, b, true, s), "taper");
+    return RTMHIP_OK;
+}
+
+// Backward pass of snapshot RTM: only the receiver field propagates (the
+// receiver half of backward_loop: upper tapers fused, frame taper, receiver
+// injection); at every snapshot step img += k * (snapshot * PR).
+template <typename T, typename S>
+int snap_backward_loop(const rtmhip_shot *shot, const ShotCtx &c, const float *V2DT2,
+                       const float *taper, T **Pr, T **PPr, const S *snaps, int k,
+                       const float *rcv_chunk, float *img, int it_hi, int nsteps)
+{
+    hipStream_t s = g_state.stream;
+    const int b = shot->blen;
+    const size_t per = snap_cells(shot);
+    for (int it = it_hi; it > it_hi - nsteps; --it) {
+        const TaperFuse tf{taper, b, it == it_hi ? 1 : 2};
+        HIPCHK(stencil(c.radius, *Pr, *PPr, V2DT2, c.cf, c.g, nullptr, &tf), "stencil");
+        HIPCHK(launch_taper_frame(*PPr, taper, c.g, b, c.radius, tf.pp_taps, s), "frame taper");
+        HIPCHK(launch_receivers(*PPr, const_cast<float *>(rcv_chunk), c.g, c.rx0, c.ry0, c.rz,
+                                shot->rcv.dist_x, shot->rcv.dist_y, shot->rcv.count_x,
+                                shot->rcv.count_y, shot->ntstep, it % shot->ntstep, true, s),
+               "receivers");
+        const int j = snap_slot(it, shot->nt, k);
+        if (j >= 0)
+            HIPCHK(launch_snap_image(img, snaps + (size_t)j * per, (const T *)*PPr, c.g, b,
+                                     (float)k, s),
+                   "snapshot image");
+        T *t = *Pr;
+        *Pr = *PPr;
+        *PPr = t;
+    }
+    HIPCHK(launch_taper(*PPr, taper, c.g, b, true, s), "taper");
+    return RTMHIP_OK;
+}
+
+int check_snap(const rtmhip_shot *shot, const void *snaps, int snap_format, int k)
+{
+    if (!snaps) return fail(RTMHIP_ERR_INVALID, "NULL snapshot buffer");
+    if (snap_format != RTMHIP_SNAP_F32 && snap_format != RTMHIP_SNAP_F16)
+        return fail(RTMHIP_ERR_UNSUPPORTED, "snapshot format %d (F32 or F16)", snap_format);
+    if (k < 1 || k > shot->nt)
+        return fail(RTMHIP_ERR_INVALID, "snapshot interval %d outside [1, nt]", k);
+    if ((long long)shot->nxe * shot->nye >= (1LL << 31))
+        return fail(RTMHIP_ERR_INVALID, "nxe*nye must stay below 2^31");
+    return RTMHIP_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -733,6 +818,70 @@ int rtmhip_forward_steps(const rtmhip_shot *shot, int hbc, const float *V2DT2,
         using T = decltype(t);
         return forward_loop<T>(shot, c, hbc, V2DT2, taper, ricker, as<T>(P), as<T>(PP),
                                reinterpret_cast<T *>(upb), it0, nsteps);
+    });
+}
+
+int rtmhip_snap_buffer_bytes(const rtmhip_shot *shot, int snap_format, int k, size_t *bytes)
+{
+    if (!shot || !bytes) return fail(RTMHIP_ERR_INVALID, "NULL pointer");
+    if (shot->nt <= 0 || shot->nze <= 2 * shot->blen)
+        return fail(RTMHIP_ERR_INVALID, "bad shot geometry");
+    if (k < 1) return fail(RTMHIP_ERR_INVALID, "snapshot interval %d below 1", k);
+    const size_t es = snap_format == RTMHIP_SNAP_F32 ? 4 : snap_format == RTMHIP_SNAP_F16 ? 2 : 0;
+    if (!es) return fail(RTMHIP_ERR_UNSUPPORTED, "snapshot format %d (F32 or F16)", snap_format);
+    *bytes = (size_t)((shot->nt - 1) / k + 1) * snap_cells(shot) * es;
+    return RTMHIP_OK;
+}
+
+int rtmhip_snap_forward_steps(const rtmhip_shot *shot, const float *V2DT2, const float *coefs,
+                              const float *taper, const float *ricker, float **P, float **PP,
+                              void *snaps, int snap_format, int k, int it0, int nsteps)
+{
+    NEED_INIT();
+    ShotCtx c;
+    int rc;
+    if ((rc = prepare_shot(shot, coefs, &c))) return rc;
+    if ((rc = check_format(shot))) return rc;
+    if (!V2DT2 || !ricker || !taper || !P || !PP || !*P || !*PP)
+        return fail(RTMHIP_ERR_INVALID, "NULL pointer");
+    if ((rc = check_snap(shot, snaps, snap_format, k))) return rc;
+    if (nsteps <= 0) return RTMHIP_OK;
+    if (it0 < 0 || it0 + nsteps > shot->nt)
+        return fail(RTMHIP_ERR_INVALID, "time range outside [0,%d)", shot->nt);
+    return with_format(shot, [&](auto t) {
+        using T = decltype(t);
+        if (snap_format == RTMHIP_SNAP_F32)
+            return snap_forward_loop<T, float>(shot, c, V2DT2, taper, ricker, as<T>(P), as<T>(PP),
+                                               (float *)snaps, k, it0, nsteps);
+        return snap_forward_loop<T, _Float16>(shot, c, V2DT2, taper, ricker, as<T>(P), as<T>(PP),
+                                              (_Float16 *)snaps, k, it0, nsteps);
+    });
+}
+
+int rtmhip_snap_backward_steps(const rtmhip_shot *shot, const float *V2DT2, const float *coefs,
+                               const float *taper, float **Pr, float **PPr, const void *snaps,
+                               int snap_format, int k, const float *rcv_chunk, float *img,
+                               int it_hi, int nsteps)
+{
+    NEED_INIT();
+    ShotCtx c;
+    int rc;
+    if ((rc = prepare_shot(shot, coefs, &c))) return rc;
+    if ((rc = check_format(shot))) return rc;
+    if (!V2DT2 || !taper || !Pr || !PPr || !*Pr || !*PPr || !rcv_chunk || !img)
+        return fail(RTMHIP_ERR_INVALID, "NULL pointer");
+    if ((rc = check_snap(shot, snaps, snap_format, k))) return rc;
+    if (nsteps <= 0) return RTMHIP_OK;
+    if ((rc = check_chunk(shot, it_hi - nsteps + 1, it_hi))) return rc;
+    return with_format(shot, [&](auto t) {
+        using T = decltype(t);
+        if (snap_format == RTMHIP_SNAP_F32)
+            return snap_backward_loop<T, float>(shot, c, V2DT2, taper, as<T>(Pr), as<T>(PPr),
+                                                (const float *)snaps, k, rcv_chunk, img, it_hi,
+                                                nsteps);
+        return snap_backward_loop<T, _Float16>(shot, c, V2DT2, taper, as<T>(Pr), as<T>(PPr),
+                                               (const _Float16 *)snaps, k, rcv_chunk, img, it_hi,
+                                               nsteps);
     });
 }
 

@@ -122,6 +122,14 @@ hipError_t launch_stencil_dual(int radius, const float *Ps, float *PPs, const fl
 template <typename T>
 hipError_t launch_taper(T *P, const float *taper, const Grid &g, int blen, bool upper_only,
                         hipStream_t s);
+// Snapshots over the imaging range z in [b, nz-b) ([nx][ny][nz-2b], element
+// type S: float or _Float16), and the imaging condition from one:
+// img += w * (snap * PR) on that range.
+template <typename T, typename S>
+hipError_t launch_snap_store(const T *P, S *snap, const Grid &g, int b, hipStream_t s);
+template <typename T, typename S>
+hipError_t launch_snap_image(float *img, const S *snap, const T *PR, const Grid &g, int b,
+                             float w, hipStream_t s);
 // taper3D of two grids of the same shape in one pass (Q may be NULL: one grid).
 template <typename T>
 hipError_t launch_taper_all2(T *P, T *Q, const float *taper, const Grid &g, int b,

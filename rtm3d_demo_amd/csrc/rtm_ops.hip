@@ -219,6 +219,49 @@ __global__ void __launch_bounds__(kThreads)
     }
 }
 
+// Snapshot of a wavefield over the imaging range (all x, y; z in [b, nz-b)):
+// snap is [nx][ny][nz-2b] of element type S (float, or _Float16 rounded to
+// nearest-even).  One wave per (x, y) row, z on the lanes.
+template <typename T, typename S>
+__global__ void __launch_bounds__(kThreads)
+    k_snap_store(const T *__restrict__ P, S *__restrict__ snap, Grid g, int b)
+{
+    const int lane = threadIdx.x & 63;
+    const int row = __builtin_amdgcn_readfirstlane((int)blockIdx.x * (kThreads / 64) +
+                                                   (int)(threadIdx.x >> 6));
+    if (row >= g.nx * g.ny) return;
+    const int x = row / g.ny;
+    const int y = row - x * g.ny;
+    const int lz = g.nz - 2 * b;
+    const T *__restrict__ r = P + ((long long)x * g.sx + (long long)y * g.sy + b);
+    S *__restrict__ o = snap + (long long)row * lz;
+    for (int z = lane; z < lz; z += 64) o[z] = wst<S>(wld(r[z]));
+}
+
+// Imaging condition from a snapshot: img += w * (snap * PR) on the imaging
+// range (the reference's img += PS * PR, RTMCPUPlatform.cpp:513-540, with PS
+// read from the snapshot and weighted by the snapshot interval w; w = 1
+// leaves the sum bit-identical to img + PS * PR).
+template <typename T, typename S>
+__global__ void __launch_bounds__(kThreads)
+    k_snap_image(float *__restrict__ img, const S *__restrict__ snap, const T *__restrict__ PR,
+                 Grid g, int b, float w)
+{
+    const int lane = threadIdx.x & 63;
+    const int row = __builtin_amdgcn_readfirstlane((int)blockIdx.x * (kThreads / 64) +
+                                                   (int)(threadIdx.x >> 6));
+    if (row >= g.nx * g.ny) return;
+    const int x = row / g.ny;
+    const int y = row - x * g.ny;
+    const int lz = g.nz - 2 * b;
+    const long long o = (long long)x * g.sx + (long long)y * g.sy + b;
+    const S *__restrict__ sr = snap + (long long)row * lz;
+    for (int z = lane; z < lz; z += 64) {
+        const float v = wld(sr[z]) * wld(PR[o + z]);
+        img[o + z] = img[o + z] + w * v;
+    }
+}
+
 // rtmApplySource (RTMCPUPlatform.cpp:103-113): PP[s] += value (or table[it]).
 template <typename T>
 __global__ void k_source(T *__restrict__ PP, long long o, const float *__restrict__ table, int it,
@@ -692,6 +735,29 @@ hipError_t launch_image(float *img, const T *PS, const T *PR, const Grid &g, int
     return hipGetLastError();
 }
 
+template <typename T, typename S>
+hipError_t launch_snap_store(const T *P, S *snap, const Grid &g, int b, hipStream_t s)
+{
+    if (g.nz <= 2 * b) return hipSuccess;
+    if ((long long)g.nx * g.ny >= (1LL << 31)) return hipErrorInvalidValue;
+    const long long rows = (long long)g.nx * g.ny;
+    hipLaunchKernelGGL((k_snap_store<T, S>), dim3((unsigned)((rows + 3) / 4)), dim3(kThreads), 0,
+                       s, P, snap, g, b);
+    return hipGetLastError();
+}
+
+template <typename T, typename S>
+hipError_t launch_snap_image(float *img, const S *snap, const T *PR, const Grid &g, int b,
+                             float w, hipStream_t s)
+{
+    if (g.nz <= 2 * b) return hipSuccess;
+    if ((long long)g.nx * g.ny >= (1LL << 31)) return hipErrorInvalidValue;
+    const long long rows = (long long)g.nx * g.ny;
+    hipLaunchKernelGGL((k_snap_image<T, S>), dim3((unsigned)((rows + 3) / 4)), dim3(kThreads), 0,
+                       s, img, snap, PR, g, b, w);
+    return hipGetLastError();
+}
+
 hipError_t launch_image_finish(const float *img, float *out, const Grid &g, int b,
                                const Coefs &fc, hipStream_t s)
 {
@@ -806,5 +872,16 @@ hipError_t launch_tick(int *p, int d, bool set, hipStream_t s)
 RTM_INSTANTIATE_OPS(float)
 RTM_INSTANTIATE_OPS(_Float16)
 RTM_INSTANTIATE_OPS(__bf16)
+
+#define RTM_INSTANTIATE_SNAP(T, S)                                                            \
+    template hipError_t launch_snap_store<T, S>(const T *, S *, const Grid &, int, hipStream_t); \
+    template hipError_t launch_snap_image<T, S>(float *, const S *, const T *, const Grid &, int, \
+                                                float, hipStream_t);
+RTM_INSTANTIATE_SNAP(float, float)
+RTM_INSTANTIATE_SNAP(float, _Float16)
+RTM_INSTANTIATE_SNAP(_Float16, float)
+RTM_INSTANTIATE_SNAP(_Float16, _Float16)
+RTM_INSTANTIATE_SNAP(__bf16, float)
+RTM_INSTANTIATE_SNAP(__bf16, _Float16)
 
 }  // namespace rtmhip

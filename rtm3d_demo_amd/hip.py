@@ -21,6 +21,7 @@ EXPORTED = [
     "rtmhip_step", "rtmhip_taper", "rtmhip_apply_source", "rtmhip_receivers",
     "rtmhip_upper_border", "rtmhip_image_condition", "rtmhip_image_finish", "rtmhip_stack",
     "rtmhip_model_steps", "rtmhip_forward_steps", "rtmhip_backward_steps",
+    "rtmhip_snap_buffer_bytes", "rtmhip_snap_forward_steps", "rtmhip_snap_backward_steps",
     "rtmhip_set_tiling", "rtmhip_get_tiling", "rtmhip_timing_enable", "rtmhip_timing_read",
     "rtmhip_batch_create", "rtmhip_batch_destroy", "rtmhip_batch_forward_steps",
     "rtmhip_batch_backward_steps", "rtmhip_batch_fields", "rtmhip_batch_set_stencil",
@@ -93,6 +94,10 @@ def load():
         "rtmhip_forward_steps": [C.POINTER(Shot), i, fp, fp, fp, fp, pp, pp, fp, i, i],
         "rtmhip_backward_steps": [C.POINTER(Shot), i, fp, fp, fp, pp, pp, pp, pp, fp, fp, fp,
                                   i, i, i],
+        "rtmhip_snap_buffer_bytes": [C.POINTER(Shot), i, i, C.POINTER(C.c_size_t)],
+        "rtmhip_snap_forward_steps": [C.POINTER(Shot), fp, fp, fp, fp, pp, pp, vp, i, i, i, i],
+        "rtmhip_snap_backward_steps": [C.POINTER(Shot), fp, fp, fp, pp, pp, vp, i, i, fp, fp, i,
+                                       i],
         "rtmhip_set_tiling": [i, i, i, i, i],
         "rtmhip_get_tiling": [i] + [C.POINTER(C.c_int)] * 4,
         "rtmhip_timing_enable": [i],
@@ -306,6 +311,37 @@ def backward_steps(shot, hbc, V, coefs, g, src_fields, rcv_fields, upb, rcv_chun
                                        img.ptr, it_hi, nsteps, int(fused)))
 
 
+# ---------------- snapshot RTM ----------------
+# Snapshot formats (RTMHIP_SNAP_*): name -> (code, numpy storage dtype).
+SNAP_FORMATS = {"f32": (0, np.float32), "f16": (1, np.float16)}
+
+
+def snap_buffer(shot, fmt, k):
+    """A device buffer for one shot's snapshots ([nsnap][nxe][nye][nze-2blen])."""
+    n = C.c_size_t(0)
+    check(load().rtmhip_snap_buffer_bytes(C.byref(shot), SNAP_FORMATS[fmt][0], k, C.byref(n)))
+    cells = shot.nxe * shot.nye * (shot.nze - 2 * shot.blen)
+    dt = np.dtype(SNAP_FORMATS[fmt][1])
+    return DeviceArray((n.value // (cells * dt.itemsize), shot.nxe, shot.nye,
+                        shot.nze - 2 * shot.blen), dt)
+
+
+def snap_forward_steps(shot, V, coefs, g, ricker, fields, snaps, fmt, k, it0, nsteps):
+    c = _f32(coefs)
+    check(load().rtmhip_snap_forward_steps(C.byref(shot), V.ptr, _host_ptr(c), g.ptr, ricker.ptr,
+                                           C.byref(fields.p), C.byref(fields.pp), snaps.ptr,
+                                           SNAP_FORMATS[fmt][0], k, it0, nsteps))
+
+
+def snap_backward_steps(shot, V, coefs, g, rcv_fields, snaps, fmt, k, rcv_chunk, img, it_hi,
+                        nsteps):
+    c = _f32(coefs)
+    check(load().rtmhip_snap_backward_steps(C.byref(shot), V.ptr, _host_ptr(c), g.ptr,
+                                            C.byref(rcv_fields.p), C.byref(rcv_fields.pp),
+                                            snaps.ptr, SNAP_FORMATS[fmt][0], k, rcv_chunk.ptr,
+                                            img.ptr, it_hi, nsteps))
+
+
 def set_tiling(kernel_class, tz=0, by=0, ry=0, xchunk=0):
     check(load().rtmhip_set_tiling(kernel_class, tz, by, ry, xchunk))
 
@@ -381,6 +417,7 @@ class Batch:
         if self.h:
             check(load().rtmhip_batch_destroy(self.h))
             self.h = C.c_void_p()
+
 
 def batch_steps_noalloc(members, coefs, g, direction, it, nsteps):
     """rtmhip_batch_steps_noalloc over `members` (the dicts of Batch): the

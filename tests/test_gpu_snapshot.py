@@ -1,0 +1,78 @@
+"""Snapshot RTM through the C ABI (rtmhip_snap_forward_steps /
+rtmhip_snap_backward_steps) against the oracle's snapshot RTM
+(pyoracle.snapshot_migrate_shot), bit for bit, for every wavefield format,
+both snapshot formats and several snapshot intervals, with both passes split
+into chunks.  The border is odd (11), so the upper taper's edge and the
+imaging range start inside a z pair of the two-z kernels.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+B, NT = 11, 26
+GRID = (36, 42, 70)
+SRC = (17, 19, 2)                      # interior coordinates
+RCV = (1, 2, 3, 3, 4, 4, 5)           # start x, y, depth, dist x, y, count x, y
+
+
+def _inputs(seed=3):
+    rng = np.random.default_rng(seed)
+    V = rng.uniform(0.0, 0.15, GRID).astype(np.float32)
+    coefs = np.zeros(27, np.float32)
+    c8 = np.array([-1 / 560, 8 / 315, -1 / 5, 8 / 5, -205 / 72, 8 / 5, -1 / 5, 8 / 315, -1 / 560])
+    for d, h in enumerate((10.0, 11.0, 9.0)):
+        coefs[d * 9:(d + 1) * 9] = (c8 / (h * h)).astype(np.float32)
+    g = np.exp(-(np.sqrt(-np.log(0.9)) / B * np.arange(B)) ** 2).astype(np.float32)
+    ricker = rng.uniform(-1, 1, NT).astype(np.float32)
+    rcv = rng.uniform(-1, 1, (RCV[5], RCV[6], NT)).astype(np.float32)
+    return V, coefs, g, ricker, rcv
+
+
+@pytest.mark.parametrize("sfmt, k", [("f32", 1), ("f16", 1), ("f32", 3), ("f16", 4)])
+@pytest.mark.parametrize("wfmt", ["f32", "f16", "bf16"])
+def test_snapshot_rtm_matches_oracle(gpu, oracle, wfmt, sfmt, k):
+    print("snapshot rtm", wfmt, sfmt, k, flush=True)
+    V, coefs, g, ricker, rcv = _inputs()
+    wd = gpu.WAVE_FORMATS[wfmt][1]
+    shot = gpu.make_shot(*GRID, B, 8, NT, NT, tuple(c + B for c in SRC), gpu.RcvGeom(*RCV), wfmt)
+    dV, dg, dsrc, drcv = (gpu.DeviceArray.from_host(a) for a in (V, g, ricker, rcv))
+    S, R = gpu.FieldPair(GRID, wd), gpu.FieldPair(GRID, wd)
+    img = gpu.DeviceArray(GRID).zero()
+    snaps = gpu.snap_buffer(shot, sfmt, k)
+    assert snaps.shape[0] == (NT - 1) // k + 1
+    try:
+        for it0, n in ((0, 11), (11, NT - 11)):
+            gpu.snap_forward_steps(shot, dV, coefs, dg, dsrc, S, snaps, sfmt, k, it0, n)
+        for hi, n in ((NT - 1, 12), (NT - 13, NT - 12)):
+            gpu.snap_backward_steps(shot, dV, coefs, dg, R, snaps, sfmt, k, drcv, img, hi, n)
+        got = img.download()
+    finally:
+        for d in (dV, dg, dsrc, drcv, S, R, img, snaps):
+            d.free()
+    want = oracle.snapshot_migrate_shot(V, coefs, g, ricker, B, 8, NT, SRC,
+                                        oracle.RcvGeom(*RCV), rcv, k, sfmt,
+                                        None if wfmt == "f32" else wfmt)
+    assert np.abs(want).max() > 0
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+
+
+def test_snapshot_rejects_bad_arguments(gpu):
+    V, coefs, g, ricker, rcv = _inputs()
+    shot = gpu.make_shot(*GRID, B, 8, NT, NT, tuple(c + B for c in SRC), gpu.RcvGeom(*RCV))
+    dV, dg, dsrc = (gpu.DeviceArray.from_host(a) for a in (V, g, ricker))
+    S = gpu.FieldPair(GRID)
+    snaps = gpu.snap_buffer(shot, "f32", 2)
+    try:
+        with pytest.raises(gpu.RtmError, match="interval"):
+            gpu.snap_forward_steps(shot, dV, coefs, dg, dsrc, S, snaps, "f32", 0, 0, 1)
+        with pytest.raises(gpu.RtmError, match="time range"):
+            gpu.snap_forward_steps(shot, dV, coefs, dg, dsrc, S, snaps, "f32", 2, NT - 1, 2)
+        c = np.ascontiguousarray(coefs)
+        rc = gpu.load().rtmhip_snap_forward_steps(
+            gpu.C.byref(shot), dV.ptr, c.ctypes.data_as(gpu.C.c_void_p), dg.ptr, dsrc.ptr,
+            gpu.C.byref(S.p), gpu.C.byref(S.pp), snaps.ptr, 7, 2, 0, 1)
+        assert rc == gpu.ERR_UNSUPPORTED
+    finally:
+        for d in (dV, dg, dsrc, S, snaps):
+            d.free()
