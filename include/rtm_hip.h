@@ -392,6 +392,18 @@ int rtmhip_ae_gemm_test(const void *A, long long sam, long long sak, int a_half,
 int rtmhip_set_tiling(int kernel_class, int tz, int by, int ry, int xchunk);
 int rtmhip_get_tiling(int kernel_class, int *tz, int *by, int *ry, int *xchunk);
 
+/* How the HBC upper tapers (taperUpper3D of both fields before each step,
+ * RTMFiniteDifferencesKernel_HBC.cpp:160-165 and :311-316) reach the single-shot
+ * loops (rtmhip_forward_steps, rtmhip_backward_steps with fused = 1,
+ * rtmhip_snap_*) for one wave format: mode 1 fuses them into the stencil
+ * (tapered on load; a tapered kernel holds more registers), mode 0 runs them
+ * as one in-place pass over both fields' upper slabs followed by the
+ * untapered stencil, mode -1 restores the default (fused for F32, separate
+ * for F16 and BF16).  Results are identical in both modes.  Batches and
+ * fused = 2 always fuse. */
+int rtmhip_set_taper_fusion(int wave_format, int mode);
+int rtmhip_get_taper_fusion(int wave_format, int *fused);
+
 /* When enabled, every launch of the stencil kernels is bracketed by HIP
  * events on the library stream; rtmhip_timing_read() synchronises and returns
  * the number of launches and their summed duration for one kernel class,

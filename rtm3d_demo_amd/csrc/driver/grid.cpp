@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <cctype>
 #include <cmath>
+#include <cstdint>
 #include <cstdlib>
 #include <cstring>
 #include <fstream>
@@ -66,14 +67,49 @@ void borders_abc(Cube &e, int b)
 
 namespace {
 
-// randlinearval (rtm-3d/src/misc/Misc.cpp:91-102) on libc rand().
-float randlin(float v, int k, int nkb)
+// The sequence of libc rand() after srand(seed) (glibc's default additive
+// feedback generator, "TYPE_3": 31 words seeded by the Park-Miller minimal
+// standard generator, 310 outputs discarded, then o = (r[n-31] + r[n-3]) >> 1
+// modulo 2^32), with its own state: the reference's border fill calls rand()
+// and so depends on this exact sequence, while a private state lets several
+// grids be filled at once in one process.
+class LibcRand {
+  public:
+    explicit LibcRand(unsigned seed)
+    {
+        int32_t w = seed ? (int32_t)seed : 1;
+        ring_[0] = w;
+        for (int i = 1; i < 31; ++i) {
+            // 16807 * w mod (2^31 - 1) without overflow (Schrage)
+            const int32_t q = w / 127773, r = w % 127773;
+            w = 16807 * r - 2836 * q;
+            if (w < 0) w += 2147483647;
+            ring_[i] = w;
+        }
+        for (int i = 0; i < 310; ++i) next();
+    }
+    int next()
+    {
+        // ring_[k] holds r[n-31+k mod 31]; the newest value replaces r[n-31]
+        const uint32_t v = (uint32_t)ring_[head_] + (uint32_t)ring_[(head_ + 28) % 31];
+        ring_[head_] = (int32_t)v;
+        head_ = (head_ + 1) % 31;
+        return (int)(v >> 1);
+    }
+
+  private:
+    int32_t ring_[31];
+    int head_ = 3;  // the front word; the rear one is three behind
+};
+
+// randlinearval (rtm-3d/src/misc/Misc.cpp:91-102) on the libc rand() sequence.
+float rand_linear(LibcRand &rng, float v, int k, int nkb)
 {
     float vrand = v;
     const float dist = (float)(k * 1.0);
     const float range = (float)(nkb * 1.0);
     const float perc = (dist / range) * 100;
-    const float seed = (float)((std::rand() % 100) * 1.0);
+    const float seed = (float)((rng.next() % 100) * 1.0);
     if (seed < perc && perc > 70.)
         vrand = (float)((double)v * (1.0 / (double)perc) + (double)v * ((double)seed / 100.0));
     return vrand;
@@ -87,7 +123,8 @@ float randlin(float v, int k, int nkb)
 // reference's nthreads = 1 output.
 void borders_rbc(Cube &e, int b)
 {
-    std::srand(1);
+    LibcRand rng(1);  // srand(1): the sequence of a fresh process
+    auto randlin = [&rng](float v, int k, int nkb) { return rand_linear(rng, v, k, nkb); };
     const int nx = e.nx, ny = e.ny, nz = e.nz;
     for (int ix = b; ix < nx - b; ix++)
         for (int iy = b; iy < ny - b; iy++)

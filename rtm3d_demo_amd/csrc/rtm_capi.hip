@@ -50,7 +50,11 @@ struct State {
     Timing tm[4];
     StencilTiling tiling[4] = {};  // per kernel class; tz == 0: built-in default
     int tiling_gen = 0;            // bumped by every rtmhip_set_tiling call
+    // per wave format: the HBC upper tapers of the single-shot loops applied
+    // in place by a separate pass (true) or fused into the stencil (false)
+    bool taper_sep[3] = {false, true, true};
 };
+constexpr bool kTaperSepDefault[3] = {false, true, true};
 
 State g_state;
 thread_local std::string g_err;
@@ -553,11 +557,17 @@ int forward_loop(const rtmhip_shot *shot, const ShotCtx &c, int hbc, const float
     // first step of a call, whose PP already carries one taper).  The taper
     // still pending on P is applied in place after the last step, so the
     // state between calls is the reference's.
+    // With the separate-taper option the two tapers run in place (one pass
+    // over both fields) before a plain step, as the reference orders them.
+    const bool sep = g_state.taper_sep[shot->wave_format];
     for (int it = it0; it < it0 + nsteps; ++it) {
         T *t = *P;
         *P = *PP;
         *PP = t;
-        if (hbc) {
+        if (hbc && sep) {
+            HIPCHK(launch_taper_upper2(*P, *PP, taper, c.g, shot->blen, s), "taper");
+            HIPCHK(stencil(c.radius, *P, *PP, V2DT2, c.cf, c.g, nullptr), "stencil");
+        } else if (hbc) {
             const TaperFuse tf{taper, shot->blen, it == it0 ? 1 : 2};
             HIPCHK(stencil(c.radius, *P, *PP, V2DT2, c.cf, c.g, nullptr, &tf), "stencil");
             HIPCHK(launch_taper_frame(*PP, taper, c.g, shot->blen, c.radius, tf.pp_taps, s),
@@ -568,7 +578,7 @@ int forward_loop(const rtmhip_shot *shot, const ShotCtx &c, int hbc, const float
         HIPCHK(launch_source(*PP, c.src_off, ricker, it, 0.f, s), "source");
         if (hbc) HIPCHK(launch_upb(*PP, upb, c.g, shot->blen, c.hf, it, false, s), "upb save");
     }
-    if (hbc) HIPCHK(launch_taper(*P, taper, c.g, shot->blen, true, s), "taper");
+    if (hbc && !sep) HIPCHK(launch_taper(*P, taper, c.g, shot->blen, true, s), "taper");
     return RTMHIP_OK;
 }
 
@@ -582,6 +592,9 @@ int backward_loop(const rtmhip_shot *shot, const ShotCtx &c, int hbc, const floa
     const int z0 = b, z1 = shot->nze - b;
     const bool dual = fused == 2 && std::is_same<T, float>::value &&
                       dual_step_supported(c.radius, shot->wave_format);
+    // separate in-place receiver tapers before an untapered fused step (not
+    // with the dual kernel, which fuses them)
+    const bool sep = hbc && fused && !dual && g_state.taper_sep[shot->wave_format];
     for (int it = it_hi; it > it_hi - nsteps; --it) {
         const int lt = it % shot->ntstep;
         if constexpr (std::is_same<T, float>::value) {
@@ -645,12 +658,11 @@ int backward_loop(const rtmhip_shot *shot, const ShotCtx &c, int hbc, const floa
             S = *Ps;
         }
         // Receiver wavefield and imaging (:311-345).
-        if (hbc && !fused) {
-            HIPCHK(launch_taper(*Pr, taper, c.g, b, true, s), "taper");
-            HIPCHK(launch_taper(*PPr, taper, c.g, b, true, s), "taper");
-        }
+        if (hbc && !fused) HIPCHK(launch_taper_upper2(*Pr, *PPr, taper, c.g, b, s), "taper");
         if (fused) {
-            // tapers fused as in the forward loop; finishing taper below
+            // tapers fused as in the forward loop (finishing taper below), or
+            // applied in place first
+            if (sep) HIPCHK(launch_taper_upper2(*Pr, *PPr, taper, c.g, b, s), "taper");
             const TaperFuse tf{taper, b, it == it_hi ? 1 : 2};
             ImageFuse fz;
             fz.img = img;
@@ -664,9 +676,10 @@ int backward_loop(const rtmhip_shot *shot, const ShotCtx &c, int hbc, const floa
             fz.ry0 = c.ry0;
             fz.rdy = std::max(1, shot->rcv.dist_y);
             fz.rny = shot->rcv.count_y;
-            HIPCHK(stencil(c.radius, *Pr, *PPr, V2DT2, c.cf, c.g, &fz, hbc ? &tf : nullptr),
+            HIPCHK(stencil(c.radius, *Pr, *PPr, V2DT2, c.cf, c.g, &fz,
+                           hbc && !sep ? &tf : nullptr),
                    "fused stencil");
-            if (hbc)
+            if (hbc && !sep)
                 HIPCHK(launch_taper_frame(*PPr, taper, c.g, b, c.radius, tf.pp_taps, s),
                        "frame taper");
             HIPCHK(launch_receivers_image(*PPr, rcv_chunk, c.g, c.rx0, c.ry0, c.rz,
@@ -687,7 +700,7 @@ int backward_loop(const rtmhip_shot *shot, const ShotCtx &c, int hbc, const floa
         *PPr = t;
     }
     // pending in-place taper of the older receiver field (fused HBC path)
-    if (hbc && fused) HIPCHK(launch_taper(*PPr, taper, c.g, b, true, s), "taper");
+    if (hbc && fused && !sep) HIPCHK(launch_taper(*PPr, taper, c.g, b, true, s), "taper");
     return RTMHIP_OK;
 }
 
@@ -716,19 +729,26 @@ int snap_forward_loop(const rtmhip_shot *shot, const ShotCtx &c, const float *V2
     hipStream_t s = g_state.stream;
     const int b = shot->blen;
     const size_t per = snap_cells(shot);
+    const bool sep = g_state.taper_sep[shot->wave_format];
     for (int it = it0; it < it0 + nsteps; ++it) {
         T *t = *P;
         *P = *PP;
         *PP = t;
-        const TaperFuse tf{taper, b, it == it0 ? 1 : 2};
-        HIPCHK(stencil(c.radius, *P, *PP, V2DT2, c.cf, c.g, nullptr, &tf), "stencil");
-        HIPCHK(launch_taper_frame(*PP, taper, c.g, b, c.radius, tf.pp_taps, s), "frame taper");
+        if (sep) {
+            HIPCHK(launch_taper_upper2(*P, *PP, taper, c.g, b, s), "taper");
+            HIPCHK(stencil(c.radius, *P, *PP, V2DT2, c.cf, c.g, nullptr), "stencil");
+        } else {
+            const TaperFuse tf{taper, b, it == it0 ? 1 : 2};
+            HIPCHK(stencil(c.radius, *P, *PP, V2DT2, c.cf, c.g, nullptr, &tf), "stencil");
+            HIPCHK(launch_taper_frame(*PP, taper, c.g, b, c.radius, tf.pp_taps, s),
+                   "frame taper");
+        }
         HIPCHK(launch_source(*PP, c.src_off, ricker, it, 0.f, s), "source");
         const int j = snap_slot(it, shot->nt, k);
         if (j >= 0)
             HIPCHK(launch_snap_store(*PP, snaps + (size_t)j * per, c.g, b, s), "snapshot");
     }
-    HIPCHK(launch_taper(*P, taper, c.g, b, true, s), "taper");
+    if (!sep) HIPCHK(launch_taper(*P, taper, c.g, b, true, s), "taper");
     return RTMHIP_OK;
 }
 
@@ -743,10 +763,17 @@ int snap_backward_loop(const rtmhip_shot *shot, const ShotCtx &c, const float *V
     hipStream_t s = g_state.stream;
     const int b = shot->blen;
     const size_t per = snap_cells(shot);
+    const bool sep = g_state.taper_sep[shot->wave_format];
     for (int it = it_hi; it > it_hi - nsteps; --it) {
-        const TaperFuse tf{taper, b, it == it_hi ? 1 : 2};
-        HIPCHK(stencil(c.radius, *Pr, *PPr, V2DT2, c.cf, c.g, nullptr, &tf), "stencil");
-        HIPCHK(launch_taper_frame(*PPr, taper, c.g, b, c.radius, tf.pp_taps, s), "frame taper");
+        if (sep) {
+            HIPCHK(launch_taper_upper2(*Pr, *PPr, taper, c.g, b, s), "taper");
+            HIPCHK(stencil(c.radius, *Pr, *PPr, V2DT2, c.cf, c.g, nullptr), "stencil");
+        } else {
+            const TaperFuse tf{taper, b, it == it_hi ? 1 : 2};
+            HIPCHK(stencil(c.radius, *Pr, *PPr, V2DT2, c.cf, c.g, nullptr, &tf), "stencil");
+            HIPCHK(launch_taper_frame(*PPr, taper, c.g, b, c.radius, tf.pp_taps, s),
+                   "frame taper");
+        }
         HIPCHK(launch_receivers(*PPr, const_cast<float *>(rcv_chunk), c.g, c.rx0, c.ry0, c.rz,
                                 shot->rcv.dist_x, shot->rcv.dist_y, shot->rcv.count_x,
                                 shot->rcv.count_y, shot->ntstep, it % shot->ntstep, true, s),
@@ -760,7 +787,7 @@ int snap_backward_loop(const rtmhip_shot *shot, const ShotCtx &c, const float *V
         *Pr = *PPr;
         *PPr = t;
     }
-    HIPCHK(launch_taper(*PPr, taper, c.g, b, true, s), "taper");
+    if (!sep) HIPCHK(launch_taper(*PPr, taper, c.g, b, true, s), "taper");
     return RTMHIP_OK;
 }
 
@@ -924,6 +951,27 @@ int rtmhip_set_tiling(int kernel_class, int tz, int by, int ry, int xchunk)
         return fail(RTMHIP_ERR_UNSUPPORTED, "tile shape %dx%dx%d not compiled", tz, by, ry);
     g_state.tiling[kernel_class] = StencilTiling{tz, by, ry, xchunk};
     ++g_state.tiling_gen;
+    return RTMHIP_OK;
+}
+
+int rtmhip_set_taper_fusion(int wave_format, int mode)
+{
+    NEED_INIT();
+    if (wave_format < RTMHIP_WAVE_F32 || wave_format > RTMHIP_WAVE_BF16)
+        return fail(RTMHIP_ERR_INVALID, "unknown wave_format %d", wave_format);
+    if (mode < -1 || mode > 1)
+        return fail(RTMHIP_ERR_INVALID, "taper fusion mode %d (1 fused, 0 separate, -1 default)",
+                    mode);
+    g_state.taper_sep[wave_format] = mode < 0 ? kTaperSepDefault[wave_format] : mode == 0;
+    return RTMHIP_OK;
+}
+
+int rtmhip_get_taper_fusion(int wave_format, int *fused)
+{
+    NEED_INIT();
+    if (wave_format < RTMHIP_WAVE_F32 || wave_format > RTMHIP_WAVE_BF16 || !fused)
+        return fail(RTMHIP_ERR_INVALID, "bad arguments");
+    *fused = g_state.taper_sep[wave_format] ? 0 : 1;
     return RTMHIP_OK;
 }
 

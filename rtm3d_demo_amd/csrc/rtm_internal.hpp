@@ -130,6 +130,10 @@ hipError_t launch_snap_store(const T *P, S *snap, const Grid &g, int b, hipStrea
 template <typename T, typename S>
 hipError_t launch_snap_image(float *img, const S *snap, const T *PR, const Grid &g, int b,
                              float w, hipStream_t s);
+// taperUpper3D of two grids of the same shape in one pass (Q may be NULL).
+template <typename T>
+hipError_t launch_taper_upper2(T *P, T *Q, const float *taper, const Grid &g, int b,
+                               hipStream_t s);
 // taper3D of two grids of the same shape in one pass (Q may be NULL: one grid).
 template <typename T>
 hipError_t launch_taper_all2(T *P, T *Q, const float *taper, const Grid &g, int b,

@@ -15,6 +15,31 @@ namespace {
 
 constexpr int kThreads = 256;
 
+// Division of a dividend 0 <= n < 2^31 by a divisor d >= 1 fixed for a launch:
+// q = umulhi(n, m) >> s with l = ceil(log2 d), m = ceil(2^(31+l) / d) and
+// s = l - 1 (Granlund and Montgomery, "Division by invariant integers using
+// multiplication", PLDI 1994, with N = 31: m < 2^32 and the result is exact
+// for every such n); d = 1 is passed through.  A 32-bit division costs tens of
+// instructions per lane, which made the border kernels ALU-bound.
+struct FastDiv {
+    uint32_t m;
+    int s;  // -1 for d == 1
+};
+
+inline FastDiv fast_div(uint32_t d)
+{
+    if (d <= 1) return FastDiv{0u, -1};
+    int l = 0;
+    while ((1u << l) < d) ++l;
+    const uint64_t m = ((uint64_t(1) << (31 + l)) + d - 1) / d;
+    return FastDiv{(uint32_t)m, l - 1};
+}
+
+__device__ __forceinline__ int fdiv(int n, FastDiv d)
+{
+    return d.s < 0 ? n : (int)(__umulhi((uint32_t)n, d.m) >> d.s);
+}
+
 __device__ __forceinline__ long long goff(const Grid &g, int x, int y, int z)
 {
     return (long long)x * g.sx + (long long)y * g.sy + z;
@@ -27,37 +52,170 @@ __device__ __forceinline__ float gfac(const float *__restrict__ t, int i, int n,
     return i < b ? t[b - 1 - i] : t[i - (n - b)];
 }
 
-// taperUpper3D (RTMCPUPlatform.cpp:566-657): every cell with z < blen.
+// taperUpper3D (RTMCPUPlatform.cpp:566-657): every cell with z < blen, of
+// one or two grids (blockIdx.y picks the grid), as ((v * f1) * f2) * f3:
 //   x, y interior           : v*gz
 //   x in a border (any y)   : ((v*gy if y in a border)*gx)*gz
 //   y in a border, x inner  : (v*gy)*gz
+// Cells are numbered z fastest over [0, nx*ny*b); every thread handles
+// kUpperPer of them kThreads apart and issues their loads first.
+constexpr int kUpperPer = 4;
+
 template <typename T>
 __global__ void __launch_bounds__(kThreads)
-    k_taper_upper(T *__restrict__ P, const float *__restrict__ t, Grid g, int b)
+    k_taper_upper(T *__restrict__ P0, T *__restrict__ P1, const float *__restrict__ t, Grid g,
+                  int b, int n, FastDiv db, FastDiv dny)
 {
-    const long long n = (long long)g.nx * g.ny * b;
-    const long long i = (long long)blockIdx.x * kThreads + threadIdx.x;
-    if (i >= n) return;
-    const int z = (int)(i % b);
-    const long long r = i / b;
-    const int y = (int)(r % g.ny);
-    const int x = (int)(r / g.ny);
+    T *__restrict__ P = blockIdx.y ? P1 : P0;
+    const int i0 = (int)blockIdx.x * (kThreads * kUpperPer) + (int)threadIdx.x;
+    long long off[kUpperPer];
+    float f1[kUpperPer], f2[kUpperPer], f3[kUpperPer], v[kUpperPer];
+#pragma unroll
+    for (int k = 0; k < kUpperPer; ++k) {
+        const int i = i0 + k * kThreads;
+        off[k] = -1;
+        v[k] = 0.0f;
+        if (i >= n) continue;
+        const int r = fdiv(i, db);
+        const int z = i - r * b;
+        const int x = fdiv(r, dny);
+        const int y = r - x * g.ny;
+        const bool bx = x < b || x >= g.nx - b;
+        const bool by = y < b || y >= g.ny - b;
+        const float gy = by ? gfac(t, y, g.ny, b) : 1.0f;
+        const float gz = t[b - 1 - z];
+        f1[k] = bx ? gy : (by ? gy : gz);
+        f2[k] = bx ? gfac(t, x, g.nx, b) : (by ? gz : 1.0f);
+        f3[k] = bx ? gz : 1.0f;
+        off[k] = (long long)x * g.sx + (long long)y * g.sy + z;
+        v[k] = wld(P[off[k]]);
+    }
+#pragma unroll
+    for (int k = 0; k < kUpperPer; ++k) {
+        if (off[k] < 0) continue;
+        float w = v[k] * f1[k];
+        w *= f2[k];
+        w *= f3[k];
+        P[off[k]] = wst<T>(w);
+    }
+}
+
+// 16-byte groups of G = 16/sizeof(T) consecutive z cells, for the border
+// kernels' vector forms: one 16-byte load and store per lane.  Used when the
+// border width and nz are multiples of G and the fields are 16-byte aligned
+// (then no group straddles a border edge).
+template <typename T>
+struct alignas(16) Group16 {
+    static constexpr int G = 16 / (int)sizeof(T);
+    T v[G];
+};
+
+template <typename T>
+__device__ __forceinline__ void taper_group(T *__restrict__ p, const float *__restrict__ t,
+                                            int z0, int nz, int b, float f1, float f2,
+                                            bool bz_factor, int zmode)
+{
+    // zmode: 0 no z factor; 1 z factor after f1 and f2 ((v*f1)*f2)*gz;
+    //        2 z factor in second place (v*f1)*gz
+    Group16<T> a = *reinterpret_cast<Group16<T> *>(p);
+#pragma unroll
+    for (int k = 0; k < Group16<T>::G; ++k) {
+        const int z = z0 + k;
+        const float gz = bz_factor ? gfac(t, z, nz, b) : 1.0f;
+        float w = wld(a.v[k]) * f1;
+        if (zmode == 2) {
+            w *= gz;
+        } else {
+            w *= f2;
+            if (zmode == 1) w *= gz;
+        }
+        a.v[k] = wst<T>(w);
+    }
+    *reinterpret_cast<Group16<T> *>(p) = a;
+}
+
+// taperUpper3D over 16-byte groups: the band z < b of every (x, y) row.
+template <typename T>
+__global__ void __launch_bounds__(kThreads)
+    k_taper_upper_v(T *__restrict__ P0, T *__restrict__ P1, const float *__restrict__ t, Grid g,
+                    int b, int ngroups, FastDiv dbg, FastDiv dny)
+{
+    constexpr int G = Group16<T>::G;
+    T *__restrict__ P = blockIdx.y ? P1 : P0;
+    const int q = (int)(blockIdx.x * kThreads + threadIdx.x);
+    if (q >= ngroups) return;
+    const int bg = b / G;
+    const int r = fdiv(q, dbg);
+    const int z0 = (q - r * bg) * G;
+    const int x = fdiv(r, dny);
+    const int y = r - x * g.ny;
     const bool bx = x < b || x >= g.nx - b;
     const bool by = y < b || y >= g.ny - b;
-    const long long o = goff(g, x, y, z);
-    float v = wld(P[o]);
-    const float gz = t[b - 1 - z];
-    if (!bx && !by) {
-        v *= gz;
-    } else if (bx) {
-        if (by) v *= gfac(t, y, g.ny, b);
-        v *= gfac(t, x, g.nx, b);
-        v *= gz;
+    const float gy = by ? gfac(t, y, g.ny, b) : 1.0f;
+    // x border: ((v*gy)*gx)*gz; y border: (v*gy)*gz; inner: v*gz
+    const float f1 = bx || by ? gy : 1.0f;
+    const float f2 = bx ? gfac(t, x, g.nx, b) : 1.0f;
+    taper_group(P + ((long long)x * g.sx + (long long)y * g.sy + z0), t, z0, g.nz, b, f1, f2,
+                true, 1);
+}
+
+// taper3D over 16-byte groups (the front-right edge's interior-z cells
+// excepted): the z-border groups of the rows with x and y interior, then
+// every group of the rows with x or y in a border, as k_taper_all.
+template <typename T>
+__global__ void __launch_bounds__(kThreads)
+    k_taper_all_v(T *__restrict__ P0, T *__restrict__ P1, const float *__restrict__ t, Grid g,
+                  int b, int nA, int nB, FastDiv d2bg, FastDiv dly, FastDiv dnzg, FastDiv dny,
+                  FastDiv d2b)
+{
+    constexpr int G = Group16<T>::G;
+    T *__restrict__ P = blockIdx.y ? P1 : P0;
+    const int i = (int)(blockIdx.x * kThreads + threadIdx.x);
+    const int b2 = 2 * b, bg2 = b2 / G, nzg = g.nz / G;
+    int x, y, z0;
+    bool bx, by;
+    if (i < nA) {
+        const int r = fdiv(i, d2bg), j = (i - r * bg2) * G;
+        const int xi = fdiv(r, dly);
+        x = b + xi;
+        y = b + (r - xi * (g.ny - b2));
+        z0 = j < b ? j : g.nz - b2 + j;
+        bx = by = false;
+    } else if (i - nA < nB) {
+        const int e = i - nA;
+        const int r = fdiv(e, dnzg);
+        z0 = (e - r * nzg) * G;
+        const int nxs = b2 * g.ny;
+        if (r < nxs) {
+            const int xx = fdiv(r, dny);
+            y = r - xx * g.ny;
+            x = xx < b ? xx : g.nx - b2 + xx;
+            bx = true;
+            by = y < b || y >= g.ny - b;
+        } else {
+            const int q = r - nxs, xi = fdiv(q, d2b), yy = q - xi * b2;
+            x = b + xi;
+            y = yy < b ? yy : g.ny - b2 + yy;
+            bx = false;
+            by = true;
+        }
     } else {
-        v *= gfac(t, y, g.ny, b);
-        v *= gz;
+        return;
     }
-    P[o] = wst<T>(v);
+    const bool bz = z0 < b || z0 >= g.nz - b;  // the whole group (b, nz multiples of G)
+    if (bx && by && !bz && x < b && y >= g.ny - b) return;  // front-right edge
+    const float gx = bx ? gfac(t, x, g.nx, b) : 1.0f;
+    const float gy = by ? gfac(t, y, g.ny, b) : 1.0f;
+    T *p = P + ((long long)x * g.sx + (long long)y * g.sy + z0);
+    if (bz) {
+        // (bx: [gy] gx gz) | (by: gy gz) | gz
+        if (bx) taper_group(p, t, z0, g.nz, b, gy, gx, true, 1);
+        else if (by) taper_group(p, t, z0, g.nz, b, gy, 1.0f, true, 2);
+        else taper_group(p, t, z0, g.nz, b, 1.0f, 1.0f, true, 2);
+    } else {
+        // (bx and by: gx gy) | gy | gx
+        taper_group(p, t, z0, g.nz, b, bx ? gx : gy, bx && by ? gy : 1.0f, false, 0);
+    }
 }
 
 // The slab cells (z < b) that the stencil does not update: z < R, or x / y
@@ -132,7 +290,7 @@ template <typename T>
 __global__ void __launch_bounds__(kThreads)
     k_taper_frame(T *__restrict__ P, const float *__restrict__ t, Grid g, int b, int R, int taps)
 {
-    const long long i = (long long)blockIdx.x * kThreads + threadIdx.x;
+    const int i = (int)(blockIdx.x * kThreads + threadIdx.x);  // count < 2^31 (launcher)
     int x, y, z;
     if (!frame_cell(g, b, R, i, x, y, z)) return;
     const long long o = goff(g, x, y, z);
@@ -170,52 +328,104 @@ __global__ void __launch_bounds__(kThreads)
 
 // taper3D (RTMCPUPlatform.cpp:710-920) for every border cell except the
 // front-right edge's interior-z cells, of one or two grids (blockIdx.y picks
-// the grid).  One wave per (x, y) row, z on the lanes: in a row with x or y in
-// a border every z is a border cell; otherwise, and in the front-right
-// column (x < b, y >= ny-b), only z < b and z >= nz-b are (the 2b cells are
-// packed onto consecutive lanes).  x, y and their factors are wave-uniform.
-// Per-cell factor order follows the reference loop nests.
+// the grid), over two packed index ranges:
+//   [0, nA)        the z-border cells (z < b or z >= nz-b) of the rows with x
+//                  and y interior: 2b consecutive cells per row, z factor only;
+//   [nA, nA + nB)  every cell of the rows with x or y in a border (the x
+//                  border slabs for all y first, then the y border rows of
+//                  the interior x), nz per row.
+// Each cell gets up to three factors in the reference loop nests' order,
+// ((v * f1) * f2) * f3, unused ones being 1 (v * 1 == v exactly).  Every
+// thread handles kPer cells kThreads apart and issues their loads first.
+constexpr int kTaperPer = 4;
+
+struct TaperCell {
+    long long off;
+    float f1, f2, f3;
+    bool ok;
+};
+
+struct TaperDivs {
+    FastDiv b2, ly, nz, ny;  // by 2b, ny-2b, nz, ny
+};
+
+__device__ __forceinline__ TaperCell taper_cell(const float *__restrict__ t, const Grid &g, int b,
+                                                int nA, int nB, const TaperDivs &dv, int i)
+{
+    TaperCell c{0, 1.0f, 1.0f, 1.0f, false};
+    const int b2 = 2 * b;
+    int x, y, z;
+    bool bx, by;
+    if (i < nA) {
+        const int ly = g.ny - b2;
+        const int r = fdiv(i, dv.b2), j = i - r * b2;
+        const int xi = fdiv(r, dv.ly);
+        x = b + xi;
+        y = b + (r - xi * ly);
+        z = j < b ? j : g.nz - b2 + j;
+        bx = by = false;
+    } else if (i - nA < nB) {
+        const int e = i - nA;
+        const int r = fdiv(e, dv.nz);
+        z = e - r * g.nz;
+        const int nxs = b2 * g.ny;  // rows of the two x border slabs
+        if (r < nxs) {
+            const int xx = fdiv(r, dv.ny);
+            y = r - xx * g.ny;
+            x = xx < b ? xx : g.nx - b2 + xx;
+            bx = true;
+            by = y < b || y >= g.ny - b;
+        } else {
+            const int q = r - nxs, xi = fdiv(q, dv.b2), yy = q - xi * b2;
+            x = b + xi;
+            y = yy < b ? yy : g.ny - b2 + yy;
+            bx = false;
+            by = true;
+        }
+    } else {
+        return c;
+    }
+    const bool bz = z < b || z >= g.nz - b;
+    if (bx && by && !bz && x < b && y >= g.ny - b) return c;  // front-right edge
+    const float gx = bx ? gfac(t, x, g.nx, b) : 1.0f;
+    const float gy = by ? gfac(t, y, g.ny, b) : 1.0f;
+    const float gz = bz ? gfac(t, z, g.nz, b) : 1.0f;
+    if (bz) {
+        // (bx: [gy] gx gz) | (by: gy gz) | gz
+        c.f1 = bx ? gy : (by ? gy : gz);
+        c.f2 = bx ? gx : (by ? gz : 1.0f);
+        c.f3 = bx ? gz : 1.0f;
+    } else {
+        // (bx and by: gx gy) | gy | gx
+        c.f1 = bx ? gx : gy;
+        c.f2 = bx && by ? gy : 1.0f;
+    }
+    c.off = (long long)x * g.sx + (long long)y * g.sy + z;
+    c.ok = true;
+    return c;
+}
+
 template <typename T>
 __global__ void __launch_bounds__(kThreads)
     k_taper_all(T *__restrict__ P0, T *__restrict__ P1, const float *__restrict__ t, Grid g,
-                int b)
+                int b, int nA, int nB, TaperDivs dv)
 {
     T *__restrict__ P = blockIdx.y ? P1 : P0;
-    const int lane = threadIdx.x & 63;
-    const int row = __builtin_amdgcn_readfirstlane((int)blockIdx.x * (kThreads / 64) +
-                                                   (int)(threadIdx.x >> 6));
-    if (row >= g.nx * g.ny) return;
-    const int x = row / g.ny;
-    const int y = row - x * g.ny;
-    const bool bx = x < b || x >= g.nx - b;
-    const bool by = y < b || y >= g.ny - b;
-    const float gx = bx ? gfac(t, x, g.nx, b) : 1.0f;
-    const float gy = by ? gfac(t, y, g.ny, b) : 1.0f;
-    T *__restrict__ r = P + ((long long)x * g.sx + (long long)y * g.sy);
-    const bool full = (bx || by) && !(x < b && y >= g.ny - b);
-    const int n = full ? g.nz : 2 * b;
-    for (int i = lane; i < n; i += 64) {
-        const int z = full || i < b ? i : g.nz - 2 * b + i;
-        const bool bz = z < b || z >= g.nz - b;
-        float v = wld(r[z]);
-        if (bx && bz) {
-            if (by) v *= gy;
-            v *= gx;
-            v *= gfac(t, z, g.nz, b);
-        } else if (by && bz) {
-            v *= gy;
-            v *= gfac(t, z, g.nz, b);
-        } else if (bx && by) {
-            v *= gx;
-            v *= gy;
-        } else if (bz) {
-            v *= gfac(t, z, g.nz, b);
-        } else if (by) {
-            v *= gy;
-        } else {
-            v *= gx;
-        }
-        r[z] = wst<T>(v);
+    const int i0 = (int)blockIdx.x * (kThreads * kTaperPer) + (int)threadIdx.x;
+    TaperCell c[kTaperPer];
+    float v[kTaperPer];
+#pragma unroll
+    for (int k = 0; k < kTaperPer; ++k) {
+        c[k] = taper_cell(t, g, b, nA, nB, dv, i0 + k * kThreads);
+        v[k] = c[k].ok ? wld(P[c[k].off]) : 0.0f;
+    }
+#pragma unroll
+    for (int k = 0; k < kTaperPer; ++k) {
+        if (!c[k].ok) continue;
+        float w = v[k] * c[k].f1;
+        w *= c[k].f2;
+        w *= c[k].f3;
+        P[c[k].off] = wst<T>(w);
     }
 }
 
@@ -619,6 +829,16 @@ __global__ void k_tick(int *p, int d, int set)
     if (threadIdx.x == 0 && blockIdx.x == 0) *p = set ? d : *p + d;
 }
 
+// The 16-byte group kernels apply: G-cell groups never straddle a border
+// edge or a row, and every group is 16-byte aligned.
+template <typename T>
+bool vec16_ok(const T *P, const T *Q, const Grid &g, int b, int G)
+{
+    const auto al = [](const void *p) { return ((uintptr_t)p & 15u) == 0; };
+    return b % G == 0 && g.nz % G == 0 && g.sy % G == 0 && g.sx % G == 0 && al(P) &&
+           (!Q || al(Q));
+}
+
 inline unsigned nblocks(long long n)
 {
     return (unsigned)((n + kThreads - 1) / kThreads);
@@ -637,12 +857,29 @@ hipError_t launch_taper(T *P, const float *taper, const Grid &g, int b, bool upp
                         hipStream_t s)
 {
     if (b <= 0) return hipSuccess;
-    if (upper_only) {
-        const long long n = (long long)g.nx * g.ny * b;
-        hipLaunchKernelGGL(k_taper_upper<T>, dim3(nblocks(n)), dim3(kThreads), 0, s, P, taper, g, b);
+    if (upper_only) return launch_taper_upper2(P, (T *)nullptr, taper, g, b, s);
+    return launch_taper_all2(P, (T *)nullptr, taper, g, b, s);
+}
+
+template <typename T>
+hipError_t launch_taper_upper2(T *P, T *Q, const float *taper, const Grid &g, int b,
+                               hipStream_t s)
+{
+    if (b <= 0) return hipSuccess;
+    const long long n = (long long)g.nx * g.ny * b;
+    const long long per = (long long)kThreads * kUpperPer;
+    if (n + per >= (1LL << 31)) return hipErrorInvalidValue;
+    constexpr int G = Group16<T>::G;
+    if (vec16_ok(P, Q, g, b, G)) {
+        const long long ng = n / G;
+        hipLaunchKernelGGL(k_taper_upper_v<T>, dim3(nblocks(ng), Q ? 2 : 1), dim3(kThreads), 0, s,
+                           P, Q, taper, g, b, (int)ng, fast_div(b / G), fast_div(g.ny));
         return hipGetLastError();
     }
-    return launch_taper_all2(P, (T *)nullptr, taper, g, b, s);
+    hipLaunchKernelGGL(k_taper_upper<T>, dim3((unsigned)((n + per - 1) / per), Q ? 2 : 1),
+                       dim3(kThreads), 0, s, P, Q, taper, g, b, (int)n, fast_div(b),
+                       fast_div(g.ny));
+    return hipGetLastError();
 }
 
 template <typename T>
@@ -660,9 +897,22 @@ hipError_t launch_taper_all2(T *P, T *Q, const float *taper, const Grid &g, int 
         hipLaunchKernelGGL(k_taper_front_right<T>, dim3(nblocks(nfr), ng), dim3(kThreads), 0, s,
                            P, Q, taper, g, b);
     }
-    const long long rows = (long long)g.nx * g.ny;
-    hipLaunchKernelGGL(k_taper_all<T>, dim3((unsigned)((rows + kThreads / 64 - 1) / (kThreads / 64)), ng),
-                       dim3(kThreads), 0, s, P, Q, taper, g, b);
+    const long long nA = (long long)(g.nx - 2 * b) * (g.ny - 2 * b) * 2 * b;
+    const long long nB = ((long long)2 * b * g.ny + (long long)(g.nx - 2 * b) * 2 * b) * g.nz;
+    const long long per = (long long)kThreads * kTaperPer;
+    if (nA + nB + per >= (1LL << 31)) return hipErrorInvalidValue;
+    constexpr int G = Group16<T>::G;
+    if (vec16_ok(P, Q, g, b, G)) {
+        const long long ngr = (nA + nB) / G;
+        hipLaunchKernelGGL(k_taper_all_v<T>, dim3(nblocks(ngr), ng), dim3(kThreads), 0, s, P, Q,
+                           taper, g, b, (int)(nA / G), (int)(nB / G), fast_div(2 * b / G),
+                           fast_div(g.ny - 2 * b), fast_div(g.nz / G), fast_div(g.ny),
+                           fast_div(2 * b));
+        return hipGetLastError();
+    }
+    const TaperDivs dv{fast_div(2 * b), fast_div(g.ny - 2 * b), fast_div(g.nz), fast_div(g.ny)};
+    hipLaunchKernelGGL(k_taper_all<T>, dim3((unsigned)((nA + nB + per - 1) / per), ng),
+                       dim3(kThreads), 0, s, P, Q, taper, g, b, (int)nA, (int)nB, dv);
     return hipGetLastError();
 }
 
@@ -674,6 +924,7 @@ hipError_t launch_taper_frame(T *P, const float *taper, const Grid &g, int b, in
     const long long lz = b - R;
     const long long n = (long long)g.nx * g.ny * R + (long long)2 * R * g.ny * lz +
                         (long long)(g.nx - 2 * R) * 2 * R * lz;
+    if (n + kThreads >= (1LL << 31)) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_taper_frame<T>, dim3(nblocks(n)), dim3(kThreads), 0, s, P, taper, g, b, R,
                        taps);
     return hipGetLastError();
@@ -852,6 +1103,8 @@ hipError_t launch_tick(int *p, int d, bool set, hipStream_t s)
                                         hipStream_t);                                           \
     template hipError_t launch_taper_all2<T>(T *, T *, const float *, const Grid &, int,        \
                                              hipStream_t);                                      \
+    template hipError_t launch_taper_upper2<T>(T *, T *, const float *, const Grid &, int,      \
+                                               hipStream_t);                                    \
     template hipError_t launch_taper_frame<T>(T *, const float *, const Grid &, int, int, int,  \
                                               hipStream_t);                                     \
     template hipError_t launch_source<T>(T *, long long, const float *, int, float, hipStream_t); \

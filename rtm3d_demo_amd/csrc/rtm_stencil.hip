@@ -2311,6 +2311,19 @@ hipError_t launch_radius(const T *P, T *PP, const float *V, const Coefs &cf, con
             return launch_wave<T, R, 8, 4>(P, PP, V, cf, g, fuse, taper, flag, s, xchunk);
         }
     }
+    if constexpr (R == 4 && std::is_same<T, float>::value) {
+        // FP32 without a requested shape: the plain step on two-z 128 x 1 x 4
+        // waves (64 x 1 x 8 waves where pairs do not fit), the tapered step
+        // on 64 x 1 x 8 waves (salt grid: 0.412 and 0.463 ms against 0.447
+        // and 0.498 for full rows 2 x 6); the fused receiver step keeps the
+        // full rows below (0.699 ms against 0.79 or more on every wave shape).
+        if (!t && !fuse && (!taper || taper->b <= 8192)) {
+            const bool tapered = taper && taper->b > 0;
+            if (!tapered && wave2_fits(P, PP, V, fuse, g))
+                return launch_wave2<T, R, 4, 4>(P, PP, V, cf, g, fuse, taper, flag, s, xchunk);
+            return launch_wave<T, R, 8, 4>(P, PP, V, cf, g, fuse, taper, flag, s, xchunk);
+        }
+    }
     if constexpr (R == 4) {
         // full-row blocks by default whenever a compiled shape covers nz (and
         // the taper table fits in LDS beside the plane buffers)

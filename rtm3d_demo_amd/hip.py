@@ -23,6 +23,7 @@ EXPORTED = [
     "rtmhip_model_steps", "rtmhip_forward_steps", "rtmhip_backward_steps",
     "rtmhip_snap_buffer_bytes", "rtmhip_snap_forward_steps", "rtmhip_snap_backward_steps",
     "rtmhip_set_tiling", "rtmhip_get_tiling", "rtmhip_timing_enable", "rtmhip_timing_read",
+    "rtmhip_set_taper_fusion", "rtmhip_get_taper_fusion",
     "rtmhip_batch_create", "rtmhip_batch_destroy", "rtmhip_batch_forward_steps",
     "rtmhip_batch_backward_steps", "rtmhip_batch_fields", "rtmhip_batch_set_stencil",
     "rtmhip_batch_set_graph", "rtmhip_batch_steps_noalloc",
@@ -99,6 +100,8 @@ def load():
         "rtmhip_snap_backward_steps": [C.POINTER(Shot), fp, fp, fp, pp, pp, vp, i, i, fp, fp, i,
                                        i],
         "rtmhip_set_tiling": [i, i, i, i, i],
+        "rtmhip_set_taper_fusion": [i, i],
+        "rtmhip_get_taper_fusion": [i, C.POINTER(C.c_int)],
         "rtmhip_get_tiling": [i] + [C.POINTER(C.c_int)] * 4,
         "rtmhip_timing_enable": [i],
         "rtmhip_timing_read": [i, C.POINTER(C.c_longlong), C.POINTER(C.c_double)],
@@ -344,6 +347,18 @@ def snap_backward_steps(shot, V, coefs, g, rcv_fields, snaps, fmt, k, rcv_chunk,
 
 def set_tiling(kernel_class, tz=0, by=0, ry=0, xchunk=0):
     check(load().rtmhip_set_tiling(kernel_class, tz, by, ry, xchunk))
+
+
+def set_taper_fusion(fmt, mode):
+    """mode 1: upper tapers fused into the stencil; 0: a separate in-place
+    pass; -1: the format's default."""
+    check(load().rtmhip_set_taper_fusion(WAVE_FORMATS[fmt][0], mode))
+
+
+def get_taper_fusion(fmt):
+    v = C.c_int()
+    check(load().rtmhip_get_taper_fusion(WAVE_FORMATS[fmt][0], C.byref(v)))
+    return v.value
 
 
 def get_tiling(kernel_class):

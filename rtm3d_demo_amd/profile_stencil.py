@@ -14,16 +14,19 @@ On the 592x592x336 grid of the 512x512x256 + 40 salt workload it launches:
     (mode 2: two seeding steps, then 5 dual steps).
 Fields hold random values; counters do not depend on them.
 
-    python3 rtm3d_demo_amd/profile_stencil.py --summarize FETCH_DIR WRITE_DIR
-
-(--tiling TZ,BY,RY as the only arguments runs the plain step with that shape).
+    python3 rtm3d_demo_amd/profile_stencil.py --summarize FETCH_DIR WRITE_DIR [--tiling ...]
 
 turns the counter_collection.csv files of a FETCH_SIZE pass and a WRITE_SIZE
 pass into per-kernel bytes per dispatch.  Each counter is scaled by what it
 reads for k_stack, whose true traffic is known (8 B read + 4 B written per
 cell), and the first kernel of each stencil class is printed as
-"traffic_bytes_per_launch[class C]: N", the lines bench.py reads (0 plain,
-1 fused receiver, 2 tapered forward, 3 dual backward).
+"traffic_bytes_per_launch[class C, tile T]: N", the lines bench.py reads (0
+plain, 1 fused receiver, 2 tapered forward, 3 dual backward; T the class's
+tile shape TZxBYxRY, or "default").
+
+--tiling C:TZ,BY,RY (repeatable) sets the tile shape of stencil class C for
+the sequence; give the same options to --summarize so that the lines carry
+the shapes.
 """
 import csv
 import glob
@@ -62,7 +65,18 @@ def _short(name):
     return name.split("(")[0]
 
 
-def summarize(fetch_dir, write_dir):
+def parse_tilings(argv):
+    """{class: (tz, by, ry)} from repeated --tiling C:TZ,BY,RY options."""
+    out = {}
+    for i, a in enumerate(argv):
+        if a == "--tiling":
+            c, shape = argv[i + 1].split(":")
+            out[int(c)] = tuple(int(v) for v in shape.split(","))
+    return out
+
+
+def summarize(fetch_dir, write_dir, tilings=None):
+    tilings = tilings or {}
     fetch = _per_dispatch(fetch_dir, "FETCH_SIZE")
     write = _per_dispatch(write_dir, "WRITE_SIZE")
     cells = SHAPE[0] * SHAPE[1] * SHAPE[2]
@@ -103,23 +117,22 @@ def summarize(fetch_dir, write_dir):
         short, t = first[cls]
         print("class %d (%s): algorithmic %.0f B per launch (%.0f B x %d updated cells)"
               % (cls, short, algo[cls] * interior, algo[cls], interior))
-        print("traffic_bytes_per_launch[class %d]: %.0f" % (cls, t))
+        tile = "%dx%dx%d" % tilings[cls] if cls in tilings else "default"
+        print("traffic_bytes_per_launch[class %d, tile %s]: %.0f" % (cls, tile, t))
 
 
 def main():
-    if len(sys.argv) == 4 and sys.argv[1] == "--summarize":
-        summarize(sys.argv[2], sys.argv[3])
+    tilings = parse_tilings(sys.argv)
+    if len(sys.argv) >= 4 and sys.argv[1] == "--summarize":
+        summarize(sys.argv[2], sys.argv[3], tilings)
         return 0
     from rtm3d_demo_amd import hip, host
-    tiling = None
-    if len(sys.argv) == 3 and sys.argv[1] == "--tiling":  # e.g. --tiling 64,1,8 (plain step)
-        tiling = tuple(int(v) for v in sys.argv[2].split(","))
 
     shape = SHAPE
     b, order, nt = 40, 8, 16
     hip.init(0)
-    if tiling:
-        hip.set_tiling(0, *tiling, 0)
+    for cls, t in tilings.items():
+        hip.set_tiling(cls, *t, 0)
     rng = np.random.default_rng(0)
 
     def rand(lo, hi):

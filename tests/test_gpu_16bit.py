@@ -34,7 +34,7 @@ def test_modeling_16bit_matches_oracle(gpu, oracle, cases, golden, fmt):
 @pytest.mark.parametrize("fused", [True, False])
 @pytest.mark.parametrize("bc", ["hbc", "rbc"])
 @pytest.mark.parametrize("fmt", FMT16)
-def test_migration_16bit_matches_oracle(gpu, oracle, cases, golden, fmt, bc, fused):
+def test_migration_16bit_matches_oracle(gpu, oracle, cases, golden, fmt, bc, fused, fusion16):
     print("migration", fmt, bc, fused, flush=True)
     p = cases["small_" + bc]
     vpe = golden("small_" + bc)["vpe"]
@@ -56,15 +56,28 @@ def shape16(request, gpu):
         gpu.set_tiling(cls, 0)
 
 
+@pytest.fixture(params=[-1, 1], ids=["tdefault", "tfused"])
+def fusion16(request, gpu):
+    """The 16-bit default (upper tapers in a separate in-place pass) or the
+    tapers fused into the stencil."""
+    for fmt in FMT16:
+        gpu.set_taper_fusion(fmt, request.param)
+    yield request.param
+    for fmt in FMT16:
+        gpu.set_taper_fusion(fmt, -1)
+
+
 # two and six waves per z row (full-row kernel), seven (z-tiled kernel);
 # odd nz: one z per lane for every wave shape
 GRIDS16 = [(40, 44, 90), (32, 36, 340), (30, 32, 420), (30, 34, 91)]
 
 
+@pytest.mark.parametrize("b", [12, 11])
 @pytest.mark.parametrize("grid", GRIDS16)
 @pytest.mark.parametrize("fmt", FMT16)
-def test_fused_forward_16bit_matches_operator_sequence(gpu, oracle, fmt, grid, shape16):
-    b, order, nt = 12, 8, 4
+def test_fused_forward_16bit_matches_operator_sequence(gpu, oracle, fmt, grid, shape16, fusion16,
+                                                       b):
+    order, nt = 8, 4
     rng = np.random.default_rng(31)
     P0 = oracle.to_format(rng.uniform(-1, 1, grid).astype(np.float32), fmt)
     PP0 = oracle.to_format(rng.uniform(-1, 1, grid).astype(np.float32), fmt)
@@ -97,10 +110,12 @@ def test_fused_forward_16bit_matches_operator_sequence(gpu, oracle, fmt, grid, s
         d.free()
 
 
+@pytest.mark.parametrize("b", [12, 11])
 @pytest.mark.parametrize("grid", GRIDS16)
 @pytest.mark.parametrize("fmt", FMT16)
-def test_fused_backward_16bit_matches_operator_sequence(gpu, oracle, fmt, grid, shape16):
-    b, order, nt = 12, 8, 8
+def test_fused_backward_16bit_matches_operator_sequence(gpu, oracle, fmt, grid, shape16, fusion16,
+                                                        b):
+    order, nt = 8, 8
     rng = np.random.default_rng(32)
     q = lambda: oracle.to_format(rng.uniform(-1, 1, grid).astype(np.float32), fmt)
     Ps, PPs, Pr, PPr = q(), q(), q(), q()
@@ -157,3 +172,64 @@ def test_image_error_vs_fp64(gpu, oracle, cases, golden, fmt, bound):
     err = float(np.linalg.norm(img - ref) / np.linalg.norm(ref))
     print("image rel-L2 vs FP64 (%s): %.3e" % (fmt, err), flush=True)
     assert err < bound
+
+
+@pytest.mark.parametrize("fmt", ["f32", "f16", "bf16"])
+def test_taper_vector_forms_match_oracle(gpu, oracle, fmt):
+    """Border widths and nz that are multiples of the 16-byte group (4 FP32
+    or 8 16-bit cells) take the grouped taper kernels: the all-border taper
+    of the modeling loop and the separate upper-taper pass of the HBC loops,
+    on non-zero fields, against the operator sequence."""
+    grid, b, order, nt = (40, 44, 96), 16, 8, 4
+    rng = np.random.default_rng(33)
+    wdt = np.float32 if fmt == "f32" else np.uint16
+    q = (lambda a: a) if fmt == "f32" else (lambda a: oracle.to_format(a, fmt))
+    P0 = q(rng.uniform(-1, 1, grid).astype(np.float32))
+    PP0 = q(rng.uniform(-1, 1, grid).astype(np.float32))
+    v2 = rng.uniform(0.0, 0.15, grid).astype(np.float32)
+    g = oracle.taper_table(b, np.float32(0.3))
+    coefs = oracle.stencil_coefs(order, 10.0, 11.0, 9.0)
+    src = rng.uniform(-1, 1, nt).astype(np.float32)
+    ofmt = None if fmt == "f32" else fmt
+    V, dg, ds = (gpu.DeviceArray.from_host(a) for a in (v2, g, src))
+    shot = gpu.make_shot(*grid, b, order, nt, nt, (20, 22, b + 2),
+                         gpu.RcvGeom(0, 0, 0, 1, 1, 1, 1), fmt)
+    # modeling: taper all borders of both fields, step, source
+    P, PP = P0.copy(), PP0.copy()
+    rec = np.zeros((1, 1, nt), np.float32)
+    for it in range(nt):
+        P, PP = PP, P
+        oracle.taper(P, g, b, False, ofmt)
+        oracle.taper(PP, g, b, False, ofmt)
+        oracle.step(P, PP, v2, coefs, order, ofmt)
+        oracle.apply_source(PP, 20, 22, b + 2, src[it], ofmt)
+    F = gpu.FieldPair(grid, wdt)
+    F.a.upload(P0)
+    F.b.upload(PP0)
+    drec = gpu.DeviceArray((1, 1, nt)).zero()
+    gpu.model_steps(shot, V, coefs, dg, ds, F, drec, 0, nt)
+    assert np.array_equal(F.P.download(), P) and np.array_equal(F.PP.download(), PP)
+    # HBC forward with the separate upper-taper pass
+    gpu.set_taper_fusion(fmt, 0)
+    try:
+        P, PP = P0.copy(), PP0.copy()
+        upb = np.zeros((nt, grid[0], grid[1], 4), wdt)
+        for it in range(nt):
+            P, PP = PP, P
+            oracle.taper(P, g, b, True, ofmt)
+            oracle.taper(PP, g, b, True, ofmt)
+            oracle.step(P, PP, v2, coefs, order, ofmt)
+            oracle.apply_source(PP, 20, 22, b + 2, src[it], ofmt)
+            oracle.upb(PP, upb, b, order, it, False, ofmt)
+        F.a.upload(P0)
+        F.b.upload(PP0)
+        F.p.value, F.pp.value = F.a.ptr.value, F.b.ptr.value
+        dupb = gpu.DeviceArray(upb.shape, wdt).zero()
+        gpu.forward_steps(shot, True, V, coefs, dg, ds, F, dupb, 0, nt)
+        assert np.array_equal(F.P.download(), P) and np.array_equal(F.PP.download(), PP)
+        assert np.array_equal(dupb.download(), upb)
+        dupb.free()
+    finally:
+        gpu.set_taper_fusion(fmt, -1)
+        for d in (F, V, dg, ds, drec):
+            d.free()

@@ -203,13 +203,26 @@ def xchunk(request, gpu):
 GRIDS = [(40, 44, 90), (32, 36, 340), (30, 32, 420), (30, 34, 91)]
 
 
+@pytest.fixture(params=[1, 0], ids=["tfused", "tsep"])
+def fusion32(request, gpu):
+    """Upper tapers fused into the FP32 stencil (the default) or applied by
+    a separate in-place pass."""
+    gpu.set_taper_fusion("f32", request.param)
+    yield request.param
+    gpu.set_taper_fusion("f32", -1)
+
+
+@pytest.mark.parametrize("b", [12, 11])
 @pytest.mark.parametrize("shape", GRIDS)
 @pytest.mark.parametrize("splits", [(5,), (2, 1, 2)])
-def test_fused_taper_forward_matches_operator_sequence(gpu, oracle, splits, xchunk, shape):
-    """HBC forward steps with the upper taper fused into the stencil equal
-    the reference's operator sequence (taper, taper, step, source, save
-    UPB) on non-zero fields, whether run in one call or split into calls."""
-    b, order, nt = 12, 8, 5
+def test_fused_taper_forward_matches_operator_sequence(gpu, oracle, splits, xchunk, shape, b,
+                                                       fusion32):
+    """HBC forward steps with the upper taper fused into the stencil (or in a
+    separate pass) equal the reference's operator sequence (taper, taper,
+    step, source, save UPB) on non-zero fields, whether run in one call or
+    split into calls.  An odd border puts the taper edge inside a z pair of
+    the two-z kernels."""
+    order, nt = 8, 5
     rng = np.random.default_rng(21)
     P0 = rng.uniform(-1, 1, shape).astype(np.float32)
     PP0 = rng.uniform(-1, 1, shape).astype(np.float32)
@@ -242,14 +255,19 @@ def test_fused_taper_forward_matches_operator_sequence(gpu, oracle, splits, xchu
         d.free()
 
 
-@pytest.mark.parametrize("mode", [1, 2])
+@pytest.mark.parametrize("b", [12, 11])
+@pytest.mark.parametrize("mode, fusion", [(1, 1), (1, 0), (2, 1)],
+                         ids=["m1-tfused", "m1-tsep", "m2"])
 @pytest.mark.parametrize("shape", GRIDS)
 @pytest.mark.parametrize("splits", [(3,), (1, 2)])
-def test_fused_backward_matches_operator_sequence(gpu, oracle, splits, xchunk, shape, mode):
+def test_fused_backward_matches_operator_sequence(gpu, oracle, splits, xchunk, shape, mode,
+                                                  fusion, b):
     """Fused receiver step (tapers, stencil, receiver injection, imaging) --
-    and with mode 2 the source field's backward step in the same pass --
-    against the reference's operator sequence on non-zero fields."""
-    b, order, nt = 12, 8, 8
+    with the tapers fused or in a separate pass, and with mode 2 the source
+    field's backward step in the same pass -- against the reference's
+    operator sequence on non-zero fields; odd and even borders."""
+    gpu.set_taper_fusion("f32", fusion)
+    order, nt = 8, 8
     rng = np.random.default_rng(22)
     Ps, PPs, Pr, PPr, img = (rng.uniform(-1, 1, shape).astype(np.float32) for _ in range(5))
     v2 = rng.uniform(0.0, 0.15, shape).astype(np.float32)
@@ -278,6 +296,7 @@ def test_fused_backward_matches_operator_sequence(gpu, oracle, splits, xchunk, s
         oracle.receivers(PPr, rcv, b, og, nt, it, True)
         oracle.image(img, Ps, PPr, b)
         Pr, PPr = PPr, Pr
+    gpu.set_taper_fusion("f32", -1)
     assert _bits_equal(S.P.download(), Ps) and _bits_equal(S.PP.download(), PPs)
     assert _bits_equal(R.P.download(), Pr) and _bits_equal(R.PP.download(), PPr)
     assert _bits_equal(d["img"].download(), img)

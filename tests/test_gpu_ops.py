@@ -83,7 +83,8 @@ def test_invalid_arguments(gpu, oracle):
 
 
 @pytest.mark.parametrize("upper", [False, True])
-@pytest.mark.parametrize("shape,blen", [((37, 45, 70), 10), ((30, 22, 26), 4), ((64, 70, 90), 20)])
+@pytest.mark.parametrize("shape,blen", [((37, 45, 70), 10), ((30, 22, 26), 4), ((64, 70, 90), 20),
+                                        ((64, 70, 96), 16), ((30, 34, 40), 8)])
 def test_taper(gpu, oracle, upper, shape, blen):
     P = _rand(shape, 4)
     g = oracle.taper_table(blen, np.float32(0.7))

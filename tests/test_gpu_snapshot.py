@@ -29,10 +29,12 @@ def _inputs(seed=3):
     return V, coefs, g, ricker, rcv
 
 
+@pytest.mark.parametrize("fusion", [-1, 1, 0], ids=["tdefault", "tfused", "tsep"])
 @pytest.mark.parametrize("sfmt, k", [("f32", 1), ("f16", 1), ("f32", 3), ("f16", 4)])
 @pytest.mark.parametrize("wfmt", ["f32", "f16", "bf16"])
-def test_snapshot_rtm_matches_oracle(gpu, oracle, wfmt, sfmt, k):
-    print("snapshot rtm", wfmt, sfmt, k, flush=True)
+def test_snapshot_rtm_matches_oracle(gpu, oracle, wfmt, sfmt, k, fusion):
+    print("snapshot rtm", wfmt, sfmt, k, fusion, flush=True)
+    gpu.set_taper_fusion(wfmt, fusion)
     V, coefs, g, ricker, rcv = _inputs()
     wd = gpu.WAVE_FORMATS[wfmt][1]
     shot = gpu.make_shot(*GRID, B, 8, NT, NT, tuple(c + B for c in SRC), gpu.RcvGeom(*RCV), wfmt)
@@ -48,6 +50,7 @@ def test_snapshot_rtm_matches_oracle(gpu, oracle, wfmt, sfmt, k):
             gpu.snap_backward_steps(shot, dV, coefs, dg, R, snaps, sfmt, k, drcv, img, hi, n)
         got = img.download()
     finally:
+        gpu.set_taper_fusion(wfmt, -1)
         for d in (dV, dg, dsrc, drcv, S, R, img, snaps):
             d.free()
     want = oracle.snapshot_migrate_shot(V, coefs, g, ricker, B, 8, NT, SRC,
