@@ -8,6 +8,8 @@
 #include <cstdlib>
 #include <cstring>
 #include <fstream>
+#include <stdexcept>
+#include <string>
 
 #include "param.hpp"
 
@@ -236,25 +238,11 @@ std::vector<float> laplacian_taps(int order)
              (float)(-205. / 72.), (float)(8. / 5.), (float)(-1. / 5.), (float)(8. / 315.),
              (float)(-1. / 560.)};
         break;
-    default: {
-        // Windowed series for other even orders (Misc.cpp:106-137).
-        const float alpha = .54f, beta = 6.f;
-        const float h_beta = (float)(0.5 * beta);
-        const float alpha1 = (float)(2. * alpha - 1.0), alpha2 = (float)(2. * (1.0 - alpha));
-        float central = 0.f;
-        int msign = -1;
-        for (int ix = 1; ix <= order / 2; ix++) {
-            msign = -msign;
-            const float coef_filt = (float)((2. * msign) / (ix * ix));
-            const float arg = (float)(M_PI * ix / (2. * (order / 2 + 2)));
-            const float coef_wind =
-                std::pow(alpha1 + alpha2 * std::cos(arg) * std::cos(arg), h_beta);
-            c[order / 2 + ix] = coef_filt * coef_wind;
-            central = central + c[order / 2 + ix];
-            c[order / 2 - ix] = c[order / 2 + ix];
-        }
-        c[order / 2] = (float)(-2. * central);
-    }
+    default:
+        // the reference derives other even orders from a windowed series
+        // (Misc.cpp:106-137); the device kernels run orders 2 to 8 only
+        throw std::runtime_error("stencil order " + std::to_string(order) +
+                                 " not supported (2, 4, 6 or 8)");
     }
     return c;
 }

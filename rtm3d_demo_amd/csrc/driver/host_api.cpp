@@ -2,6 +2,7 @@
 // scripting and tests: the set-up arithmetic the driver feeds to the device,
 // and an in-process run of a JSON input.
 #include <cstring>
+#include <exception>
 #include <string>
 
 #include "controller.hpp"
@@ -11,10 +12,16 @@ using namespace rtm3d;
 
 extern "C" {
 
-void rtm3d_stencil_coefs(int order, float dx, float dy, float dz, float *out)
+// Returns 0, or 1 for an order the device kernels do not run (out untouched).
+int rtm3d_stencil_coefs(int order, float dx, float dy, float dz, float *out)
 {
-    const std::vector<float> c = stencil_coefs(order, dx, dy, dz);
-    std::memcpy(out, c.data(), c.size() * sizeof(float));
+    try {
+        const std::vector<float> c = stencil_coefs(order, dx, dy, dz);
+        std::memcpy(out, c.data(), c.size() * sizeof(float));
+        return 0;
+    } catch (const std::exception &) {
+        return 1;
+    }
 }
 
 void rtm3d_ricker(int nt, float dt, float fpeak, float *out)

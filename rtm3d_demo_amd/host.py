@@ -21,6 +21,7 @@ def load():
         L = C.CDLL(LIBPATH)
         i, f = C.c_int, C.c_float
         L.rtm3d_stencil_coefs.argtypes = [i, f, f, f, _f32p]
+        L.rtm3d_stencil_coefs.restype = i
         L.rtm3d_ricker.argtypes = [i, f, f, _f32p]
         L.rtm3d_taper_table.argtypes = [i, f, _f32p]
         L.rtm3d_extend_velocity.argtypes = [_f32p, i, i, i, i, i, _f32p]
@@ -35,7 +36,8 @@ def load():
 
 def stencil_coefs(order, dx, dy, dz):
     out = np.zeros(3 * (order + 1), np.float32)
-    load().rtm3d_stencil_coefs(order, dx, dy, dz, out)
+    if load().rtm3d_stencil_coefs(order, dx, dy, dz, out) != 0:
+        raise ValueError("stencil order %d not supported (2, 4, 6 or 8)" % order)
     return out
 
 
