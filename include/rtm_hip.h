@@ -353,6 +353,30 @@ int rtmhip_ae_encode(rtmhip_ae *ae, const void *bricks, int n, void *codes);
 /* Codes + scales of a region's n bricks -> the region's cells of field. */
 int rtmhip_ae_decode(rtmhip_ae *ae, const void *codes, const float *scales, int n, float *field,
                      int nxe, int nye, int nze, int x0, int y0, int z0, int x1, int y1, int z1);
+/* Snapshot RTM with coded snapshots: the loops of rtmhip_snap_forward_steps
+ * and rtmhip_snap_backward_steps (same snapshot steps, same imaging
+ * img += k * (snapshot * PR)) with each snapshot's imaging region (all x, y;
+ * z in [blen, nze-blen)) kept as codes + scales instead of an F32/F16 copy:
+ * the forward pass cuts it into bricks in `work` and encodes them into slot
+ * j of `codes` ([nsnap][bricks][2048] FP16) and `scales`
+ * ([nsnap][bricks] FP32); the backward pass decodes slot j into `work` as a
+ * compact [nxe][nye][nze-2*blen] FP32 field and images with it.  The result
+ * equals rtmhip_snap_backward_steps (F32 snapshots) run on the decoded
+ * snapshots.  FP32 wavefields only (RTMHIP_ERR_UNSUPPORTED otherwise).
+ * rtmhip_snap_ae_sizes gives the bricks per snapshot and the byte sizes of
+ * the three buffers (any output may be NULL).  The autoencoder's workspace is
+ * used by both passes: one shot at a time per autoencoder. */
+int rtmhip_snap_ae_sizes(const rtmhip_shot *shot, int k, int *bricks, size_t *code_bytes,
+                         size_t *scale_bytes, size_t *work_bytes);
+int rtmhip_snap_ae_forward_steps(const rtmhip_shot *shot, const float *V2DT2, const float *coefs,
+                                 const float *taper, const float *ricker, float **P, float **PP,
+                                 rtmhip_ae *ae, void *codes, float *scales, void *work, int k,
+                                 int it0, int nsteps);
+int rtmhip_snap_ae_backward_steps(const rtmhip_shot *shot, const float *V2DT2,
+                                  const float *coefs, const float *taper, float **Pr, float **PPr,
+                                  rtmhip_ae *ae, const void *codes, const float *scales,
+                                  void *work, int k, const float *rcv_chunk, float *img,
+                                  int it_hi, int nsteps);
 /* One matrix-core GEMM of the autoencoder, exposed for tests:
  * C(m,n) (+)= sum_k A(m,k) B(k,n) (+ bias[n], tanh if act), with
  * A(m,k) = A[m*sam + k*sak], B(k,n) = B[k*sbk + n*sbn], FP16 (x_half != 0)

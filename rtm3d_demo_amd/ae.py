@@ -171,6 +171,48 @@ def _decimate(region_field, factors=(2, 2, 4)):
     return out.astype(np.float32)
 
 
+class CodedSnapshots:
+    """Snapshot RTM of one shot with coded snapshots through the library's
+    loops (rtmhip_snap_ae_forward_steps / rtmhip_snap_ae_backward_steps): the
+    snapshot steps are nt-1, nt-1-k, ... and the backward pass images
+    img += k * (decoded snapshot * PR).  Owns the codes ([nsnap][bricks][2048]
+    FP16), scales ([nsnap][bricks] FP32) and the work buffer; FP32 wavefields."""
+
+    def __init__(self, shot, k, codec):
+        nb = C.c_int(0)
+        cb, sb, wb = C.c_size_t(0), C.c_size_t(0), C.c_size_t(0)
+        hip.check(hip.load().rtmhip_snap_ae_sizes(C.byref(shot), k, C.byref(nb), C.byref(cb),
+                                                  C.byref(sb), C.byref(wb)))
+        self.shot, self.k, self.codec, self.nb = shot, k, codec, nb.value
+        self.nsnap = (shot.nt - 1) // k + 1
+        self.codes = hip.DeviceArray((self.nsnap, self.nb, CODES), np.uint16)
+        self.scales = hip.DeviceArray((self.nsnap, self.nb))
+        self.work = hip.DeviceArray((wb.value // 4,))
+        assert self.codes.nbytes == cb.value and self.scales.nbytes == sb.value
+
+    def forward_steps(self, V, coefs, g, ricker, fields, it0, nsteps):
+        c = hip._f32(coefs)
+        hip.check(hip.load().rtmhip_snap_ae_forward_steps(
+            C.byref(self.shot), V.ptr, hip._host_ptr(c), g.ptr, ricker.ptr, C.byref(fields.p),
+            C.byref(fields.pp), self.codec.h, self.codes.ptr, self.scales.ptr, self.work.ptr,
+            self.k, it0, nsteps))
+
+    def backward_steps(self, V, coefs, g, rcv_fields, rcv_chunk, img, it_hi, nsteps):
+        c = hip._f32(coefs)
+        hip.check(hip.load().rtmhip_snap_ae_backward_steps(
+            C.byref(self.shot), V.ptr, hip._host_ptr(c), g.ptr, C.byref(rcv_fields.p),
+            C.byref(rcv_fields.pp), self.codec.h, self.codes.ptr, self.scales.ptr,
+            self.work.ptr, self.k, rcv_chunk.ptr, img.ptr, it_hi, nsteps))
+
+    def nbytes(self):
+        """Bytes kept per shot (codes and scales)."""
+        return self.codes.nbytes + self.scales.nbytes
+
+    def free(self):
+        for d in (self.codes, self.scales, self.work):
+            d.free()
+
+
 def forward_snapshots(shot, V, coefs, g, src, nt, k, stores, upb):
     """HBC forward propagation of one shot; after every k-th step the source
     field at that time (PP after the source injection) goes to each store."""

@@ -718,17 +718,16 @@ inline size_t snap_cells(const rtmhip_shot *shot)
 }
 
 // Forward propagation of snapshot RTM: the HBC forward step sequence of
-// forward_loop (upper tapers fused, frame taper, source) with the source
-// wavefield of every snapshot step stored over the imaging range instead of
-// the upper-border history.
-template <typename T, typename S>
-int snap_forward_loop(const rtmhip_shot *shot, const ShotCtx &c, const float *V2DT2,
-                      const float *taper, const float *ricker, T **P, T **PP, S *snaps, int k,
-                      int it0, int nsteps)
+// forward_loop (upper tapers fused or separate, frame taper, source) with
+// the source wavefield of every snapshot step handed to store(PP, slot)
+// instead of the upper-border history.
+template <typename T, typename Store>
+int snap_forward_run(const rtmhip_shot *shot, const ShotCtx &c, const float *V2DT2,
+                     const float *taper, const float *ricker, T **P, T **PP, int k, int it0,
+                     int nsteps, Store &&store)
 {
     hipStream_t s = g_state.stream;
     const int b = shot->blen;
-    const size_t per = snap_cells(shot);
     const bool sep = g_state.taper_sep[shot->wave_format];
     for (int it = it0; it < it0 + nsteps; ++it) {
         T *t = *P;
@@ -745,24 +744,40 @@ int snap_forward_loop(const rtmhip_shot *shot, const ShotCtx &c, const float *V2
         }
         HIPCHK(launch_source(*PP, c.src_off, ricker, it, 0.f, s), "source");
         const int j = snap_slot(it, shot->nt, k);
-        if (j >= 0)
-            HIPCHK(launch_snap_store(*PP, snaps + (size_t)j * per, c.g, b, s), "snapshot");
+        if (j >= 0) {
+            const int rc = store((const T *)*PP, j);
+            if (rc) return rc;
+        }
     }
     if (!sep) HIPCHK(launch_taper(*P, taper, c.g, b, true, s), "taper");
     return RTMHIP_OK;
 }
 
-// Backward pass of snapshot RTM: only the receiver field propagates (the
-// receiver half of backward_loop: upper tapers fused, frame taper, receiver
-// injection); at every snapshot step img += k * (snapshot * PR).
 template <typename T, typename S>
-int snap_backward_loop(const rtmhip_shot *shot, const ShotCtx &c, const float *V2DT2,
-                       const float *taper, T **Pr, T **PPr, const S *snaps, int k,
-                       const float *rcv_chunk, float *img, int it_hi, int nsteps)
+int snap_forward_loop(const rtmhip_shot *shot, const ShotCtx &c, const float *V2DT2,
+                      const float *taper, const float *ricker, T **P, T **PP, S *snaps, int k,
+                      int it0, int nsteps)
+{
+    const size_t per = snap_cells(shot);
+    return snap_forward_run(shot, c, V2DT2, taper, ricker, P, PP, k, it0, nsteps,
+                            [&](const T *f, int j) {
+                                HIPCHK(launch_snap_store(f, snaps + (size_t)j * per, c.g,
+                                                         shot->blen, g_state.stream),
+                                       "snapshot");
+                                return RTMHIP_OK;
+                            });
+}
+
+// Backward pass of snapshot RTM: only the receiver field propagates (the
+// receiver half of backward_loop: upper tapers, step, receiver injection);
+// at every snapshot step image(PR, slot) applies img += k * (snapshot * PR).
+template <typename T, typename Image>
+int snap_backward_run(const rtmhip_shot *shot, const ShotCtx &c, const float *V2DT2,
+                      const float *taper, T **Pr, T **PPr, int k, const float *rcv_chunk,
+                      int it_hi, int nsteps, Image &&image)
 {
     hipStream_t s = g_state.stream;
     const int b = shot->blen;
-    const size_t per = snap_cells(shot);
     const bool sep = g_state.taper_sep[shot->wave_format];
     for (int it = it_hi; it > it_hi - nsteps; --it) {
         if (sep) {
@@ -779,16 +794,31 @@ int snap_backward_loop(const rtmhip_shot *shot, const ShotCtx &c, const float *V
                                 shot->rcv.count_y, shot->ntstep, it % shot->ntstep, true, s),
                "receivers");
         const int j = snap_slot(it, shot->nt, k);
-        if (j >= 0)
-            HIPCHK(launch_snap_image(img, snaps + (size_t)j * per, (const T *)*PPr, c.g, b,
-                                     (float)k, s),
-                   "snapshot image");
+        if (j >= 0) {
+            const int rc = image((const T *)*PPr, j);
+            if (rc) return rc;
+        }
         T *t = *Pr;
         *Pr = *PPr;
         *PPr = t;
     }
     if (!sep) HIPCHK(launch_taper(*PPr, taper, c.g, b, true, s), "taper");
     return RTMHIP_OK;
+}
+
+template <typename T, typename S>
+int snap_backward_loop(const rtmhip_shot *shot, const ShotCtx &c, const float *V2DT2,
+                       const float *taper, T **Pr, T **PPr, const S *snaps, int k,
+                       const float *rcv_chunk, float *img, int it_hi, int nsteps)
+{
+    const size_t per = snap_cells(shot);
+    return snap_backward_run(shot, c, V2DT2, taper, Pr, PPr, k, rcv_chunk, it_hi, nsteps,
+                             [&](const T *pr, int j) {
+                                 HIPCHK(launch_snap_image(img, snaps + (size_t)j * per, pr, c.g,
+                                                          shot->blen, (float)k, g_state.stream),
+                                        "snapshot image");
+                                 return RTMHIP_OK;
+                             });
 }
 
 int check_snap(const rtmhip_shot *shot, const void *snaps, int snap_format, int k)
@@ -1690,6 +1720,81 @@ int ae_check_region(int nxe, int nye, int nze, int x0, int y0, int z0, int x1, i
     return RTMHIP_OK;
 }
 
+// n FP16 bricks -> n x 2048 FP16 codes, in passes of at most maxb bricks.
+hipError_t ae_encode(const rtmhip_ae *a, const void *bricks, int n, void *codes)
+{
+    hipError_t e = hipSuccess;
+    for (int b0 = 0; b0 < n && e == hipSuccess; b0 += a->maxb) {
+        const int nb = std::min(a->maxb, n - b0);
+        const char *in = static_cast<const char *>(bricks) + (size_t)b0 * kBrickCells * 2;
+        e = ae_layer(a, 0, in, 1, a->h1, nb);
+        if (e == hipSuccess) e = ae_layer(a, 1, a->h1, 0, a->z, nb);
+        if (e == hipSuccess)
+            e = launch_f32_to_f16(a->z, static_cast<char *>(codes) + (size_t)b0 * kCodes * 2,
+                                  kCodes * nb, g_state.stream);
+    }
+    return e;
+}
+
+// Codes + scales of the n bricks of region r -> the region's cells of field.
+hipError_t ae_decode(const rtmhip_ae *a, const void *codes, const float *scales, int n,
+                     const Grid &g, const AeRegion &r, float *field)
+{
+    hipError_t e = hipSuccess;
+    for (int b0 = 0; b0 < n && e == hipSuccess; b0 += a->maxb) {
+        const int nb = std::min(a->maxb, n - b0);
+        e = launch_f16_to_f32(static_cast<const char *>(codes) + (size_t)b0 * kCodes * 2, a->z,
+                              kCodes * nb, g_state.stream);
+        if (e == hipSuccess) e = ae_layer(a, 2, a->z, 0, a->h3, nb);
+        if (e == hipSuccess) e = ae_layer(a, 3, a->h3, 0, a->y, nb);
+        if (e == hipSuccess) e = launch_unbricks(a->y, scales, g, r, field, b0, nb, g_state.stream);
+    }
+    return e;
+}
+
+// The imaging region of a shot's snapshots (all x, y; z in [z0, z0 + lz))
+// as bricks: on the extended grid (z0 = blen) or on a compact
+// [nxe][nye][nze-2*blen] field (z0 = 0).
+AeRegion snap_region(const rtmhip_shot *shot, int z0)
+{
+    const int lz = shot->nze - 2 * shot->blen;
+    AeRegion r;
+    r.x0 = 0;
+    r.y0 = 0;
+    r.z0 = z0;
+    r.x1 = shot->nxe;
+    r.y1 = shot->nye;
+    r.z1 = z0 + lz;
+    r.nbx = (shot->nxe + 31) / 32;
+    r.nby = (shot->nye + 31) / 32;
+    r.nbz = (lz + 31) / 32;
+    return r;
+}
+
+long long snap_region_bricks(const rtmhip_shot *shot)
+{
+    const AeRegion r = snap_region(shot, 0);
+    return (long long)r.nbx * r.nby * r.nbz;
+}
+
+size_t snap_ae_work_bytes(const rtmhip_shot *shot)
+{
+    const size_t bricks = (size_t)snap_region_bricks(shot) * kBrickCells * 2;
+    const size_t field = snap_cells(shot) * 4;
+    return std::max(bricks, field);
+}
+
+int check_snap_ae(const rtmhip_shot *shot, const rtmhip_ae *a, const void *codes,
+                  const float *scales, const void *work, int k)
+{
+    if (!a || !codes || !scales || !work) return fail(RTMHIP_ERR_INVALID, "NULL pointer");
+    if (shot->wave_format != RTMHIP_WAVE_F32)
+        return fail(RTMHIP_ERR_UNSUPPORTED, "coded snapshots need FP32 wavefields");
+    if (snap_region_bricks(shot) > 0x7fffffffLL)
+        return fail(RTMHIP_ERR_INVALID, "imaging region of more than 2^31 bricks");
+    return check_snap(shot, codes, RTMHIP_SNAP_F32, k);
+}
+
 }  // namespace
 
 extern "C" {
@@ -1871,15 +1976,7 @@ int rtmhip_ae_encode(rtmhip_ae *a, const void *bricks, int n, void *codes)
 {
     NEED_INIT();
     if (!a || !bricks || !codes) return fail(RTMHIP_ERR_INVALID, "NULL pointer");
-    for (int b0 = 0; b0 < n; b0 += a->maxb) {
-        const int nb = std::min(a->maxb, n - b0);
-        const char *in = static_cast<const char *>(bricks) + (size_t)b0 * kBrickCells * 2;
-        HIPCHK(ae_layer(a, 0, in, 1, a->h1, nb), "encode");
-        HIPCHK(ae_layer(a, 1, a->h1, 0, a->z, nb), "encode");
-        HIPCHK(launch_f32_to_f16(a->z, static_cast<char *>(codes) + (size_t)b0 * kCodes * 2,
-                                 kCodes * nb, g_state.stream),
-               "encode");
-    }
+    HIPCHK(ae_encode(a, bricks, n, codes), "encode");
     return RTMHIP_OK;
 }
 
@@ -1895,16 +1992,91 @@ int rtmhip_ae_decode(rtmhip_ae *a, const void *codes, const float *scales, int n
     if ((long long)n != (long long)r.nbx * r.nby * r.nbz)
         return fail(RTMHIP_ERR_INVALID, "%d codes for a region of %d bricks", n,
                     r.nbx * r.nby * r.nbz);
-    for (int b0 = 0; b0 < n; b0 += a->maxb) {
-        const int nb = std::min(a->maxb, n - b0);
-        HIPCHK(launch_f16_to_f32(static_cast<const char *>(codes) + (size_t)b0 * kCodes * 2, a->z,
-                                 kCodes * nb, g_state.stream),
-               "decode");
-        HIPCHK(ae_layer(a, 2, a->z, 0, a->h3, nb), "decode");
-        HIPCHK(ae_layer(a, 3, a->h3, 0, a->y, nb), "decode");
-        HIPCHK(launch_unbricks(a->y, scales, g, r, field, b0, nb, g_state.stream), "decode");
-    }
+    HIPCHK(ae_decode(a, codes, scales, n, g, r, field), "decode");
     return RTMHIP_OK;
+}
+
+int rtmhip_snap_ae_sizes(const rtmhip_shot *shot, int k, int *bricks, size_t *code_bytes,
+                         size_t *scale_bytes, size_t *work_bytes)
+{
+    if (!shot) return fail(RTMHIP_ERR_INVALID, "NULL pointer");
+    if (shot->nt <= 0 || shot->nze <= 2 * shot->blen || shot->nxe <= 0 || shot->nye <= 0)
+        return fail(RTMHIP_ERR_INVALID, "bad shot geometry");
+    if (k < 1) return fail(RTMHIP_ERR_INVALID, "snapshot interval %d below 1", k);
+    const long long nb = snap_region_bricks(shot);
+    if (nb > 0x7fffffffLL) return fail(RTMHIP_ERR_INVALID, "imaging region of more than 2^31 bricks");
+    const size_t nsnap = (size_t)((shot->nt - 1) / k + 1);
+    if (bricks) *bricks = (int)nb;
+    if (code_bytes) *code_bytes = nsnap * (size_t)nb * kCodes * 2;
+    if (scale_bytes) *scale_bytes = nsnap * (size_t)nb * 4;
+    if (work_bytes) *work_bytes = snap_ae_work_bytes(shot);
+    return RTMHIP_OK;
+}
+
+int rtmhip_snap_ae_forward_steps(const rtmhip_shot *shot, const float *V2DT2, const float *coefs,
+                                 const float *taper, const float *ricker, float **P, float **PP,
+                                 rtmhip_ae *ae, void *codes, float *scales, void *work, int k,
+                                 int it0, int nsteps)
+{
+    NEED_INIT();
+    ShotCtx c;
+    int rc;
+    if ((rc = prepare_shot(shot, coefs, &c))) return rc;
+    if ((rc = check_format(shot))) return rc;
+    if (!V2DT2 || !ricker || !taper || !P || !PP || !*P || !*PP)
+        return fail(RTMHIP_ERR_INVALID, "NULL pointer");
+    if ((rc = check_snap_ae(shot, ae, codes, scales, work, k))) return rc;
+    if (nsteps <= 0) return RTMHIP_OK;
+    if (it0 < 0 || it0 + nsteps > shot->nt)
+        return fail(RTMHIP_ERR_INVALID, "time range outside [0,%d)", shot->nt);
+    const AeRegion r = snap_region(shot, shot->blen);
+    const long long nb = snap_region_bricks(shot);
+    return snap_forward_run(shot, c, V2DT2, taper, ricker, P, PP, k, it0, nsteps,
+                            [&](const float *f, int j) {
+                                HIPCHK(launch_bricks(f, c.g, r, work, scales + (size_t)j * nb,
+                                                     g_state.stream),
+                                       "snapshot bricks");
+                                HIPCHK(ae_encode(ae, work, (int)nb,
+                                                 static_cast<char *>(codes) +
+                                                     (size_t)j * nb * kCodes * 2),
+                                       "snapshot encode");
+                                return RTMHIP_OK;
+                            });
+}
+
+int rtmhip_snap_ae_backward_steps(const rtmhip_shot *shot, const float *V2DT2,
+                                  const float *coefs, const float *taper, float **Pr, float **PPr,
+                                  rtmhip_ae *ae, const void *codes, const float *scales,
+                                  void *work, int k, const float *rcv_chunk, float *img,
+                                  int it_hi, int nsteps)
+{
+    NEED_INIT();
+    ShotCtx c;
+    int rc;
+    if ((rc = prepare_shot(shot, coefs, &c))) return rc;
+    if ((rc = check_format(shot))) return rc;
+    if (!V2DT2 || !taper || !Pr || !PPr || !*Pr || !*PPr || !rcv_chunk || !img)
+        return fail(RTMHIP_ERR_INVALID, "NULL pointer");
+    if ((rc = check_snap_ae(shot, ae, codes, scales, work, k))) return rc;
+    if (nsteps <= 0) return RTMHIP_OK;
+    if ((rc = check_chunk(shot, it_hi - nsteps + 1, it_hi))) return rc;
+    const int lz = shot->nze - 2 * shot->blen;
+    const AeRegion r = snap_region(shot, 0);
+    const Grid cg = dense_grid(shot->nxe, shot->nye, lz);
+    const long long nb = snap_region_bricks(shot);
+    float *snap = static_cast<float *>(work);
+    return snap_backward_run(shot, c, V2DT2, taper, Pr, PPr, k, rcv_chunk, it_hi, nsteps,
+                             [&](const float *pr, int j) {
+                                 HIPCHK(ae_decode(ae,
+                                                  static_cast<const char *>(codes) +
+                                                      (size_t)j * nb * kCodes * 2,
+                                                  scales + (size_t)j * nb, (int)nb, cg, r, snap),
+                                        "snapshot decode");
+                                 HIPCHK(launch_snap_image(img, (const float *)snap, pr, c.g,
+                                                          shot->blen, (float)k, g_state.stream),
+                                        "snapshot image");
+                                 return RTMHIP_OK;
+                             });
 }
 
 int rtmhip_ae_gemm_test(const void *A, long long sam, long long sak, int a_half, const void *B,

@@ -30,6 +30,7 @@ EXPORTED = [
     "rtmhip_ae_param_count", "rtmhip_ae_create", "rtmhip_ae_destroy", "rtmhip_ae_get_params",
     "rtmhip_ae_set_params", "rtmhip_ae_gradient", "rtmhip_ae_train", "rtmhip_ae_bricks",
     "rtmhip_ae_encode", "rtmhip_ae_decode", "rtmhip_ae_gemm_test", "rtmhip_freq_image",
+    "rtmhip_snap_ae_sizes", "rtmhip_snap_ae_forward_steps", "rtmhip_snap_ae_backward_steps",
 ]
 
 
@@ -127,6 +128,13 @@ def load():
         "rtmhip_ae_gemm_test": [vp, C.c_longlong, C.c_longlong, i, vp, C.c_longlong,
                                 C.c_longlong, i, fp, C.c_longlong, fp, C.c_longlong, i,
                                 C.c_longlong, i, i],
+        "rtmhip_snap_ae_sizes": [C.POINTER(Shot), i, C.POINTER(C.c_int),
+                                 C.POINTER(C.c_size_t), C.POINTER(C.c_size_t),
+                                 C.POINTER(C.c_size_t)],
+        "rtmhip_snap_ae_forward_steps": [C.POINTER(Shot), fp, fp, fp, fp, pp, pp, vp, vp, fp,
+                                         vp, i, i, i],
+        "rtmhip_snap_ae_backward_steps": [C.POINTER(Shot), fp, fp, fp, pp, pp, vp, vp, fp, vp,
+                                          i, fp, fp, i, i],
     }
     for name, args in sig.items():
         f = getattr(L, name)
