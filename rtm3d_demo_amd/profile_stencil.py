@@ -27,10 +27,18 @@ tile shape TZxBYxRY, or "default").
 --tiling C:TZ,BY,RY (repeatable) sets the tile shape of stencil class C for
 the sequence; give the same options to --summarize so that the lines carry
 the shapes.
+
+--wave-format f16 (or bf16) runs the 16-bit sequence instead: k_stack, then
+5 HBC forward steps, 7 fused backward steps (mode 1) and 5 + 5 snapshot-RTM
+steps (16-bit snapshots) with 16-bit wavefields and the library's defaults
+(separate upper-taper pass, two-z waves).  Its summary (--summarize with the
+same option) gives per-kernel bytes and the 16-bit algorithmic bytes per
+updated cell: 10 for a step (V stays FP32), 20 for the imaging step.
 """
 import csv
 import glob
 import os
+import re
 import sys
 
 import numpy as np
@@ -60,7 +68,37 @@ def _per_dispatch(d, counter):
     return {k: (sum(v) / len(v), len(v)) for k, v in out.items()}
 
 
+_MANGLED_TYPES = {"f": "float", "DF16_": "_Float16", "DF16b": "__bf16", "u6__bf16": "__bf16"}
+
+
+def _demangle_args(args):
+    """Template arguments of an Itanium-mangled kernel name (the subset the
+    library's kernels use: f, DF16_, DF16b / u6__bf16, Li<n>E, Lb<0|1>E)."""
+    out = []
+    i = 0
+    while i < len(args):
+        for code, name in _MANGLED_TYPES.items():
+            if args.startswith(code, i):
+                out.append(name)
+                i += len(code)
+                break
+        else:
+            m = re.match(r"L([ib])(\d+)E", args[i:])
+            if not m:
+                break
+            out.append(("true" if m.group(2) == "1" else "false") if m.group(1) == "b"
+                       else m.group(2))
+            i += m.end()
+    return out
+
+
 def _short(name):
+    if name.startswith("_Z"):
+        # names the profiler left mangled (16-bit template arguments)
+        m = re.search(r"\d+(k_\w+?)I(.*?)EEv", name)
+        if m:
+            return "%s<%s>" % (m.group(1), ", ".join(_demangle_args(m.group(2))))
+        return name
     name = name.replace("rtmhip::(anonymous namespace)::", "").replace("void ", "")
     return name.split("(")[0]
 
@@ -75,7 +113,7 @@ def parse_tilings(argv):
     return out
 
 
-def summarize(fetch_dir, write_dir, tilings=None):
+def summarize(fetch_dir, write_dir, tilings=None, wave_format="f32"):
     tilings = tilings or {}
     fetch = _per_dispatch(fetch_dir, "FETCH_SIZE")
     write = _per_dispatch(write_dir, "WRITE_SIZE")
@@ -112,6 +150,16 @@ def summarize(fetch_dir, write_dir, tilings=None):
         if cls is not None and cls not in first:
             first[cls] = (short, r + w)
     print()
+    if wave_format != "f32":
+        # 16-bit wavefields: P, PP read, PP written (2 B each), V (4 B); the
+        # imaging step adds PS (2 B) and the FP32 image read and written
+        for cls in sorted(first):
+            short, t = first[cls]
+            bpc = 20.0 if cls == 1 else 10.0
+            print("%s class %d (%s): algorithmic %.0f B per launch (%.0f B x %d cells), "
+                  "traffic %.0f B (%.2f x)" % (wave_format, cls, short, bpc * interior, bpc,
+                                               interior, t, t / (bpc * interior)))
+        return
     algo = {0: 16.0, 1: 28.0, 2: 16.0, 3: 36.0}
     for cls in sorted(first):
         short, t = first[cls]
@@ -121,12 +169,59 @@ def summarize(fetch_dir, write_dir, tilings=None):
         print("traffic_bytes_per_launch[class %d, tile %s]: %.0f" % (cls, tile, t))
 
 
+def parse_wave_format(argv):
+    return argv[argv.index("--wave-format") + 1] if "--wave-format" in argv else "f32"
+
+
+def run16(hip, host, fmt):
+    """The 16-bit launch sequence (see the module docstring)."""
+    shape = SHAPE
+    b, order, nt = 40, 8, 16
+    rng = np.random.default_rng(0)
+
+    def rand(lo, hi):
+        out = np.empty(shape, np.float32)
+        for x in range(shape[0]):
+            out[x] = rng.uniform(lo, hi, shape[1:])
+        return out
+
+    wd = hip.WAVE_FORMATS[fmt][1]
+    V = hip.DeviceArray.from_host(rand(0.0, 0.2))
+    img = hip.DeviceArray(shape).zero()
+    coefs = host.stencil_coefs(order, 12.5, 12.5, 12.5)
+    g = hip.DeviceArray.from_host(host.taper_table(b, 0.9))
+    src = hip.DeviceArray.from_host(host.ricker(nt, 1.1e-3, 25.0))
+    F, R = hip.FieldPair(shape, wd), hip.FieldPair(shape, wd)
+    for f in (F.a, F.b, R.a, R.b):
+        f.zero()
+    upb = hip.DeviceArray((nt, shape[0], shape[1], 4), wd).zero()
+    rcv = hip.DeviceArray((128, 128, nt)).zero()
+    geom = hip.RcvGeom(0, 0, 0, 4, 4, 128, 128)
+    shot = hip.make_shot(*shape, b, order, nt, nt, (296, 296, 42), geom, fmt)
+    for _ in range(3):
+        hip.stack(img, V)
+    hip.forward_steps(shot, True, V, coefs, g, src, F, upb, 0, 5)
+    hip.backward_steps(shot, True, V, coefs, g, F, R, upb, rcv, img, nt - 1, 7, True)
+    upb.free()
+    snaps = hip.snap_buffer(shot, "f16", 1)
+    hip.snap_forward_steps(shot, V, coefs, g, src, F, snaps, "f16", 1, 0, 5)
+    hip.snap_backward_steps(shot, V, coefs, g, R, snaps, "f16", 1, rcv, img, nt - 1, 5)
+    hip.synchronize()
+    print("profile sequence done", flush=True)
+
+
 def main():
     tilings = parse_tilings(sys.argv)
+    wfmt = parse_wave_format(sys.argv)
     if len(sys.argv) >= 4 and sys.argv[1] == "--summarize":
-        summarize(sys.argv[2], sys.argv[3], tilings)
+        summarize(sys.argv[2], sys.argv[3], tilings, wfmt)
         return 0
     from rtm3d_demo_amd import hip, host
+    if wfmt != "f32":
+        hip.init(0)
+        run16(hip, host, wfmt)
+        hip.finalize()
+        return 0
 
     shape = SHAPE
     b, order, nt = 40, 8, 16
