@@ -81,6 +81,19 @@ def cut_bricks(field, region, bricks, scales):
                                           scales.ptr))
 
 
+def collect_bricks(snaps, slots, pool, row, scales):
+    """Cuts the given slots of an F32 snapshot buffer
+    ([nsnap][nxe][nye][nze-2blen], device) into FP16 bricks at rows row, ...
+    of pool ([n][32768], device); scales ([bricks], device) is scratch.
+    Returns the next free row."""
+    region = (0, 0, 0) + tuple(snaps.shape[1:])
+    nb = region_bricks(region)
+    for j in slots:
+        cut_bricks(snaps.slot(j), region, pool.view(row, nb), scales)
+        row += nb
+    return row
+
+
 class SnapshotStore:
     """Source snapshots every `k` steps: 'exact' (FP32 copies of the imaging
     region's grid), 'ae' (codes + scales), or 'decimate' (the same storage

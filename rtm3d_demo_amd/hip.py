@@ -189,9 +189,26 @@ class DeviceArray:
         return self
 
     def free(self):
-        if self.ptr:
+        if self.ptr and not getattr(self, "base", None):
             check(load().rtmhip_free(self.ptr))
-            self.ptr = C.c_void_p()
+        self.ptr = C.c_void_p()
+
+    def view(self, start, count):
+        """Non-owning view of elements [start, start+count) of the leading
+        axis (valid while this array lives; free() on it releases nothing)."""
+        assert 0 <= start and count > 0 and start + count <= self.shape[0]
+        v = DeviceArray.__new__(DeviceArray)
+        v.shape, v.dtype, v.base = (count,) + self.shape[1:], self.dtype, self
+        row = self.nbytes // self.shape[0]
+        v.nbytes = row * count
+        v.ptr = C.c_void_p(self.ptr.value + start * row)
+        return v
+
+    def slot(self, i):
+        """Non-owning view of element i of the leading axis, without that axis."""
+        v = self.view(i, 1)
+        v.shape = v.shape[1:]
+        return v
 
     @classmethod
     def from_host(cls, a, dtype=np.float32):
