@@ -13,7 +13,7 @@
 //   L3  transposed L2, 32 -> 8, tanh           [64 rows x 32]   * [32 x 512]
 //   L4  transposed L1, 8 -> 1                  [4096 rows x 8]  * [8 x 8]
 // so one brick of 32768 FP16 values (64 KB) becomes 2048 FP16 codes (4 KB)
-// plus its scale.  The GEMMs run on the matrix cores (MFMA 16x16x16, FP16
+// plus its scale.  The GEMMs run on the matrix cores (MFMA 16x16x32, FP16
 // operands, FP32 accumulation) for inference and for training (forward,
 // backward, split-K weight gradients).
 #include "rtm_internal.hpp"
@@ -23,7 +23,7 @@
 namespace rtmhip {
 namespace {
 
-typedef _Float16 half4_t __attribute__((ext_vector_type(4)));
+typedef _Float16 half8_t __attribute__((ext_vector_type(8)));
 typedef float float4_t __attribute__((ext_vector_type(4)));
 
 constexpr int kGemmThreads = 256;  // 4 waves; 64 x 64 tile of C, 32 x 32 per wave
@@ -38,10 +38,18 @@ __device__ __forceinline__ float load_elem(const void *p, long long i, int is_ha
 // B(k, n) = B[k*sbk + n*sbn] (either FP32 or FP16 in memory, FP16 operands).
 // Epilogue: + bias[n], tanh (act 1); or atomic accumulation of a split-K
 // partial (accumulate != 0; blockIdx.z selects the K range).
+//
+// gfx950's MFMA 16x16x32 (FP16 in, FP32 accumulate): lane l supplies row
+// l % 16 of A and column l % 16 of B with eight k values each, k-slot group
+// l / 16; A and B use the same k assignment, so the sum over k is the same
+// whichever k each slot of the instruction stands for.  D(i, j) of a tile is
+// lane j + 16 * (i / 4), item i % 4.  Both operands sit in LDS k-contiguous
+// (B transposed), so each lane reads its eight values as one 16-byte load.
 __global__ void __launch_bounds__(kGemmThreads) k_gemm(AeGemm p)
 {
-    __shared__ _Float16 As[64][16 + 4];
-    __shared__ _Float16 Bs[16][64 + 4];
+    constexpr int KT = 32, KP = KT + 8;  // k per tile; padded LDS row (80 B)
+    __shared__ __align__(16) _Float16 As[64][KP];
+    __shared__ __align__(16) _Float16 Bt[64][KP];
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const long long m0 = (long long)blockIdx.x * 64;
     const int n0 = blockIdx.y * 64;
@@ -51,13 +59,13 @@ __global__ void __launch_bounds__(kGemmThreads) k_gemm(AeGemm p)
     float4_t acc[2][2];
     for (int i = 0; i < 2; ++i)
         for (int j = 0; j < 2; ++j) acc[i][j] = float4_t{0.f, 0.f, 0.f, 0.f};
-    for (long long k0 = kb; k0 < ke; k0 += 16) {
+    for (long long k0 = kb; k0 < ke; k0 += KT) {
         __syncthreads();
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const int idx = tid * 4 + q;  // 0 .. 1023
+        for (int q = 0; q < 8; ++q) {
+            const int idx = tid * 8 + q;  // 0 .. 2047
             {
-                const int mm = idx >> 4, kk = idx & 15;
+                const int mm = idx >> 5, kk = idx & 31;
                 const long long m = m0 + mm, k = k0 + kk;
                 float v = 0.f;
                 if (m < p.M && k < ke) v = load_elem(p.A, m * p.sam + k * p.sak, p.a_half);
@@ -69,26 +77,21 @@ __global__ void __launch_bounds__(kGemmThreads) k_gemm(AeGemm p)
                 const int n = n0 + nn;
                 float v = 0.f;
                 if (n < p.N && k < ke) v = load_elem(p.B, k * p.sbk + (long long)n * p.sbn, p.b_half);
-                Bs[kk][nn] = (_Float16)v;
+                Bt[nn][kk] = (_Float16)v;
             }
         }
         __syncthreads();
-        const int r = lane & 15, kq = (lane >> 4) * 4;
+        const int r = lane & 15, kq = (lane >> 4) * 8;
 #pragma unroll
         for (int i = 0; i < 2; ++i) {
-            half4_t a;
-#pragma unroll
-            for (int q = 0; q < 4; ++q) a[q] = As[wm + 16 * i + r][kq + q];
+            const half8_t a = *reinterpret_cast<const half8_t *>(&As[wm + 16 * i + r][kq]);
 #pragma unroll
             for (int j = 0; j < 2; ++j) {
-                half4_t b;
-#pragma unroll
-                for (int q = 0; q < 4; ++q) b[q] = Bs[kq + q][wn + 16 * j + r];
-                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16f16(a, b, acc[i][j], 0, 0, 0);
+                const half8_t b = *reinterpret_cast<const half8_t *>(&Bt[wn + 16 * j + r][kq]);
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, acc[i][j], 0, 0, 0);
             }
         }
     }
-    // D(i, j) of a 16 x 16 tile: lane j + 16 * (i / 4), item i % 4
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
@@ -313,10 +316,10 @@ hipError_t launch_ae_gemm(const AeGemm &p, hipStream_t s)
             const long long want = std::max(1LL, 1024 / std::max(1LL, tiles));
             q.kchunk = (int)std::min(65536LL, std::max(256LL, (q.K + want - 1) / want));
         }
-        q.kchunk = (q.kchunk + 15) / 16 * 16;
+        q.kchunk = (q.kchunk + 31) / 32 * 32;
         splits = (q.K + q.kchunk - 1) / q.kchunk;
     } else {
-        q.kchunk = (int)((q.K + 15) / 16 * 16);
+        q.kchunk = (int)((q.K + 31) / 32 * 32);
     }
     const long long gm = (q.M + 63) / 64, gn = (q.N + 63) / 64;
     if (gm > 0x7fffffffLL || gn > 65535 || splits > 65535) return hipErrorInvalidValue;
