@@ -423,8 +423,13 @@ int rtmhip_get_tiling(int kernel_class, int *tz, int *by, int *ry, int *xchunk);
  * (tapered on load; a tapered kernel holds more registers), mode 0 runs them
  * as one in-place pass over both fields' upper slabs followed by the
  * untapered stencil, mode -1 restores the default (fused for F32, separate
- * for F16 and BF16).  Results are identical in both modes.  Batches and
- * fused = 2 always fuse. */
+ * for F16 and BF16).  Mode 2 also fuses the every-border tapers of
+ * rtmhip_model_steps (taper3D of both fields before each step,
+ * RTMFiniteDifferencesKernel.cpp:180-181; stencil order 8), which otherwise
+ * run as one in-place pass (the default: on the 592x592x336 salt grid the
+ * fused step is no faster).  Results are identical in every mode.  Batches
+ * and fused = 2 always fuse the upper tapers.  rtmhip_get_taper_fusion
+ * returns the mode (0, 1 or 2). */
 int rtmhip_set_taper_fusion(int wave_format, int mode);
 int rtmhip_get_taper_fusion(int wave_format, int *fused);
 

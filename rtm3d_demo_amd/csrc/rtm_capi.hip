@@ -51,8 +51,10 @@ struct State {
     StencilTiling tiling[4] = {};  // per kernel class; tz == 0: built-in default
     int tiling_gen = 0;            // bumped by every rtmhip_set_tiling call
     // per wave format: the HBC upper tapers of the single-shot loops applied
-    // in place by a separate pass (true) or fused into the stencil (false)
+    // in place by a separate pass (true) or fused into the stencil (false);
+    // the modeling loop's every-border tapers fused (true) or separate
     bool taper_sep[3] = {false, true, true};
+    bool taper_all_fused[3] = {false, false, false};
 };
 constexpr bool kTaperSepDefault[3] = {false, true, true};
 
@@ -530,19 +532,38 @@ int model_loop(const rtmhip_shot *shot, const ShotCtx &c, const float *V2DT2, co
                const float *ricker, T **P, T **PP, float *rcv_chunk, int it0, int nsteps)
 {
     hipStream_t s = g_state.stream;
+    // With the every-border fusion (rtmhip_set_taper_fusion mode 2, order 8)
+    // the two in-place taper3D passes of each step are fused into the
+    // stencil as for the HBC upper taper: P tapered on load, the centre PP
+    // `pp_taps` times, PP's frame (the cells the step does not update)
+    // tapered in place before the step; the taper still pending on P after
+    // the last step is applied in place, so the state between calls is the
+    // reference's.  Off by default: on the salt grid the fused step
+    // (one-z waves with a taper chain on every value) is no faster than the
+    // separate pass plus the two-z untapered step.
+    const int b = shot->blen;
+    const bool fused = g_state.taper_all_fused[shot->wave_format] && c.radius == 4 && b > 0;
     for (int it = it0; it < it0 + nsteps; ++it) {
         T *t = *P;
         *P = *PP;
         *PP = t;
-        // taperAllBorders of P, then of PP: disjoint cells, one pass
-        HIPCHK(launch_taper_all2(*P, *PP, taper, c.g, shot->blen, s), "taper");
-        HIPCHK(stencil(c.radius, *P, *PP, V2DT2, c.cf, c.g, nullptr), "stencil");
+        if (fused) {
+            const TaperFuse tf{taper, b, it == it0 ? 1 : 2, 1};
+            HIPCHK(launch_taper_frame_all(*PP, taper, c.g, b, c.radius, tf.pp_taps, s),
+                   "frame taper");
+            HIPCHK(stencil(c.radius, *P, *PP, V2DT2, c.cf, c.g, nullptr, &tf), "stencil");
+        } else {
+            // taperAllBorders of P, then of PP: disjoint cells, one pass
+            HIPCHK(launch_taper_all2(*P, *PP, taper, c.g, b, s), "taper");
+            HIPCHK(stencil(c.radius, *P, *PP, V2DT2, c.cf, c.g, nullptr), "stencil");
+        }
         HIPCHK(launch_source(*PP, c.src_off, ricker, it, 0.f, s), "source");
         HIPCHK(launch_receivers(*PP, rcv_chunk, c.g, c.rx0, c.ry0, c.rz, shot->rcv.dist_x,
                                 shot->rcv.dist_y, shot->rcv.count_x, shot->rcv.count_y,
                                 shot->ntstep, it % shot->ntstep, false, s),
                "receivers");
     }
+    if (fused && nsteps > 0) HIPCHK(launch_taper(*P, taper, c.g, b, false, s), "taper");
     return RTMHIP_OK;
 }
 
@@ -989,10 +1010,12 @@ int rtmhip_set_taper_fusion(int wave_format, int mode)
     NEED_INIT();
     if (wave_format < RTMHIP_WAVE_F32 || wave_format > RTMHIP_WAVE_BF16)
         return fail(RTMHIP_ERR_INVALID, "unknown wave_format %d", wave_format);
-    if (mode < -1 || mode > 1)
-        return fail(RTMHIP_ERR_INVALID, "taper fusion mode %d (1 fused, 0 separate, -1 default)",
-                    mode);
+    if (mode < -1 || mode > 2)
+        return fail(RTMHIP_ERR_INVALID,
+                    "taper fusion mode %d (2 every taper fused, 1 upper tapers fused, "
+                    "0 separate, -1 default)", mode);
     g_state.taper_sep[wave_format] = mode < 0 ? kTaperSepDefault[wave_format] : mode == 0;
+    g_state.taper_all_fused[wave_format] = mode == 2;
     return RTMHIP_OK;
 }
 
@@ -1001,7 +1024,7 @@ int rtmhip_get_taper_fusion(int wave_format, int *fused)
     NEED_INIT();
     if (wave_format < RTMHIP_WAVE_F32 || wave_format > RTMHIP_WAVE_BF16 || !fused)
         return fail(RTMHIP_ERR_INVALID, "bad arguments");
-    *fused = g_state.taper_sep[wave_format] ? 0 : 1;
+    *fused = g_state.taper_all_fused[wave_format] ? 2 : (g_state.taper_sep[wave_format] ? 0 : 1);
     return RTMHIP_OK;
 }
 

@@ -70,6 +70,7 @@ struct TaperFuse {
     const float *t;  // device table g[b]
     int b;
     int pp_taps;     // 1 or 2
+    int all;         // 0: the upper taper (taperUpper3D); 1: every border (taper3D)
 };
 
 // Tile configuration of the stencil kernel.
@@ -143,6 +144,13 @@ hipError_t launch_taper_all2(T *P, T *Q, const float *taper, const Grid &g, int 
 template <typename T>
 hipError_t launch_taper_frame(T *P, const float *taper, const Grid &g, int b, int R, int taps,
                               hipStream_t s);
+// taper3D's counterpart of launch_taper_frame (fused every-border taper):
+// the cells within R of any grid face, `taps` times in place, the front-right
+// edge's cells from the cell (b, y, z) as taper3D reads it.  Runs before the
+// step that reads the field's plane b as the step's PP.
+template <typename T>
+hipError_t launch_taper_frame_all(T *P, const float *taper, const Grid &g, int b, int R, int taps,
+                                  hipStream_t s);
 template <typename T>
 hipError_t launch_source(T *PP, long long offset, const float *table, int it, float value,
                          hipStream_t s);

@@ -297,6 +297,95 @@ __global__ void __launch_bounds__(kThreads)
     P[o] = wst<T>(frame_taper<T>(wld(P[o]), t, g, b, x, y, z, taps));
 }
 
+// taper3D at one cell, `taps` times (rounded to T between applications):
+// on the z border ((v*fy)*fx)*fz, off it (v*fx)*fy -- the x-y edge pass's
+// order, and the face passes' with the other factor 1 (exact).
+template <typename T>
+__device__ __forceinline__ float all_taper(float v, const float *__restrict__ t, const Grid &g,
+                                           int b, int x, int y, int z, int taps)
+{
+    const bool bx = x < b || x >= g.nx - b;
+    const bool by = y < b || y >= g.ny - b;
+    const bool bz = z < b || z >= g.nz - b;
+    const float fx = bx ? gfac(t, x, g.nx, b) : 1.0f;
+    const float fy = by ? gfac(t, y, g.ny, b) : 1.0f;
+    const float fz = bz ? gfac(t, z, g.nz, b) : 1.0f;
+    for (int k = 0; k < taps; ++k) {
+        if (k) v = wrnd<T>(v);
+        if (bz) {
+            v *= fy;
+            v *= fx;
+            v *= fz;
+        } else {
+            v *= fx;
+            v *= fy;
+        }
+    }
+    return v;
+}
+
+// The frame of the fused every-border step: the cells with x or y within R
+// of a grid face, every z (the z frame of the other rows is stored by the
+// stencil itself), except the front-right edge cells (x < b, y >= ny-b, z
+// inside), tapered `taps` times in place.  Index: the x frame (2R planes),
+// then the y frame of the inner planes; z fastest.
+template <typename T>
+__global__ void __launch_bounds__(kThreads)
+    k_taper_frame_all(T *__restrict__ P, const float *__restrict__ t, Grid g, int b, int R,
+                      int taps, int nX, int n)
+{
+    int i = (int)(blockIdx.x * kThreads + threadIdx.x);  // n < 2^31 (launcher)
+    if (i >= n) return;
+    const int z = i % g.nz;
+    int x, y;
+    if (i < nX) {
+        const int r = i / g.nz;
+        y = r % g.ny;
+        const int xx = r / g.ny;
+        x = xx < R ? xx : g.nx - 2 * R + xx;
+    } else {
+        const int r = (i - nX) / g.nz;
+        const int yy = r % (2 * R);
+        x = R + r / (2 * R);
+        y = yy < R ? yy : g.ny - 2 * R + yy;
+    }
+    if (x < b && y >= g.ny - b && z >= b && z < g.nz - b) return;  // k_taper_frame_quirk
+    const long long o = goff(g, x, y, z);
+    P[o] = wst<T>(all_taper<T>(wld(P[o]), t, g, b, x, y, z, taps));
+}
+
+// The front-right edge cells of that frame (x < R, or y >= ny-R), from the
+// cell (b, y, z) before this step's tapers: the face pass's (v*gy) rounded,
+// `taps` times, then (v*gx)*gy.  Runs before k_taper_frame_all, which
+// rescales the frame cells of plane b.
+template <typename T>
+__global__ void __launch_bounds__(kThreads)
+    k_taper_frame_quirk(T *__restrict__ P, const float *__restrict__ t, Grid g, int b, int R,
+                        int taps, int n1, int n)
+{
+    const int i = (int)(blockIdx.x * kThreads + threadIdx.x);  // n < 2^31 (launcher)
+    if (i >= n) return;
+    const int lz = g.nz - 2 * b;
+    const int z = b + i % lz;
+    int x, iy;
+    if (i < n1) {  // x < R, y in [ny-b, ny)
+        const int r = i / lz;
+        iy = r % b;
+        x = r / b;
+    } else {  // x in [R, b), y in [ny-R, ny)
+        const int r = (i - n1) / lz;
+        iy = b - R + r % R;
+        x = R + r / R;
+    }
+    const int y = g.ny - b + iy;
+    const float gy = t[iy];
+    float v = wld(P[goff(g, b, y, z)]);
+    for (int k = 0; k < taps; ++k) v = wrnd<T>(v * gy);
+    v *= t[b - 1 - x];
+    v *= gy;
+    P[goff(g, x, y, z)] = wst<T>(v);
+}
+
 // The front-right edge of taper3D (RTMCPUPlatform.cpp:900-904) writes cells
 // x < blen, y >= ny-blen, z interior from the face-tapered cell
 // (blen, y, z).  This kernel runs before k_taper_all so it reads that cell
@@ -931,6 +1020,29 @@ hipError_t launch_taper_frame(T *P, const float *taper, const Grid &g, int b, in
 }
 
 template <typename T>
+hipError_t launch_taper_frame_all(T *P, const float *taper, const Grid &g, int b, int R, int taps,
+                                  hipStream_t s)
+{
+    if (b <= 0 || taps <= 0) return hipSuccess;
+    if (b < R || 2 * b >= g.nx || 2 * b >= g.ny || 2 * b >= g.nz) return hipErrorInvalidValue;
+    const long long nX = (long long)2 * R * g.ny * g.nz;
+    const long long n = nX + (long long)(g.nx - 2 * R) * 2 * R * g.nz;
+    const long long lz = g.nz - 2 * b;
+    const long long n1 = (long long)R * b * lz;
+    const long long nq = n1 + (long long)(b - R) * R * lz;
+    if (n + kThreads >= (1LL << 31)) return hipErrorInvalidValue;
+    if (nq > 0) {
+        hipLaunchKernelGGL(k_taper_frame_quirk<T>, dim3(nblocks(nq)), dim3(kThreads), 0, s, P,
+                           taper, g, b, R, taps, (int)n1, (int)nq);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(k_taper_frame_all<T>, dim3(nblocks(n)), dim3(kThreads), 0, s, P, taper, g,
+                       b, R, taps, (int)nX, (int)n);
+    return hipGetLastError();
+}
+
+template <typename T>
 hipError_t launch_source(T *PP, long long offset, const float *table, int it, float value,
                          hipStream_t s)
 {
@@ -1107,6 +1219,8 @@ hipError_t launch_tick(int *p, int d, bool set, hipStream_t s)
                                                hipStream_t);                                    \
     template hipError_t launch_taper_frame<T>(T *, const float *, const Grid &, int, int, int,  \
                                               hipStream_t);                                     \
+    template hipError_t launch_taper_frame_all<T>(T *, const float *, const Grid &, int, int,   \
+                                                  int, hipStream_t);                            \
     template hipError_t launch_source<T>(T *, long long, const float *, int, float, hipStream_t); \
     template hipError_t launch_receivers<T>(T *, float *, const Grid &, int, int, int, int, int, \
                                             int, int, int, int, bool, hipStream_t);             \

@@ -631,7 +631,7 @@ __global__ void __launch_bounds__(64 * NWM * BY)
 // loads hit lines their neighbours fetch.
 // Minimum waves per SIMD asked of the compiler for the barrier-free kernel:
 // the register budget (512 / waves) it must fit.
-template <typename T, int RY, bool IMG, bool TAPER>
+template <typename T, int RY, bool IMG, int TAPER>
 constexpr int wave_min_occupancy()
 {
     // (every storage type: the tapered 8-row form spills a few bytes at 3
@@ -650,7 +650,16 @@ struct WaveGrid {
 // y0 = R + yt*RY .. +RY, planes [xb, xe); lt is the wave's private LDS rows,
 // tab the taper table in LDS (TAPER).  Shared by the single-grid and the
 // batched kernels.
-template <typename T, int R, int RY, bool IMG, bool TAPER>
+//
+// TAPER 1 fuses the upper taper (taperUpper3D), TAPER 2 the taper of every
+// border (taper3D): a value is multiplied, in the reference's per-region
+// order, by ((v*fy)*fx)*fz on the z border and by (v*fx)*fy off it, factors
+// 1 off their border.  taper3D's front-right edge (x < b, y >= ny-b, z
+// inside) takes the face-tapered value of (b, y, z): those lanes load plane
+// b (a uniform byte offset on their lane offset) and apply
+// ((v*fy) rounded *fx)*fy.  With TAPER 2 the wave also stores the tapered
+// PP of its z-frame lanes (z < R, z >= nz-R), which the step does not update.
+template <typename T, int R, int RY, bool IMG, int TAPER>
 __device__ __forceinline__ void wave_tile(const T *__restrict__ P, T *__restrict__ PP,
                                           const float *__restrict__ V, const Coefs &cf,
                                           const Grid &g, int zt, int yt, int xb, int xe,
@@ -676,14 +685,19 @@ __device__ __forceinline__ void wave_tile(const T *__restrict__ P, T *__restrict
         const int j = i < tf.b ? tf.b - 1 - i : (i >= n - tf.b ? i - (n - tf.b) : -1);
         return j >= 0 ? tab[max(j, 0)] : 1.0f;
     };
-    const bool wtap = TAPER && (z0 - R < tf.b);  // wave holds some z < b (halo included)
-    const bool ztap = wtap && (z < tf.b);
+    constexpr bool ALL = TAPER == 2;
+    // the wave holds a tapered value: some z < b (halo included), or any (ALL)
+    const bool wtap = TAPER && (ALL || z0 - R < tf.b);
+    const int zq = (int)zc;
+    const bool ztap = wtap && (ALL ? (zq < tf.b || zq >= g.nz - tf.b) : (z < tf.b));
     float fzl = 1.0f;
-    if (ztap) fzl = tab[tf.b - 1 - z];
+    if (ztap) fzl = ALL ? lat(zq, g.nz) : tab[tf.b - 1 - z];
+    const bool zmid = ALL && !ztap;  // front-right lanes (on the rows and planes concerned)
+    const bool zframe = ALL && z < g.nz && !zcomp;
 
     // own rows (uniform)
     unsigned rowb[RY];
-    unsigned ymask = 0, rmask = 0;
+    unsigned ymask = 0, rmask = 0, qmask = 0;
     float fyr[RY];
 #pragma unroll
     for (int r = 0; r < RY; ++r) {
@@ -691,6 +705,7 @@ __device__ __forceinline__ void wave_tile(const T *__restrict__ P, T *__restrict
         const int yc = min(y, g.ny - 1);
         rowb[r] = (unsigned)yc * (unsigned)g.sy * ES;
         if (y < g.ny - R) ymask |= 1u << r;
+        if (ALL && yc >= g.ny - tf.b) qmask |= 1u << r;
         if (IMG) {
             const int dy = y - fz.ry0;
             if (dy >= 0 && (dy % fz.rdy) == 0 && (dy / fz.rdy) < fz.rny) rmask |= 1u << r;
@@ -701,11 +716,13 @@ __device__ __forceinline__ void wave_tile(const T *__restrict__ P, T *__restrict
     }
     // y-halo rows (uniform): h < R above the tile, h >= R below
     unsigned yhb[2 * R];
+    unsigned qhmask = 0;
     float yhf[2 * R];
 #pragma unroll
     for (int h = 0; h < 2 * R; ++h) {
         const int gy = min(h < R ? y0 - R + h : y0 + RY + (h - R), g.ny - 1);
         yhb[h] = (unsigned)gy * (unsigned)g.sy * ES;
+        if (ALL && gy >= g.ny - tf.b) qhmask |= 1u << h;
         yhf[h] = 1.0f;
         if (wtap) yhf[h] = __builtin_bit_cast(
             float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, lat(gy, g.ny))));
@@ -718,12 +735,11 @@ __device__ __forceinline__ void wave_tile(const T *__restrict__ P, T *__restrict
     const int zcol = zk < R ? zk : 64 + zk;
     const int zhy = min(y0 + zr, g.ny - 1);
     const unsigned zhoff = ((unsigned)zhy * (unsigned)g.sy + (unsigned)zhzc) * ES;
-    const bool zhtap = wtap && zhzc < tf.b;
+    const bool zhtap = wtap && (ALL ? (zhzc < tf.b || zhzc >= g.nz - tf.b) : zhzc < tf.b);
     float zhfy = 1.0f, zhfz = 1.0f;
-    if (zhtap) {
-        zhfy = lat(zhy, g.ny);
-        zhfz = tab[tf.b - 1 - zhzc];
-    }
+    if (zhtap || (ALL && wtap)) zhfy = lat(zhy, g.ny);
+    if (zhtap) zhfz = ALL ? lat(zhzc, g.nz) : tab[tf.b - 1 - zhzc];
+    const bool zhq = ALL && !zhtap && zhy >= g.ny - tf.b;  // front-right lane (planes x < b)
 
     float q[NR][RY];
     float yh[2][2 * R], zh[2];
@@ -733,35 +749,76 @@ __device__ __forceinline__ void wave_tile(const T *__restrict__ P, T *__restrict
                                                     __builtin_bit_cast(int, lat(x, g.nx))))
                     : 1.0f;
     };
-    // one in-place taper pass of the reference, rounded to the storage type;
-    // factors are exactly 1 where no taper applies
-    auto tap = [&](float v, float fx, float fy) -> float {
-        return wrnd<T>(taper1(v, ztap ? fx : 1.0f, ztap ? fy : 1.0f, fzl));
+    // one in-place taper pass of the reference at a value, rounded to the
+    // storage type; factors are exactly 1 where no taper applies.  zt: the
+    // value is on the tapered z border; q: a front-right value (ALL)
+    auto tapv = [&](float v, float fx, float fy, bool zt, float fz, bool q) -> float {
+        float a = 1.0f, c = 1.0f, d = 1.0f;
+        if (zt) {
+            a = fy;
+            c = fx;
+            d = fz;
+        } else if (ALL) {
+            a = q ? fy : fx;
+            c = q ? fx : fy;
+            d = q ? fy : 1.0f;
+        }
+        v *= a;
+        if (ALL && q) v = wrnd<T>(v);  // the face pass stored it
+        v *= c;
+        v *= d;
+        return wrnd<T>(v);
     };
+    auto tap = [&](float v, float fx, float fy) -> float {
+        return tapv(v, fx, fy, ztap, fzl, false);
+    };
+    // byte offset from plane x to plane b (front-right values, x < b)
+    auto qdelta = [&](int x) -> unsigned { return (unsigned)(tf.b - x) * (unsigned)g.sx * ES; };
     auto load_ring = [&](int x, float(&dst)[RY]) {
         const auto rs = plane_rsrc(P + (size_t)x * (size_t)g.sx);
+        if (ALL && x < tf.b && qmask) {
+            const unsigned qoff = zmid ? zoff + qdelta(x) : zoff;
 #pragma unroll
-        for (int r = 0; r < RY; ++r) dst[r] = ld_buf<T>(rs, rowb[r], zoff);
+            for (int r = 0; r < RY; ++r)
+                dst[r] = ld_buf<T>(rs, rowb[r], ((qmask >> r) & 1u) ? qoff : zoff);
+        } else {
+#pragma unroll
+            for (int r = 0; r < RY; ++r) dst[r] = ld_buf<T>(rs, rowb[r], zoff);
+        }
     };
     auto taper_ring = [&](int x, float(&dst)[RY]) {
         if (TAPER && wtap) {
             const float fx = plane_fx(x);
+            const bool qx = ALL && x < tf.b;
 #pragma unroll
-            for (int r = 0; r < RY; ++r) dst[r] = tap(dst[r], fx, fyr[r]);
+            for (int r = 0; r < RY; ++r)
+                dst[r] = tapv(dst[r], fx, fyr[r], ztap, fzl, qx && ((qmask >> r) & 1u) && zmid);
         }
     };
     auto fetch_halo = [&](int x, int set) {
         const auto rs = plane_rsrc(P + (size_t)x * (size_t)g.sx);
+        if (ALL && x < tf.b) {
+            const unsigned dq = qdelta(x);
+            const unsigned qoff = zmid ? zoff + dq : zoff;
 #pragma unroll
-        for (int h = 0; h < 2 * R; ++h) yh[set][h] = ld_buf<T>(rs, yhb[h], zoff);
-        zh[set] = ld_buf<T>(rs, 0u, zhoff);
+            for (int h = 0; h < 2 * R; ++h)
+                yh[set][h] = ld_buf<T>(rs, yhb[h], ((qhmask >> h) & 1u) ? qoff : zoff);
+            zh[set] = ld_buf<T>(rs, 0u, zhq ? zhoff + dq : zhoff);
+        } else {
+#pragma unroll
+            for (int h = 0; h < 2 * R; ++h) yh[set][h] = ld_buf<T>(rs, yhb[h], zoff);
+            zh[set] = ld_buf<T>(rs, 0u, zhoff);
+        }
     };
     auto taper_halo = [&](int x, int set) {
         if (TAPER && wtap) {
             const float fx = plane_fx(x);
+            const bool qx = ALL && x < tf.b;
 #pragma unroll
-            for (int h = 0; h < 2 * R; ++h) yh[set][h] = tap(yh[set][h], fx, yhf[h]);
-            zh[set] = wrnd<T>(taper1(zh[set], zhtap ? fx : 1.0f, zhfy, zhfz));
+            for (int h = 0; h < 2 * R; ++h)
+                yh[set][h] = tapv(yh[set][h], fx, yhf[h], ztap, fzl,
+                                  qx && ((qhmask >> h) & 1u) && zmid);
+            zh[set] = tapv(zh[set], fx, zhfy, zhtap, zhfz, qx && zhq);
         }
     };
 
@@ -785,9 +842,13 @@ __device__ __forceinline__ void wave_tile(const T *__restrict__ P, T *__restrict
         const auto rps = plane_rsrc(IMG ? (const void *)(PS + pl) : (const void *)PP);
         const auto rimg = plane_rsrc(IMG ? (const void *)(fz.img + pl) : (const void *)PP);
         float c_pp[RY], c_v[RY], c_ps[RY], c_img[RY];
+        // front-right cells read PP at plane b, which this wave updates later
+        // in its march (the launcher makes the first x chunk reach b)
+        const bool qxc = ALL && x < tf.b;
+        const unsigned ppq = qxc && zmid ? zoff + qdelta(x) : zoff;
 #pragma unroll
         for (int r = 0; r < RY; ++r) {
-            c_pp[r] = ld_buf<T>(rpp, rowb[r], zoff);
+            c_pp[r] = ld_buf<T>(rpp, rowb[r], ((qmask >> r) & 1u) ? ppq : zoff);
             c_v[r] = ld_buf<float>(rv, rowb[r] * VS, zoffv);
         }
         if (IMG) {
@@ -840,10 +901,15 @@ __device__ __forceinline__ void wave_tile(const T *__restrict__ P, T *__restrict
             const float p = q[ctr][r];
             float pp = c_pp[r];
             if (TAPER && wtap) {
-                pp = tap(pp, fxc, fyr[r]);
-                if (tf.pp_taps > 1) pp = tap(pp, fxc, fyr[r]);
+                const bool qk = qxc && ((qmask >> r) & 1u) && zmid;
+                // a front-right PP after two passes: plane b's face value
+                // of the first pass, then the second pass's chain
+                if (ALL && qk && tf.pp_taps > 1) pp = wrnd<T>(pp * fyr[r]);
+                pp = tapv(pp, fxc, fyr[r], ztap, fzl, qk);
+                if (tf.pp_taps > 1 && !qk) pp = tap(pp, fxc, fyr[r]);
             }
             const float npp = 2.f * p - pp + c_v[r] * lapl[r];
+            if (ALL && zframe && ((ymask >> r) & 1u)) st_buf<T>(rpp, rowb[r], zoff, pp);
             if (((ymask >> r) & 1u) && zcomp) {
                 st_buf<T>(rpp, rowb[r], zoff, npp);
                 bad |= (npp != npp);
@@ -880,7 +946,7 @@ __device__ __forceinline__ void st_at(char *base, unsigned off, U v)
     *reinterpret_cast<U *>(base + off) = v;
 }
 
-template <typename T, int R, int RY, int WPB, bool IMG, bool TAPER>
+template <typename T, int R, int RY, int WPB, bool IMG, int TAPER>
 __global__ void __launch_bounds__(64 * WPB, (wave_min_occupancy<T, RY, IMG, TAPER>()))
     k_stencil_wave(const T *__restrict__ P, T *__restrict__ PP, const float *__restrict__ V,
                    const Coefs cf, const Grid g, const WaveGrid wg, const ImageFuse fz,
@@ -2071,11 +2137,12 @@ hipError_t launch_rows2(const T *P, T *PP, const float *V, const Coefs &cf, cons
     });
 }
 
-// One launch over the z tiles [zt0, zt0 + nzt).
-template <typename T, int R, int RY, int WPB, bool IMG, bool TAPER>
+// One launch over the z tiles [zt0, zt0 + nzt); x chunks of at least
+// minchunk planes.
+template <typename T, int R, int RY, int WPB, bool IMG, int TAPER>
 hipError_t launch_wave_cfg(const T *P, T *PP, const float *V, const Coefs &cf, const Grid &g,
                            const ImageFuse &fz, const TaperFuse &tf, int *flag, hipStream_t s,
-                           int xchunk, int zt0, int nzt)
+                           int xchunk, int zt0, int nzt, int minchunk = 1)
 {
     auto kern = k_stencil_wave<T, R, RY, WPB, IMG, TAPER>;
     WaveGrid wg;
@@ -2092,6 +2159,7 @@ hipError_t launch_wave_cfg(const T *P, T *PP, const float *V, const Coefs &cf, c
         const int n = choose_chunks((long long)wg.nzt * wg.nyt, nxi, R, cap);
         xchunk = (nxi + n - 1) / n;
     }
+    xchunk = std::max(xchunk, minchunk);
     wg.xchunk = xchunk;
     wg.nxc = (nxi + xchunk - 1) / xchunk;
     wg.waves = (long long)wg.nzt * wg.nyt * wg.nxc;
@@ -2099,6 +2167,29 @@ hipError_t launch_wave_cfg(const T *P, T *PP, const float *V, const Coefs &cf, c
     hipLaunchKernelGGL(kern, dim3((unsigned)nb), dim3(64, WPB), lds, s, P, PP, V, cf, g, wg, fz,
                        tf, tf.t, flag);
     return hipGetLastError();
+}
+
+// The step with taper3D of both fields fused (TAPER 2, no imaging), on one-z
+// waves over every z tile.  Front-right cells read plane b of P and PP: the
+// byte offset from plane x < b to b must fit the 32-bit lane offset, and the
+// first x chunk must reach plane b so that the wave reading PP there (x < b)
+// is the one that updates it later.
+template <typename T, int R, int RY, int WPB>
+hipError_t launch_wave_all(const T *P, T *PP, const float *V, const Coefs &cf, const Grid &g,
+                           const ImageFuse *fuse, const TaperFuse *taper, int *flag,
+                           hipStream_t s, int xchunk)
+{
+    if (fuse || !taper || taper->b < R || taper->b > 8192) return hipErrorInvalidValue;
+    if (2 * taper->b >= g.nx || 2 * taper->b >= g.ny || 2 * taper->b >= g.nz)
+        return hipErrorInvalidValue;
+    if ((long long)taper->b * g.sx * (long long)sizeof(T) + (long long)g.sy * g.ny * sizeof(T) >=
+        (1LL << 31))
+        return hipErrorInvalidValue;
+    ImageFuse fz;
+    TaperFuse tf;
+    fuse_args(nullptr, taper, fz, tf);
+    return launch_wave_cfg<T, R, RY, WPB, false, 2>(P, PP, V, cf, g, fz, tf, flag, s, xchunk, 0,
+                                                    (g.nz + 63) / 64, taper->b - R + 1);
 }
 
 template <typename T, int R, int RY, int WPB>
@@ -2248,6 +2339,12 @@ hipError_t launch_radius(const T *P, T *PP, const float *V, const Coefs &cf, con
 {
     constexpr bool full = std::is_same<T, float>::value;
     const int xchunk = t ? t->xchunk : 0;
+    if (taper && taper->all) {
+        // taper3D fused: 64 x 1 x 8 one-z waves whatever the class tiling
+        if constexpr (R == 4)
+            return launch_wave_all<T, R, 8, 4>(P, PP, V, cf, g, fuse, taper, flag, s, xchunk);
+        return hipErrorNotSupported;
+    }
     if constexpr (R == 4 && std::is_same<T, float>::value) {
         // full rows with two z per lane (8-byte accesses): 2 x 4 x 3
         if (t && t->tz == 2 && t->by == 4 && t->ry == 3 && (g.nz & 1) == 0 &&

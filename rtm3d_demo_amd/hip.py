@@ -375,8 +375,9 @@ def set_tiling(kernel_class, tz=0, by=0, ry=0, xchunk=0):
 
 
 def set_taper_fusion(fmt, mode):
-    """mode 1: upper tapers fused into the stencil; 0: a separate in-place
-    pass; -1: the format's default."""
+    """mode 1: upper tapers fused into the stencil; 2: also the modeling
+    loop's every-border tapers; 0: separate in-place passes; -1: the
+    format's default."""
     check(load().rtmhip_set_taper_fusion(WAVE_FORMATS[fmt][0], mode))
 
 
