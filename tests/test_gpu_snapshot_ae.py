@@ -115,3 +115,55 @@ def test_coded_snapshots_reject_16bit_wavefields_and_bad_interval(gpu):
             d.free()
         cs.free()
         codec.destroy()
+
+
+def test_codec_properties_on_full_salt_region(gpu):
+    """Encode / decode of the imaging region of the 592 x 592 x 336 salt grid
+    (z in [40, 296): 2888 bricks), size-independent properties:
+    * scaling the field by 2^k scales every decoded value by 2^k exactly
+      (bricks are normalised by their largest magnitude, so the codes are the
+      same and the scale carries the power of two);
+    * all-zero bricks decode to zero;
+    * decode writes the region's cells and no other."""
+    from rtm3d_demo_amd import ae
+    shape, b = (592, 592, 336), 40
+    region = ae.region_of(shape, b)
+    nb = ae.region_bricks(region)
+    assert nb == 2888
+    rng = np.random.default_rng(8)
+    f = np.empty(shape, np.float32)
+    for x in range(shape[0]):
+        f[x] = rng.standard_normal(shape[1:], dtype=np.float32)
+    f[:, :, 232:] = 0.0                      # the bricks at z >= 232 hold zeros only
+    codec = ae.Codec(max_bricks=1024, seed=5)
+    dF = gpu.DeviceArray.from_host(f)
+    bricks = gpu.DeviceArray((nb, ae.BRICK), np.uint16)
+    scales = gpu.DeviceArray((nb,))
+    codes = gpu.DeviceArray((nb, ae.CODES), np.uint16)
+    out = gpu.DeviceArray(shape)
+    try:
+        def roundtrip(field):
+            dF.upload(field)
+            ae.cut_bricks(dF, region, bricks, scales)
+            codec.encode(bricks, nb, codes)
+            gpu.check(gpu.load().rtmhip_fill(out.ptr, out.size, 7.0))
+            codec.decode(codes, scales, nb, out, region)
+            return out.download(), codes.download(), scales.download()
+
+        d1, c1, s1 = roundtrip(f)
+        x0, y0, z0, x1, y1, z1 = region
+        inside = d1[x0:x1, y0:y1, z0:z1]
+        assert np.isfinite(inside).all()
+        assert (d1[:, :, :z0] == 7.0).all() and (d1[:, :, z1:] == 7.0).all()
+        assert (inside[:, :, 232 - z0:] == 0.0).all()
+        assert (s1.reshape(19, 19, 8)[:, :, 6:] == 0.0).all()
+        assert (s1.reshape(19, 19, 8)[:, :, :6] > 0.0).all()
+        for k in (-3, 5):
+            dk, ck, sk = roundtrip(f * np.float32(2.0 ** k))
+            assert np.array_equal(ck, c1)
+            assert np.array_equal(sk, s1 * np.float32(2.0 ** k))
+            assert np.array_equal(dk[x0:x1, y0:y1, z0:z1], inside * np.float32(2.0 ** k))
+    finally:
+        for d in (dF, bricks, scales, codes, out):
+            d.free()
+        codec.destroy()
