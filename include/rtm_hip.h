@@ -239,6 +239,40 @@ int rtmhip_snap_backward_steps(const rtmhip_shot *shot, const float *V2DT2, cons
                                int snap_format, int k, const float *rcv_chunk, float *img,
                                int it_hi, int nsteps);
 
+/* Snapshot RTM with transform-coded snapshots: the loops of
+ * rtmhip_snap_forward_steps / rtmhip_snap_backward_steps (same snapshot
+ * steps and imaging) with each snapshot of the imaging range kept as the
+ * largest coefficients of its 3D DCT-II over 8 x 8 x 32 bricks: at most
+ * K = nxe*nye*(nze-2*blen) / keep_div coefficients per snapshot (one
+ * power-of-two threshold for the whole snapshot, so energetic bricks keep
+ * more), each stored as 32 bits (index in the brick, FP16 value over the
+ * brick's largest magnitude), plus a 12-byte table entry per brick.  With
+ * keep_div = 32 the store is about 16x smaller than FP16 snapshots.  The
+ * backward pass decodes each brick in on-chip memory and applies
+ * img += k * (v * PR) directly.  Any wave format.  Results are FP32
+ * arithmetic (fused multiply-adds in the transforms), not bit-identical to
+ * a reference operator: the image error against exact snapshots is
+ * measured.  rtmhip_snap_dct_sizes gives the byte sizes of the store (all
+ * snapshots) and of the work buffer (any output may be NULL). */
+int rtmhip_snap_dct_sizes(const rtmhip_shot *shot, int k, int keep_div, size_t *store_bytes,
+                          size_t *work_bytes);
+int rtmhip_snap_dct_forward_steps(const rtmhip_shot *shot, const float *V2DT2,
+                                  const float *coefs, const float *taper, const float *ricker,
+                                  float **P, float **PP, void *store, void *work, int k,
+                                  int keep_div, int it0, int nsteps);
+int rtmhip_snap_dct_backward_steps(const rtmhip_shot *shot, const float *V2DT2,
+                                   const float *coefs, const float *taper, float **Pr,
+                                   float **PPr, const void *store, int k, int keep_div,
+                                   const float *rcv_chunk, float *img, int it_hi, int nsteps);
+/* The coder alone: encodes the imaging range (z in [blen, nze-blen)) of one
+ * field of the given wave format into store (one snapshot's slot: the size
+ * rtmhip_snap_dct_sizes gives for nt = 1) and decodes it into the same
+ * cells of out (FP32 grid); *kept (host, optional) receives the number of
+ * coefficients kept. */
+int rtmhip_dct_roundtrip(const void *field, int wave_format, int nxe, int nye, int nze,
+                         int blen, int keep_div, void *store, void *work, float *out,
+                         unsigned *kept);
+
 /* ================= batches of shots =================
  * No reference counterpart: the reference migrates one shot at a time per
  * process (rtm-3d/src/controller/RTMAcousticController.cpp:141-200).  A batch

@@ -24,7 +24,7 @@ LIBNAME = os.path.join(LIBDIR, "librtm3d_hip.so")
 DRIVER = os.path.join(BINDIR, "rtm3d")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
-HIP_SOURCES = ["rtm_stencil.hip", "rtm_ops.hip", "rtm_ae.hip", "rtm_capi.hip"]
+HIP_SOURCES = ["rtm_stencil.hip", "rtm_ops.hip", "rtm_ae.hip", "rtm_dct.hip", "rtm_capi.hip"]
 HIP_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++20", "-fPIC", "-ffp-contract=off",
     "-fno-gpu-flush-denormals-to-zero", "-Xclang", "-target-feature", "-Xclang",

@@ -842,6 +842,60 @@ int snap_backward_loop(const rtmhip_shot *shot, const ShotCtx &c, const float *V
                              });
 }
 
+// Transform-coded snapshots: one slot per snapshot step, each the entry
+// pool (K 32-bit entries), the brick table (3 words per brick) and 3 words
+// of metadata, every part 256-byte aligned; the work buffer holds the
+// coefficients of one snapshot ([bricks][2048] FP32) and the exponent
+// histograms of the encoding blocks.
+struct DctSlots {
+    DctLayout L;
+    long long K;              // coefficient budget per snapshot
+    size_t entries, table, meta, slot;  // byte offsets in a slot, slot size
+    size_t coef, hist, work;            // byte offsets in work, work size
+};
+
+inline size_t align256(size_t n) { return (n + 255) / 256 * 256; }
+
+int dct_slots(const Grid &g, int blen, int keep_div, DctSlots *d)
+{
+    if (keep_div < 1) return fail(RTMHIP_ERR_INVALID, "keep_div %d below 1", keep_div);
+    d->L = dct_layout(g);
+    const long long cells = (long long)g.nx * g.ny * (long long)(g.nz - 2 * blen);
+    d->K = std::max(1LL, cells / keep_div);
+    if (d->K >= (1LL << 31) || d->L.nbr * 3 >= (1LL << 31))
+        return fail(RTMHIP_ERR_INVALID, "snapshot too large for 32-bit entry offsets");
+    d->entries = 0;
+    d->table = align256((size_t)d->K * 4);
+    d->meta = d->table + align256((size_t)d->L.nbr * 12);
+    d->slot = d->meta + 256;
+    d->coef = 0;
+    d->hist = align256((size_t)d->L.nbr * 2048 * 4);
+    d->work = d->hist + align256((size_t)d->L.nblk * 256 * 4);
+    return RTMHIP_OK;
+}
+
+template <typename T>
+hipError_t dct_store(const T *f, const Grid &g, int b, const DctSlots &d, void *store, int j,
+                     void *work)
+{
+    char *sl = static_cast<char *>(store) + (size_t)j * d.slot;
+    char *wk = static_cast<char *>(work);
+    return launch_dct_encode(f, g, b, g.nz - b, d.L, d.K, (float *)(wk + d.coef),
+                             (unsigned *)(wk + d.hist), (unsigned *)(sl + d.meta),
+                             (unsigned *)(sl + d.entries), (unsigned *)(sl + d.table),
+                             g_state.stream);
+}
+
+template <typename T>
+hipError_t dct_image(const void *store, int j, const DctSlots &d, const Grid &g, int b,
+                     const T *pr, float *img, float w)
+{
+    const char *sl = static_cast<const char *>(store) + (size_t)j * d.slot;
+    return launch_dct_decode(reinterpret_cast<const unsigned *>(sl + d.entries),
+                             reinterpret_cast<const unsigned *>(sl + d.table), d.L, g, b,
+                             g.nz - b, pr, img, w, (float *)nullptr, g_state.stream);
+}
+
 int check_snap(const rtmhip_shot *shot, const void *snaps, int snap_format, int k)
 {
     if (!snaps) return fail(RTMHIP_ERR_INVALID, "NULL snapshot buffer");
@@ -961,6 +1015,120 @@ int rtmhip_snap_backward_steps(const rtmhip_shot *shot, const float *V2DT2, cons
                                                (const _Float16 *)snaps, k, rcv_chunk, img, it_hi,
                                                nsteps);
     });
+}
+
+int rtmhip_snap_dct_sizes(const rtmhip_shot *shot, int k, int keep_div, size_t *store_bytes,
+                          size_t *work_bytes)
+{
+    if (!shot) return fail(RTMHIP_ERR_INVALID, "NULL pointer");
+    int rc = check_grid(shot->nxe, shot->nye, shot->nze);
+    if (rc) return rc;
+    if (shot->nt <= 0 || shot->nze <= 2 * shot->blen)
+        return fail(RTMHIP_ERR_INVALID, "bad shot geometry");
+    if (k < 1) return fail(RTMHIP_ERR_INVALID, "snapshot interval %d below 1", k);
+    DctSlots d;
+    if ((rc = dct_slots(dense_grid(shot->nxe, shot->nye, shot->nze), shot->blen, keep_div, &d)))
+        return rc;
+    if (store_bytes) *store_bytes = (size_t)((shot->nt - 1) / k + 1) * d.slot;
+    if (work_bytes) *work_bytes = d.work;
+    return RTMHIP_OK;
+}
+
+int rtmhip_snap_dct_forward_steps(const rtmhip_shot *shot, const float *V2DT2,
+                                  const float *coefs, const float *taper, const float *ricker,
+                                  float **P, float **PP, void *store, void *work, int k,
+                                  int keep_div, int it0, int nsteps)
+{
+    NEED_INIT();
+    ShotCtx c;
+    int rc;
+    if ((rc = prepare_shot(shot, coefs, &c))) return rc;
+    if ((rc = check_format(shot))) return rc;
+    if (!V2DT2 || !ricker || !taper || !P || !PP || !*P || !*PP || !store || !work)
+        return fail(RTMHIP_ERR_INVALID, "NULL pointer");
+    if (k < 1 || k > shot->nt)
+        return fail(RTMHIP_ERR_INVALID, "snapshot interval %d outside [1, nt]", k);
+    DctSlots d;
+    if ((rc = dct_slots(c.g, shot->blen, keep_div, &d))) return rc;
+    if (nsteps <= 0) return RTMHIP_OK;
+    if (it0 < 0 || it0 + nsteps > shot->nt)
+        return fail(RTMHIP_ERR_INVALID, "time range outside [0,%d)", shot->nt);
+    return with_format(shot, [&](auto t) {
+        using T = decltype(t);
+        return snap_forward_run(shot, c, V2DT2, taper, ricker, as<T>(P), as<T>(PP), k, it0,
+                                nsteps, [&](const T *f, int j) {
+                                    HIPCHK(dct_store(f, c.g, shot->blen, d, store, j, work),
+                                           "snapshot transform coding");
+                                    return RTMHIP_OK;
+                                });
+    });
+}
+
+int rtmhip_snap_dct_backward_steps(const rtmhip_shot *shot, const float *V2DT2,
+                                   const float *coefs, const float *taper, float **Pr,
+                                   float **PPr, const void *store, int k, int keep_div,
+                                   const float *rcv_chunk, float *img, int it_hi, int nsteps)
+{
+    NEED_INIT();
+    ShotCtx c;
+    int rc;
+    if ((rc = prepare_shot(shot, coefs, &c))) return rc;
+    if ((rc = check_format(shot))) return rc;
+    if (!V2DT2 || !taper || !Pr || !PPr || !*Pr || !*PPr || !rcv_chunk || !img || !store)
+        return fail(RTMHIP_ERR_INVALID, "NULL pointer");
+    if (k < 1 || k > shot->nt)
+        return fail(RTMHIP_ERR_INVALID, "snapshot interval %d outside [1, nt]", k);
+    DctSlots d;
+    if ((rc = dct_slots(c.g, shot->blen, keep_div, &d))) return rc;
+    if (nsteps <= 0) return RTMHIP_OK;
+    if ((rc = check_chunk(shot, it_hi - nsteps + 1, it_hi))) return rc;
+    return with_format(shot, [&](auto t) {
+        using T = decltype(t);
+        return snap_backward_run(shot, c, V2DT2, taper, as<T>(Pr), as<T>(PPr), k, rcv_chunk,
+                                 it_hi, nsteps, [&](const T *pr, int j) {
+                                     HIPCHK(dct_image(store, j, d, c.g, shot->blen, pr, img,
+                                                      (float)k),
+                                            "snapshot decoding");
+                                     return RTMHIP_OK;
+                                 });
+    });
+}
+
+int rtmhip_dct_roundtrip(const void *field, int wave_format, int nxe, int nye, int nze,
+                         int blen, int keep_div, void *store, void *work, float *out,
+                         unsigned *kept)
+{
+    NEED_INIT();
+    if (!field || !store || !work || !out) return fail(RTMHIP_ERR_INVALID, "NULL pointer");
+    int rc = check_grid(nxe, nye, nze);
+    if (rc) return rc;
+    if (blen < 0 || 2 * blen >= nze) return fail(RTMHIP_ERR_INVALID, "bad blen %d", blen);
+    const Grid g = dense_grid(nxe, nye, nze);
+    DctSlots d;
+    if ((rc = dct_slots(g, blen, keep_div, &d))) return rc;
+    const char *sl = static_cast<const char *>(store);
+    hipError_t e;
+    switch (wave_format) {
+    case RTMHIP_WAVE_F32: e = dct_store((const float *)field, g, blen, d, store, 0, work); break;
+    case RTMHIP_WAVE_F16: e = dct_store((const _Float16 *)field, g, blen, d, store, 0, work); break;
+    case RTMHIP_WAVE_BF16: e = dct_store((const __bf16 *)field, g, blen, d, store, 0, work); break;
+    default: return fail(RTMHIP_ERR_UNSUPPORTED, "wave_format %d", wave_format);
+    }
+    HIPCHK(e, "transform coding");
+    HIPCHK(launch_dct_decode(reinterpret_cast<const unsigned *>(sl + d.entries),
+                             reinterpret_cast<const unsigned *>(sl + d.table), d.L, g, blen,
+                             nze - blen, (const float *)nullptr, (float *)nullptr, 0.0f, out,
+                             g_state.stream),
+           "decoding");
+    if (kept) {
+        unsigned meta[3];
+        HIPCHK(hipMemcpyAsync(meta, sl + d.meta, sizeof meta, hipMemcpyDeviceToHost,
+                              g_state.stream),
+               "hipMemcpyAsync");
+        HIPCHK(hipStreamSynchronize(g_state.stream), "hipStreamSynchronize");
+        *kept = meta[1];
+    }
+    return RTMHIP_OK;
 }
 
 int rtmhip_backward_steps(const rtmhip_shot *shot, int hbc, const float *V2DT2,

@@ -131,6 +131,27 @@ hipError_t launch_snap_store(const T *P, S *snap, const Grid &g, int b, hipStrea
 template <typename T, typename S>
 hipError_t launch_snap_image(float *img, const S *snap, const T *PR, const Grid &g, int b,
                              float w, hipStream_t s);
+// Transform-coded snapshots (rtm_dct.hip): 8 x 8 x 32 bricks of the grid,
+// 3D DCT-II, one power-of-two threshold per snapshot keeping at most K
+// coefficients, entries {index in brick (11 bits), FP16 value / brick scale}
+// and a brick table {first entry, count, scale bits} (3 x 32 bits).
+struct DctLayout {
+    int nbx, nby, nbz;  // bricks along x, y, z
+    int nblk;           // blocks per launch (each loops over bricks)
+    long long nbr;      // bricks
+};
+DctLayout dct_layout(const Grid &g);
+// coef: [nbr][2048] FP32 scratch; hist: [nblk][256] scratch; meta: 3 words
+// (threshold bits, kept count, entry counter); the cells z in [z0, z1) enter.
+template <typename T>
+hipError_t launch_dct_encode(const T *F, const Grid &g, int z0, int z1, const DctLayout &L,
+                             long long K, float *coef, unsigned *hist, unsigned *meta,
+                             unsigned *entries, unsigned *table, hipStream_t s);
+// PR != NULL: img += w * (v * PR) on z in [z0, z1); else out = v there.
+template <typename T>
+hipError_t launch_dct_decode(const unsigned *entries, const unsigned *table, const DctLayout &L,
+                             const Grid &g, int z0, int z1, const T *PR, float *img, float w,
+                             float *out, hipStream_t s);
 // taperUpper3D of two grids of the same shape in one pass (Q may be NULL).
 template <typename T>
 hipError_t launch_taper_upper2(T *P, T *Q, const float *taper, const Grid &g, int b,

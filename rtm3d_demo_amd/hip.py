@@ -31,6 +31,8 @@ EXPORTED = [
     "rtmhip_ae_set_params", "rtmhip_ae_gradient", "rtmhip_ae_train", "rtmhip_ae_bricks",
     "rtmhip_ae_encode", "rtmhip_ae_decode", "rtmhip_ae_gemm_test", "rtmhip_freq_image",
     "rtmhip_snap_ae_sizes", "rtmhip_snap_ae_forward_steps", "rtmhip_snap_ae_backward_steps",
+    "rtmhip_snap_dct_sizes", "rtmhip_snap_dct_forward_steps", "rtmhip_snap_dct_backward_steps",
+    "rtmhip_dct_roundtrip",
 ]
 
 
@@ -135,6 +137,13 @@ def load():
                                          vp, i, i, i],
         "rtmhip_snap_ae_backward_steps": [C.POINTER(Shot), fp, fp, fp, pp, pp, vp, vp, fp, vp,
                                           i, fp, fp, i, i],
+        "rtmhip_snap_dct_sizes": [C.POINTER(Shot), i, i, C.POINTER(C.c_size_t),
+                                  C.POINTER(C.c_size_t)],
+        "rtmhip_snap_dct_forward_steps": [C.POINTER(Shot), fp, fp, fp, fp, pp, pp, vp, vp, i, i,
+                                          i, i],
+        "rtmhip_snap_dct_backward_steps": [C.POINTER(Shot), fp, fp, fp, pp, pp, vp, i, i, fp, fp,
+                                           i, i],
+        "rtmhip_dct_roundtrip": [vp, i, i, i, i, i, i, vp, vp, fp, C.POINTER(C.c_uint)],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -368,6 +377,59 @@ def snap_backward_steps(shot, V, coefs, g, rcv_fields, snaps, fmt, k, rcv_chunk,
                                             C.byref(rcv_fields.p), C.byref(rcv_fields.pp),
                                             snaps.ptr, SNAP_FORMATS[fmt][0], k, rcv_chunk.ptr,
                                             img.ptr, it_hi, nsteps))
+
+
+class DctSnapshots:
+    """Transform-coded snapshots of one shot (rtmhip_snap_dct_*): the store of
+    every snapshot and the work buffer; keep_div sets the coefficient budget
+    (nxe*nye*(nze-2*blen) / keep_div per snapshot)."""
+
+    def __init__(self, shot, k, keep_div=32):
+        sb, wb = C.c_size_t(0), C.c_size_t(0)
+        check(load().rtmhip_snap_dct_sizes(C.byref(shot), k, keep_div, C.byref(sb), C.byref(wb)))
+        self.shot, self.k, self.keep_div = shot, k, keep_div
+        self.store = DeviceArray((sb.value,), np.uint8)
+        self.work = DeviceArray((wb.value,), np.uint8)
+
+    def forward_steps(self, V, coefs, g, ricker, fields, it0, nsteps):
+        c = _f32(coefs)
+        check(load().rtmhip_snap_dct_forward_steps(
+            C.byref(self.shot), V.ptr, _host_ptr(c), g.ptr, ricker.ptr, C.byref(fields.p),
+            C.byref(fields.pp), self.store.ptr, self.work.ptr, self.k, self.keep_div, it0, nsteps))
+
+    def backward_steps(self, V, coefs, g, rcv_fields, rcv_chunk, img, it_hi, nsteps):
+        c = _f32(coefs)
+        check(load().rtmhip_snap_dct_backward_steps(
+            C.byref(self.shot), V.ptr, _host_ptr(c), g.ptr, C.byref(rcv_fields.p),
+            C.byref(rcv_fields.pp), self.store.ptr, self.k, self.keep_div, rcv_chunk.ptr, img.ptr,
+            it_hi, nsteps))
+
+    def nbytes(self):
+        return self.store.nbytes
+
+    def free(self):
+        self.store.free()
+        self.work.free()
+
+
+def dct_roundtrip(field, fmt, blen, keep_div=32):
+    """Encodes the imaging range of a device field and decodes it: returns
+    (decoded FP32 host array with the other cells 0, kept coefficients)."""
+    nxe, nye, nze = field.shape
+    shot = make_shot(nxe, nye, nze, blen, 8, 1, 1, (0, 0, 0), RcvGeom(0, 0, 0, 1, 1, 1, 1))
+    sb, wb = C.c_size_t(0), C.c_size_t(0)
+    check(load().rtmhip_snap_dct_sizes(C.byref(shot), 1, keep_div, C.byref(sb), C.byref(wb)))
+    store, work = DeviceArray((sb.value,), np.uint8), DeviceArray((wb.value,), np.uint8)
+    out = DeviceArray(field.shape).zero()
+    kept = C.c_uint(0)
+    try:
+        check(load().rtmhip_dct_roundtrip(field.ptr, WAVE_FORMATS[fmt][0], nxe, nye, nze, blen,
+                                          keep_div, store.ptr, work.ptr, out.ptr,
+                                          C.byref(kept)))
+        return out.download(), kept.value
+    finally:
+        for d in (store, work, out):
+            d.free()
 
 
 def set_tiling(kernel_class, tz=0, by=0, ry=0, xchunk=0):
