@@ -422,6 +422,42 @@ def snapshot_leg(hip, sh, f32_shot_s, k=4):
         sh.release(wfmt)
     for c, tl in enumerate(saved):
         hip.set_tiling(c, *tl) if tl[0] else hip.set_tiling(c, 0)
+    # transform-coded snapshots (keep 1/32 of the coefficients per snapshot)
+    for wfmt in ("f16", "f32"):
+        if wfmt == "f16":
+            for c in range(3):
+                hip.set_tiling(c, 0)
+        else:
+            for c, tl in enumerate(saved):
+                hip.set_tiling(c, *tl) if tl[0] else hip.set_tiling(c, 0)
+        shot, F, R = sh.fields(wfmt)
+        st = hip.DctSnapshots(shot, k, 32)
+        for f in (F.a, F.b, R.a, R.b, sh.img):
+            f.zero()
+        F.p.value, F.pp.value = F.a.ptr.value, F.b.ptr.value
+        R.p.value, R.pp.value = R.a.ptr.value, R.b.ptr.value
+        hip.synchronize()
+        t = time.perf_counter()
+        st.forward_steps(sh.V, sh.coefs, sh.g, sh.src, F, 0, sh.nt)
+        st.backward_steps(sh.V, sh.coefs, sh.g, R, sh.rcv, sh.img, sh.nt - 1, sh.nt)
+        hip.synchronize()
+        s = time.perf_counter() - t
+        img = sh.img.download()
+        if not np.isfinite(img).all() or not np.abs(img).max() > 0:
+            raise RuntimeError("invalid DCT-snapshot RTM image")
+        del img
+        key = "%s_wavefields_dct_snapshots" % wfmt
+        out[key] = {"shot_s": round(s, 3), "shots_per_hour": round(3600.0 / s, 1),
+                    "speedup_vs_f32_hbc": round(f32_shot_s / s, 3),
+                    "snapshot_GB": round(st.nbytes() / 1e9, 2),
+                    "coding": "3D DCT-II on 8x8x32 bricks, at most 1/32 of the coefficients "
+                              "per snapshot (one threshold per snapshot), FP16 values"}
+        log("snapshot RTM %s: %.3f s (%.2fx the FP32 HBC shot), %.2f GB"
+            % (key, s, f32_shot_s / s, st.nbytes() / 1e9))
+        st.free()
+        sh.release(wfmt)
+    for c, tl in enumerate(saved):
+        hip.set_tiling(c, *tl) if tl[0] else hip.set_tiling(c, 0)
     # coded snapshots (FP32 wavefields): the work per snapshot does not depend
     # on the weights, so a seeded network times it; the accuracy leg images
     # with a trained one
@@ -527,6 +563,23 @@ def accuracy_crop(hip, host, workloads, formats=("f32", "f16", "bf16"), crop=(12
         snap_imgs[(wfmt, sfmt, k)] = finish(img)
         for x in (S, R, snaps, img):
             x.free()
+    # transform-coded snapshots every 4th step, keeping 1/32 and 1/64 of the
+    # coefficients
+    for wfmt, div in (("f32", 32), ("f16", 32), ("f32", 64)):
+        wdt = hip.WAVE_FORMATS[wfmt][1]
+        shot = hip.make_shot(*shape, b, order, nt, nt, (cx // 2 + b, cy // 2 + b, 2 + b), geom,
+                             wfmt)
+        S, R = hip.FieldPair(shape, wdt), hip.FieldPair(shape, wdt)
+        for f in (S.a, S.b, R.a, R.b):
+            f.zero()
+        st = hip.DctSnapshots(shot, 4, div)
+        img = hip.DeviceArray(shape).zero()
+        st.forward_steps(Vh, coefs, dg, dsrc, S, 0, nt)
+        st.backward_steps(Vh, coefs, dg, R, rcv32, img, nt - 1, nt)
+        snap_imgs[(wfmt, "dct%d" % div, 4)] = finish(img)
+        for x in (S, R, img):
+            x.free()
+        st.free()
     # coded snapshots (every 4th step) with a codec trained on the snapshots of
     # two other shots of the crop (steps after nt/3, every 50th)
     from rtm3d_demo_amd import ae
@@ -625,9 +678,10 @@ def precision_leg(hip, host, workloads, budget, nt=4000, f32_rate=None):
     out = {"workload": "hbc_rtm_salt800_nt%d" % nt, "grid": [800, 800, 400],
            "extended_grid": list(shape), "dx_m": d, "border": b, "nt": nt,
            "modeling_s": round(model_s, 3),
-           "snapshots": "HBC reconstruction: FP16 snapshots of every 4th step over the "
-                        "imaging range would take 1000 x 880 x 880 x 400 x 2 B = 620 GB, "
-                        "more than the GPU's 288 GB"}
+           "snapshots": "the HBC shots reconstruct the source field (FP16 snapshots of every "
+                        "4th step over the imaging range would take 1000 x 880 x 880 x 400 x "
+                        "2 B = 620 GB, more than the GPU's 288 GB); snapshots every 4th step "
+                        "run transform-coded (f16_wavefields_dct_snapshots_k4)"}
     saved = [hip.get_tiling(c) for c in range(3)]
     img = hip.DeviceArray(shape)
     for fmt in ("f16", "bf16", "f32"):
@@ -669,10 +723,30 @@ def precision_leg(hip, host, workloads, budget, nt=4000, f32_rate=None):
             r["plain_step_achieved_GBs"] = round((4.0 + 3 * bpc) * U / a0 / 1e9, 1)
         out[fmt] = r
         log("precision leg %s: %.3f s" % (fmt, s))
+        if fmt == "f16":
+            # snapshot RTM: FP16 wavefields, transform-coded snapshots of
+            # every 4th step (keep 1/32), instead of HBC reconstruction
+            S, R = hip.FieldPair(shape, wdt), hip.FieldPair(shape, wdt)
+            for f in (S.a, S.b, R.a, R.b):
+                f.zero()
+            st = hip.DctSnapshots(shot, 4, 32)
+            img.zero()
+            hip.synchronize()
+            t = time.perf_counter()
+            st.forward_steps(V, coefs, g, src, S, 0, nt)
+            st.backward_steps(V, coefs, g, R, rcv, img, nt - 1, nt)
+            hip.synchronize()
+            s = time.perf_counter() - t
+            out["f16_wavefields_dct_snapshots_k4"] = {
+                "shot_s": round(s, 3), "shots_per_hour": round(3600.0 / s, 1),
+                "snapshot_GB": round(st.nbytes() / 1e9, 2)}
+            log("precision leg f16 + DCT snapshots: %.3f s, %.1f GB" % (s, st.nbytes() / 1e9))
+            for x in (S, R, st):
+                x.free()
     for c, tl in enumerate(saved):
         hip.set_tiling(c, *tl) if tl[0] else hip.set_tiling(c, 0)
     if "f32" in out:
-        for fmt in ("f16", "bf16"):
+        for fmt in ("f16", "bf16", "f16_wavefields_dct_snapshots_k4"):
             if fmt in out:
                 out[fmt]["speedup_vs_f32"] = round(out["f32"]["shot_s"] / out[fmt]["shot_s"], 3)
     for x in (V, src, g, rcv, img):
@@ -969,7 +1043,7 @@ def main():
                 line["snapshot"]["accuracy"] = dict(snap_acc, case=case)
         except Exception as e:
             line["accuracy_vs_fp64"] = {"error": "%s: %s" % (type(e).__name__, e)}
-    if budget.allows("precision", 75.0):
+    if budget.allows("precision", 90.0):
         line["precision"] = precision_leg(hip, host, workloads, budget)
     if budget.allows("batched", 90.0):
         line["batched"] = batch_leg(hip, host, workloads, args.batch_nt)

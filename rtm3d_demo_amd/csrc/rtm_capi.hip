@@ -845,8 +845,8 @@ int snap_backward_loop(const rtmhip_shot *shot, const ShotCtx &c, const float *V
 // Transform-coded snapshots: one slot per snapshot step, each the entry
 // pool (K 32-bit entries), the brick table (3 words per brick) and 3 words
 // of metadata, every part 256-byte aligned; the work buffer holds the
-// coefficients of one snapshot ([bricks][2048] FP32) and the exponent
-// histograms of the encoding blocks.
+// coefficients of one snapshot ([bricks][2048] FP32) and the 256-bin
+// exponent histogram.
 struct DctSlots {
     DctLayout L;
     long long K;              // coefficient budget per snapshot
@@ -870,7 +870,7 @@ int dct_slots(const Grid &g, int blen, int keep_div, DctSlots *d)
     d->slot = d->meta + 256;
     d->coef = 0;
     d->hist = align256((size_t)d->L.nbr * 2048 * 4);
-    d->work = d->hist + align256((size_t)d->L.nblk * 256 * 4);
+    d->work = d->hist + 1024;
     return RTMHIP_OK;
 }
 

@@ -18,7 +18,7 @@
 // imaging condition img += w * (v * PR) directly.
 //
 // A brick is handled by one block of 256 threads in LDS; blocks loop over
-// bricks, so the per-block exponent histogram stays small.
+// bricks and add their exponent histograms to one 256-bin histogram.
 #include "rtm_internal.hpp"
 
 #include <algorithm>
@@ -31,21 +31,41 @@ constexpr int kLine = 33;                   // padded z line in LDS
 constexpr int kTile = 8 * 8 * kLine;        // one brick in LDS
 constexpr int kCells = 8 * 8 * 32;          // 2048 cells / coefficients per brick
 
-// Orthonormal DCT-II matrices, D[u][i] = s(u) cos(pi (2i + 1) u / 2n).
-__device__ __forceinline__ void dct_tables(float (*d32)[32], float (*d8)[8])
+// Orthonormal DCT-II matrices, D[u][i] = s(u) cos(pi (2i + 1) u / 2n),
+// s(0) = sqrt(1/n), s(u > 0) = sqrt(2/n), evaluated at compile time (cosine
+// by range reduction and its Taylor series in double) and held in constant
+// memory: the transforms index them wave-uniformly, so they are scalar loads.
+constexpr double kPi = 3.14159265358979323846;
+constexpr double ccos(double x)
 {
-    const float pi = 3.14159265358979323846f;
-    for (int e = threadIdx.x; e < 32 * 32; e += kDctThreads) {
-        const int u = e >> 5, i = e & 31;
-        const float s = u ? 0.25f : 0.17677669529663687f;  // sqrt(2/32), sqrt(1/32)
-        d32[u][i] = s * cosf(pi * (float)((2 * i + 1) * u) / 64.0f);
+    const double two_pi = 2.0 * kPi;
+    x -= (double)(long long)(x / two_pi) * two_pi;
+    if (x > kPi) x -= two_pi;
+    double term = 1.0, sum = 1.0;
+    for (int k = 1; k < 40; ++k) {
+        term *= -x * x / (double)((2 * k - 1) * (2 * k));
+        sum += term;
     }
-    for (int e = threadIdx.x; e < 64; e += kDctThreads) {
-        const int u = e >> 3, i = e & 7;
-        const float s = u ? 0.5f : 0.35355339059327373f;    // sqrt(2/8), sqrt(1/8)
-        d8[u][i] = s * cosf(pi * (float)((2 * i + 1) * u) / 16.0f);
-    }
+    return sum;
 }
+struct DctTab {
+    float d32[32][32];
+    float d8[8][8];
+};
+constexpr DctTab make_dct_tab()
+{
+    DctTab t{};
+    for (int u = 0; u < 32; ++u)
+        for (int i = 0; i < 32; ++i)
+            t.d32[u][i] = (float)((u ? 0.25 : 0.17677669529663688) *
+                                  ccos(kPi * (double)((2 * i + 1) * u) / 64.0));
+    for (int u = 0; u < 8; ++u)
+        for (int i = 0; i < 8; ++i)
+            t.d8[u][i] = (float)((u ? 0.5 : 0.35355339059327379) *
+                                 ccos(kPi * (double)((2 * i + 1) * u) / 16.0));
+    return t;
+}
+__constant__ DctTab kDct = make_dct_tab();
 
 __device__ __forceinline__ void brick_coords(long long br, const DctLayout &L, int &x0, int &y0,
                                              int &z0)
@@ -60,15 +80,16 @@ __device__ __forceinline__ void brick_coords(long long br, const DctLayout &L, i
 }
 
 // Forward transform of the brick in t (layout [x][y][z], z padded to 33);
-// leaves coefficient (u, v, w) at t[(u*8 + v)*33 + w] and returns the
-// thread's eight x-pass outputs in out (x index u = 0..7 at y = tid>>5,
-// z = tid&31).
-__device__ __forceinline__ void dct3_forward(float *t, const float (*d32)[32],
-                                             const float (*d8)[8], float (&out)[8])
+// leaves coefficient (u, v, w) at t[(u*8 + v)*33 + w], except that the x
+// pass's outputs stay in the thread's out (x index u = 0..7 at y = tid>>5,
+// z = tid&31).  z pass: wave q computes outputs 8q .. 8q+7 of the 64 lines
+// (lane = line), so every matrix index is wave-uniform.
+__device__ __forceinline__ void dct3_forward(float *t, float (&out)[8])
 {
     const int tid = threadIdx.x;
-    {   // z: line (i, j) = tid >> 2, outputs 8q .. 8q+7
-        const int l = tid >> 2, q = tid & 3;
+    {   // z
+        const int l = tid & 63;
+        const int q = __builtin_amdgcn_readfirstlane(tid >> 6);
         float in[32];
 #pragma unroll
         for (int k = 0; k < 32; ++k) in[k] = t[l * kLine + k];
@@ -77,7 +98,7 @@ __device__ __forceinline__ void dct3_forward(float *t, const float (*d32)[32],
         for (int r = 0; r < 8; ++r) {
             float a = 0.0f;
 #pragma unroll
-            for (int k = 0; k < 32; ++k) a = __builtin_fmaf(d32[8 * q + r][k], in[k], a);
+            for (int k = 0; k < 32; ++k) a = __builtin_fmaf(kDct.d32[8 * q + r][k], in[k], a);
             o[r] = a;
         }
         __syncthreads();
@@ -94,7 +115,7 @@ __device__ __forceinline__ void dct3_forward(float *t, const float (*d32)[32],
         for (int u = 0; u < 8; ++u) {
             float a = 0.0f;
 #pragma unroll
-            for (int j = 0; j < 8; ++j) a = __builtin_fmaf(d8[u][j], in[j], a);
+            for (int j = 0; j < 8; ++j) a = __builtin_fmaf(kDct.d8[u][j], in[j], a);
             o[u] = a;
         }
         __syncthreads();
@@ -111,15 +132,14 @@ __device__ __forceinline__ void dct3_forward(float *t, const float (*d32)[32],
         for (int u = 0; u < 8; ++u) {
             float a = 0.0f;
 #pragma unroll
-            for (int i = 0; i < 8; ++i) a = __builtin_fmaf(d8[u][i], in[i], a);
+            for (int i = 0; i < 8; ++i) a = __builtin_fmaf(kDct.d8[u][i], in[i], a);
             out[u] = a;
         }
     }
 }
 
 // Inverse transform in place (coefficients in t as dct3_forward leaves them).
-__device__ __forceinline__ void dct3_inverse(float *t, const float (*d32)[32],
-                                             const float (*d8)[8])
+__device__ __forceinline__ void dct3_inverse(float *t)
 {
     const int tid = threadIdx.x;
     {   // x
@@ -131,7 +151,7 @@ __device__ __forceinline__ void dct3_inverse(float *t, const float (*d32)[32],
         for (int i = 0; i < 8; ++i) {
             float a = 0.0f;
 #pragma unroll
-            for (int u = 0; u < 8; ++u) a = __builtin_fmaf(d8[u][i], in[u], a);
+            for (int u = 0; u < 8; ++u) a = __builtin_fmaf(kDct.d8[u][i], in[u], a);
             o[i] = a;
         }
         __syncthreads();
@@ -148,7 +168,7 @@ __device__ __forceinline__ void dct3_inverse(float *t, const float (*d32)[32],
         for (int j = 0; j < 8; ++j) {
             float a = 0.0f;
 #pragma unroll
-            for (int u = 0; u < 8; ++u) a = __builtin_fmaf(d8[u][j], in[u], a);
+            for (int u = 0; u < 8; ++u) a = __builtin_fmaf(kDct.d8[u][j], in[u], a);
             o[j] = a;
         }
         __syncthreads();
@@ -157,7 +177,8 @@ __device__ __forceinline__ void dct3_inverse(float *t, const float (*d32)[32],
         __syncthreads();
     }
     {   // z
-        const int l = tid >> 2, q = tid & 3;
+        const int l = tid & 63;
+        const int q = __builtin_amdgcn_readfirstlane(tid >> 6);
         float in[32];
 #pragma unroll
         for (int w = 0; w < 32; ++w) in[w] = t[l * kLine + w];
@@ -166,7 +187,7 @@ __device__ __forceinline__ void dct3_inverse(float *t, const float (*d32)[32],
         for (int r = 0; r < 8; ++r) {
             float a = 0.0f;
 #pragma unroll
-            for (int w = 0; w < 32; ++w) a = __builtin_fmaf(d32[w][8 * q + r], in[w], a);
+            for (int w = 0; w < 32; ++w) a = __builtin_fmaf(kDct.d32[w][8 * q + r], in[w], a);
             o[r] = a;
         }
         __syncthreads();
@@ -181,18 +202,15 @@ __device__ __forceinline__ void dct3_inverse(float *t, const float (*d32)[32],
 __device__ __forceinline__ int tile_index(int e) { return (e >> 5) * kLine + (e & 31); }
 
 // Transform of every brick: coefficients to coef[brick][2048] (index
-// (u*8 + v)*32 + w), exponent histogram of the non-zero magnitudes of the
-// block's bricks to hist[block][256].
+// (u*8 + v)*32 + w); the exponent histogram of the non-zero magnitudes is
+// accumulated in LDS per block and added to hist[256] (zeroed beforehand).
 template <typename T>
 __global__ void __launch_bounds__(kDctThreads)
     k_dct_encode(const T *__restrict__ F, Grid g, int z0, int z1, DctLayout L,
                  float *__restrict__ coef, unsigned *__restrict__ hist)
 {
     __shared__ float t[kTile];
-    __shared__ float d32[32][32];
-    __shared__ float d8[8][8];
     __shared__ unsigned h[256];
-    dct_tables(d32, d8);
     h[threadIdx.x] = 0;
     __syncthreads();
     const int tid = threadIdx.x;
@@ -210,7 +228,7 @@ __global__ void __launch_bounds__(kDctThreads)
         }
         __syncthreads();
         float out[8];
-        dct3_forward(t, d32, d8, out);
+        dct3_forward(t, out);
         float *cb = coef + br * kCells;
         const int j = tid >> 5, k = tid & 31;
 #pragma unroll
@@ -221,20 +239,19 @@ __global__ void __launch_bounds__(kDctThreads)
         }
         __syncthreads();  // the next brick reuses t
     }
-    hist[(long long)blockIdx.x * 256 + tid] = h[tid];
+    if (h[tid]) atomicAdd(&hist[tid], h[tid]);
 }
+
+__global__ void k_dct_clear(unsigned *__restrict__ hist) { hist[threadIdx.x] = 0; }
 
 // Threshold of one snapshot: meta[0] = the lowest power-of-two magnitude
 // (as float bits) whose kept count (|c| >= it) is at most K, meta[1] = that
 // count, meta[2] = 0 (the entry counter of the packing pass).
 __global__ void __launch_bounds__(kDctThreads)
-    k_dct_select(const unsigned *__restrict__ hist, int nblk, long long K,
-                 unsigned *__restrict__ meta)
+    k_dct_select(const unsigned *__restrict__ hist, long long K, unsigned *__restrict__ meta)
 {
     __shared__ unsigned long long tot[256];
-    unsigned long long s = 0;
-    for (int b = 0; b < nblk; ++b) s += hist[(long long)b * 256 + threadIdx.x];
-    tot[threadIdx.x] = s;
+    tot[threadIdx.x] = hist[threadIdx.x];
     __syncthreads();
     if (threadIdx.x == 0) {
         unsigned long long cum = 0;
@@ -309,11 +326,13 @@ __global__ void __launch_bounds__(kDctThreads)
         }
         __syncthreads();
         unsigned pos = sbase + before + incl - n;
-        const float inv = scale > 0.0f ? 1.0f / scale : 0.0f;
 #pragma unroll
         for (int m = 0; m < 8; ++m) {
             if ((keep >> m) & 1u) {
-                const _Float16 hv = (_Float16)(c[m] * inv);
+                // a division, not a product with 1/scale: the reciprocal of a
+                // subnormal scale overflows (FP32 fields reach subnormal
+                // magnitudes far from the source), the quotient stays <= 1
+                const _Float16 hv = (_Float16)(c[m] / scale);
                 entries[pos++] = (unsigned)(8 * tid + m) |
                                  ((unsigned)__builtin_bit_cast(unsigned short, hv) << 16);
             }
@@ -331,9 +350,6 @@ __global__ void __launch_bounds__(kDctThreads)
                  float *__restrict__ img, float w, float *__restrict__ out)
 {
     __shared__ float t[kTile];
-    __shared__ float d32[32][32];
-    __shared__ float d8[8][8];
-    dct_tables(d32, d8);
     const int tid = threadIdx.x;
     for (long long br = blockIdx.x; br < L.nbr; br += gridDim.x) {
         for (int e = tid; e < kTile; e += kDctThreads) t[e] = 0.0f;
@@ -345,22 +361,48 @@ __global__ void __launch_bounds__(kDctThreads)
             const float v = (float)__builtin_bit_cast(_Float16, (unsigned short)(u >> 16));
             t[tile_index((int)(u & 0x7ffu))] = v * scale;
         }
-        __syncthreads();
-        dct3_inverse(t, d32, d8);
+        // the imaging operands of this thread's cells, loaded before the
+        // inverse transform so that their latency overlaps it
         int bx0, by0, bz0;
         brick_coords(br, L, bx0, by0, bz0);
+        long long o[8];
+        bool in[8];
+        // PR as raw bits in 32-bit registers (16-bit values: zero-extending
+        // loads, not half-register ones), widened after the transform
+        unsigned pr[8];
+        float im[8];
 #pragma unroll
         for (int m = 0; m < 8; ++m) {
             const int e = tid + kDctThreads * m;
             const int x = bx0 + (e >> 8), y = by0 + ((e >> 5) & 7), z = bz0 + (e & 31);
-            if (x < g.nx && y < g.ny && z >= z0 && z < z1) {
-                const long long o = (long long)x * g.sx + (long long)y * g.sy + z;
-                const float v = t[tile_index(e)];
+            in[m] = x < g.nx && y < g.ny && z >= z0 && z < z1;
+            o[m] = (long long)x * g.sx + (long long)y * g.sy + z;
+            pr[m] = 0u;
+            im[m] = 0.0f;
+            if (IMG && in[m]) {
+                if constexpr (sizeof(T) == 2)
+                    pr[m] = reinterpret_cast<const unsigned short *>(PR)[o[m]];
+                else
+                    pr[m] = reinterpret_cast<const unsigned *>(PR)[o[m]];
+                im[m] = img[o[m]];
+            }
+        }
+        __syncthreads();
+        dct3_inverse(t);
+#pragma unroll
+        for (int m = 0; m < 8; ++m) {
+            if (in[m]) {
+                const float v = t[tile_index(tid + kDctThreads * m)];
                 if (IMG) {
-                    const float p = v * wld(PR[o]);
-                    img[o] = img[o] + w * p;
+                    float prv;
+                    if constexpr (sizeof(T) == 2)
+                        prv = wld(__builtin_bit_cast(T, (unsigned short)pr[m]));
+                    else
+                        prv = __builtin_bit_cast(float, pr[m]);
+                    const float p = v * prv;
+                    img[o[m]] = im[m] + w * p;
                 } else {
-                    out[o] = v;
+                    out[o[m]] = v;
                 }
             }
         }
@@ -368,9 +410,8 @@ __global__ void __launch_bounds__(kDctThreads)
     }
 }
 
-// blocks per launch: each loops over bricks, so the exponent histograms the
-// threshold pass reads stay at 1 KB per block
-inline unsigned dct_blocks(long long nbr) { return (unsigned)std::min<long long>(nbr, 1024); }
+// blocks per launch: each loops over bricks (up to 16 per CU on 256 CUs)
+inline unsigned dct_blocks(long long nbr) { return (unsigned)std::min<long long>(nbr, 4096); }
 
 }  // namespace
 
@@ -391,11 +432,13 @@ hipError_t launch_dct_encode(const T *F, const Grid &g, int z0, int z1, const Dc
                              unsigned *entries, unsigned *table, hipStream_t s)
 {
     if (L.nbr <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_dct_encode<T>, dim3(L.nblk), dim3(kDctThreads), 0, s, F, g, z0, z1, L,
-                       coef, hist);
+    hipLaunchKernelGGL(k_dct_clear, dim3(1), dim3(256), 0, s, hist);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_dct_select, dim3(1), dim3(kDctThreads), 0, s, hist, L.nblk, K, meta);
+    hipLaunchKernelGGL(k_dct_encode<T>, dim3(L.nblk), dim3(kDctThreads), 0, s, F, g, z0, z1, L,
+                       coef, hist);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    hipLaunchKernelGGL(k_dct_select, dim3(1), dim3(kDctThreads), 0, s, hist, K, meta);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     hipLaunchKernelGGL(k_dct_pack, dim3(L.nblk), dim3(kDctThreads), 0, s, coef, L, meta, entries,
                        table);

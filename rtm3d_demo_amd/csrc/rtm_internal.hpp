@@ -141,7 +141,7 @@ struct DctLayout {
     long long nbr;      // bricks
 };
 DctLayout dct_layout(const Grid &g);
-// coef: [nbr][2048] FP32 scratch; hist: [nblk][256] scratch; meta: 3 words
+// coef: [nbr][2048] FP32 scratch; hist: 256 words of scratch; meta: 3 words
 // (threshold bits, kept count, entry counter); the cells z in [z0, z1) enter.
 template <typename T>
 hipError_t launch_dct_encode(const T *F, const Grid &g, int z0, int z1, const DctLayout &L,

@@ -64,8 +64,8 @@ def coder(f, b, keep_div):
         cum += hist[e]
     keep = (bits >= thr) & (bits != 0)
     scale = np.abs(c32).reshape(len(c32), -1).max(axis=1)
-    inv = np.where(scale > 0, np.float32(1.0) / np.maximum(scale, np.float32(1e-38)), 0)
-    q = (c32 * inv[:, None, None, None].astype(np.float32)).astype(np.float16).astype(np.float64)
+    safe = np.where(scale > 0, scale, np.float32(1.0))[:, None, None, None]
+    q = (c32 / safe).astype(np.float16).astype(np.float64)
     q = np.where(keep, q * scale[:, None, None, None].astype(np.float64), 0.0)
     rec = np.einsum("ui,vj,wk,nuvw->nijk", d8, d8, d32, q)
     return _unbricks(rec, f.shape, padded, b), int(keep.sum())
@@ -101,6 +101,7 @@ def test_coder_matches_restatement(gpu, shape, b, keep_div):
         d.free()
     want, kept_np = coder(f, b, keep_div)
     inside = np.s_[:, :, b:shape[2] - b]
+    assert np.isfinite(got).all()
     assert abs(kept - kept_np) <= max(2, kept_np // 10000), (kept, kept_np)
     assert _rel(got[inside], want[inside]) < 2e-4
     assert (got[:, :, :b] == 0).all() and (got[:, :, shape[2] - b:] == 0).all()
@@ -108,6 +109,27 @@ def test_coder_matches_restatement(gpu, shape, b, keep_div):
     print("keep 1/%d: kept %d, field rel-L2 %.2e" % (keep_div, kept, err))
     if keep_div == 1:
         assert err < 2e-3
+
+
+def test_coder_subnormal_bricks(gpu):
+    """Bricks of subnormal magnitude beside normal ones (FP32 wavefields reach
+    it far from the source) with a budget that keeps them: their scales are
+    subnormal, whose reciprocals overflow; the codes stay finite.  (All of a
+    snapshot's subnormal coefficients share the lowest exponent class, so a
+    budget that cannot hold them all drops them together.)"""
+    shape, b = (48, 48, 96), 32
+    f = _wavefield(shape, 9)
+    f[:16, :16, :] *= np.float32(1e-41)
+    d = gpu.DeviceArray.from_host(f)
+    try:
+        got, kept = gpu.dct_roundtrip(d, "f32", b, 1)
+    finally:
+        d.free()
+    assert np.isfinite(got).all()
+    inside = np.s_[:, :, b:shape[2] - b]
+    assert _rel(got[inside], f[inside]) < 2e-3
+    tiny = np.abs(got[:16, :16, b:shape[2] - b])
+    assert tiny.max() < 1e-38 and tiny.max() > 0
 
 
 @pytest.mark.parametrize("fmt", ["f32", "f16", "bf16"])
