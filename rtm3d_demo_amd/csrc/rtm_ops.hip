@@ -8,6 +8,8 @@
 // operators would with grids of that type.
 #include "rtm_internal.hpp"
 
+#include <algorithm>
+
 #pragma clang fp contract(off)
 
 namespace rtmhip {
@@ -282,17 +284,86 @@ __device__ __forceinline__ float frame_taper(float v, const float *__restrict__ 
     return v;
 }
 
+// frame_cell and image_frame_cell with 32-bit indices and divisions by launch constants
+// through a multiply-high (the single-grid kernels: a 64-bit division per
+// cell made the receiver-and-frame imaging kernel ALU-bound).  dv: divisors
+// {R, z extent of the frame, ny, 2R}.
+struct FrameDivs {
+    FastDiv r, lz, ny, r2;
+};
+__device__ __forceinline__ bool frame_cell_fd(const Grid &g, int b, int R, int i,
+                                              const FrameDivs &dv, int &x, int &y, int &z)
+{
+    const int nA = g.nx * g.ny * R;
+    const int lz = b - R;
+    const int nB = 2 * R * g.ny * lz;
+    const int nC = (g.nx - 2 * R) * 2 * R * lz;
+    if (i < nA) {
+        const int r = fdiv(i, dv.r);
+        z = i - r * R;
+        x = fdiv(r, dv.ny);
+        y = r - x * g.ny;
+    } else if ((i -= nA) < nB) {
+        const int r = fdiv(i, dv.lz);
+        z = R + (i - r * lz);
+        const int xx = fdiv(r, dv.ny);
+        y = r - xx * g.ny;
+        x = xx < R ? xx : g.nx - 2 * R + xx;
+    } else if ((i -= nB) < nC) {
+        const int r = fdiv(i, dv.lz);
+        z = R + (i - r * lz);
+        const int xr = fdiv(r, dv.r2);
+        const int yy = r - xr * 2 * R;
+        x = R + xr;
+        y = yy < R ? yy : g.ny - 2 * R + yy;
+    } else {
+        return false;
+    }
+    return true;
+}
+__device__ __forceinline__ bool image_frame_cell_fd(const Grid &g, int R, int z0, int z1, int i,
+                                                    const FrameDivs &dv, int &x, int &y, int &z)
+{
+    const int lz = z1 - z0;
+    if (lz <= 0) return false;
+    const int nA = 2 * R * g.ny * lz;
+    const int nB = (g.nx - 2 * R) * 2 * R * lz;
+    if (i < nA) {
+        const int r = fdiv(i, dv.lz);
+        z = z0 + (i - r * lz);
+        const int xx = fdiv(r, dv.ny);
+        y = r - xx * g.ny;
+        x = xx < R ? xx : g.nx - 2 * R + xx;
+    } else if ((i -= nA) < nB) {
+        const int r = fdiv(i, dv.lz);
+        z = z0 + (i - r * lz);
+        const int xr = fdiv(r, dv.r2);
+        const int yy = r - xr * 2 * R;
+        x = R + xr;
+        y = yy < R ? yy : g.ny - 2 * R + yy;
+    } else {
+        return false;
+    }
+    return true;
+}
+inline FrameDivs frame_divs(int R, int lz, int ny)
+{
+    return FrameDivs{fast_div((uint32_t)R), fast_div((uint32_t)std::max(lz, 1)),
+                     fast_div((uint32_t)ny), fast_div((uint32_t)(2 * R))};
+}
+
 // In-place upper taper, applied `taps` times, of the frame cells.  The fused
 // HBC step tapers every other slab value as it is loaded; these cells of PP
 // are not read during the step, so tapering them in place here keeps PP's
 // memory equal to the reference's after its in-place tapers.
 template <typename T>
 __global__ void __launch_bounds__(kThreads)
-    k_taper_frame(T *__restrict__ P, const float *__restrict__ t, Grid g, int b, int R, int taps)
+    k_taper_frame(T *__restrict__ P, const float *__restrict__ t, Grid g, int b, int R, int taps,
+                  FrameDivs dv)
 {
     const int i = (int)(blockIdx.x * kThreads + threadIdx.x);  // count < 2^31 (launcher)
     int x, y, z;
-    if (!frame_cell(g, b, R, i, x, y, z)) return;
+    if (!frame_cell_fd(g, b, R, i, dv, x, y, z)) return;
     const long long o = goff(g, x, y, z);
     P[o] = wst<T>(frame_taper<T>(wld(P[o]), t, g, b, x, y, z, taps));
 }
@@ -651,19 +722,20 @@ template <typename T>
 __global__ void __launch_bounds__(kThreads)
     k_receivers_image(T *__restrict__ PP, const float *__restrict__ rcv, Grid g, int rx0, int ry0,
                       int rz, int rdx, int rdy, int rnx, int rny, int ntstep, int lt,
-                      float *__restrict__ img, const T *__restrict__ ps, int z0, int z1, int R)
+                      float *__restrict__ img, const T *__restrict__ ps, int z0, int z1, int R,
+                      FrameDivs dv, FastDiv drny)
 {
-    long long i = (long long)blockIdx.x * kThreads + threadIdx.x;
-    const long long nrcv = (long long)rnx * rny;
+    const int i = (int)(blockIdx.x * kThreads + threadIdx.x);  // < 2^31 (launcher)
+    const int nrcv = rnx * rny;
     if (i < nrcv) {
-        const int ix = (int)(i / rny), iy = (int)(i % rny);
-        inject_image(PP, img, ps, g, rx0 + ix * rdx, ry0 + iy * rdy, rz, rcv[i * ntstep + lt], z0,
-                     z1);
+        const int ix = fdiv(i, drny), iy = i - ix * rny;
+        inject_image(PP, img, ps, g, rx0 + ix * rdx, ry0 + iy * rdy, rz,
+                     rcv[(long long)i * ntstep + lt], z0, z1);
         return;
     }
     int x, y, z;
-    if (!image_frame_cell(g, R, z0, z1, i - nrcv, x, y, z)) return;
-    if (is_receiver(x, y, z, rx0, ry0, rz, rdx, rdy, rnx, rny)) return;
+    if (!image_frame_cell_fd(g, R, z0, z1, i - nrcv, dv, x, y, z)) return;
+    if (z == rz && is_receiver(x, y, z, rx0, ry0, rz, rdx, rdy, rnx, rny)) return;
     const long long o = goff(g, x, y, z);
     img[o] = img[o] + wld(ps[o]) * wld(PP[o]);
 }
@@ -1015,7 +1087,7 @@ hipError_t launch_taper_frame(T *P, const float *taper, const Grid &g, int b, in
                         (long long)(g.nx - 2 * R) * 2 * R * lz;
     if (n + kThreads >= (1LL << 31)) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_taper_frame<T>, dim3(nblocks(n)), dim3(kThreads), 0, s, P, taper, g, b, R,
-                       taps);
+                       taps, frame_divs(R, b - R, g.ny));
     return hipGetLastError();
 }
 
@@ -1072,9 +1144,11 @@ hipError_t launch_receivers_image(T *PP, const float *rcv, const Grid &g, int rx
     const long long n = (long long)rnx * rny + (long long)2 * R * g.ny * lz +
                         (long long)(g.nx - 2 * R) * 2 * R * lz;
     if (n <= 0) return hipSuccess;
+    if (n + kThreads >= (1LL << 31)) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_receivers_image<T>, dim3(nblocks(n)), dim3(kThreads), 0, s, PP, rcv, g, rx0,
                        ry0, rz, rdx < 1 ? 1 : rdx, rdy < 1 ? 1 : rdy, rnx, rny, ntstep, lt, img,
-                       ps, z0, z1, R);
+                       ps, z0, z1, R, frame_divs(R, (int)lz, g.ny),
+                       fast_div((uint32_t)std::max(rny, 1)));
     return hipGetLastError();
 }
 
