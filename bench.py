@@ -581,8 +581,11 @@ def accuracy_crop(hip, host, workloads, formats=("f32", "f16", "bf16"), crop=(12
             x.free()
         st.free()
     # coded snapshots (every 4th step) with a codec trained on the snapshots of
-    # two other shots of the crop (steps after nt/3, every 50th)
+    # two other shots of the crop (steps after nt/3, every 50th); the same
+    # snapshots train the learned basis of the transform coder
     from rtm3d_demo_amd import ae
+    from rtm3d_demo_amd.basis import BasisLearner
+    learner = BasisLearner()
     t = time.perf_counter()
     lz, kt = shape[2] - 2 * b, 50
     nb = ae.region_bricks((0, 0, 0, shape[0], shape[1], lz))
@@ -597,6 +600,8 @@ def accuracy_crop(hip, host, workloads, formats=("f32", "f16", "bf16"), crop=(12
         snaps = hip.snap_buffer(tshot, "f32", kt)
         hip.snap_forward_steps(tshot, Vh, coefs, dg, dsrc, S, snaps, "f32", kt, 0, nt)
         row = ae.collect_bricks(snaps, slots, pool, row, scl)
+        for j in slots:
+            learner.add(snaps.slot(j).download(), b)
         S.free()
         snaps.free()
     codec = ae.Codec(max_bricks=256, seed=3)
@@ -621,6 +626,23 @@ def accuracy_crop(hip, host, workloads, formats=("f32", "f16", "bf16"), crop=(12
         x.free()
     cs.free()
     codec.destroy()
+    # transform-coded snapshots with the learned basis
+    basis = learner.basis()
+    for wfmt, div in (("f32", 32), ("f16", 32), ("f32", 64)):
+        wdt = hip.WAVE_FORMATS[wfmt][1]
+        shot = hip.make_shot(*shape, b, order, nt, nt, (cx // 2 + b, cy // 2 + b, 2 + b), geom,
+                             wfmt)
+        S, R = hip.FieldPair(shape, wdt), hip.FieldPair(shape, wdt)
+        for f in (S.a, S.b, R.a, R.b):
+            f.zero()
+        st = hip.DctSnapshots(shot, 4, div, basis)
+        img = hip.DeviceArray(shape).zero()
+        st.forward_steps(Vh, coefs, dg, dsrc, S, 0, nt)
+        st.backward_steps(Vh, coefs, dg, R, rcv32, img, nt - 1, nt)
+        snap_imgs[(wfmt, "dct%d_learned_basis" % div, 4)] = finish(img)
+        for x in (S, R, img):
+            x.free()
+        st.free()
     for x in (V, Vh, dsrc, dg, rcv32, out_img):
         x.free()
     t = time.perf_counter()
@@ -639,6 +661,9 @@ def accuracy_crop(hip, host, workloads, formats=("f32", "f16", "bf16"), crop=(12
         snap_acc["%s_wavefields_%s_snapshots_k%d_image_rel_l2_vs_exact_snapshot"
                  % (wfmt, sfmt, k)] = rel_l2(im, exact)
     snap_acc["coded_snapshot_codec"] = coded
+    snap_acc["learned_basis_training"] = ("principal directions of the z, x and y lines of "
+                                          "the 8x8x32 bricks of %d snapshots of the 2 "
+                                          "training shots" % (2 * len(slots)))
     case = "%dx%dx%d crop of the salt model, border 16, %d steps, %dx%d receivers" % (
         cx, cy, cz, nt, nr[0], nr[1])
     return acc, snap_acc, case
@@ -1039,8 +1064,7 @@ def main():
         try:
             acc, snap_acc, case = accuracy_crop(hip, host, workloads)
             line["accuracy_vs_fp64"] = dict(acc, case=case)
-            if "snapshot" in line:
-                line["snapshot"]["accuracy"] = dict(snap_acc, case=case)
+            line.setdefault("snapshot", {})["accuracy"] = dict(snap_acc, case=case)
         except Exception as e:
             line["accuracy_vs_fp64"] = {"error": "%s: %s" % (type(e).__name__, e)}
     if budget.allows("precision", 90.0):

@@ -253,13 +253,19 @@ int rtmhip_snap_backward_steps(const rtmhip_shot *shot, const float *V2DT2, cons
  * arithmetic (fused multiply-adds in the transforms), not bit-identical to
  * a reference operator: the image error against exact snapshots is
  * measured.  rtmhip_snap_dct_sizes gives the byte sizes of the store (all
- * snapshots) and of the work buffer (any output may be NULL). */
+ * snapshots, after an 8 KB header holding the basis) and of the work buffer
+ * (any output may be NULL).
+ * basis (host, forward and roundtrip calls): NULL for the DCT-II, or a
+ * learned separable basis -- 32x32 along z, then 8x8 along x, then 8x8 along
+ * y, row-major, each row a basis vector, rows orthonormal to 1e-4 (else
+ * RTMHIP_ERR_INVALID), the same on every forward call of one shot; the
+ * store keeps it for the backward pass. */
 int rtmhip_snap_dct_sizes(const rtmhip_shot *shot, int k, int keep_div, size_t *store_bytes,
                           size_t *work_bytes);
 int rtmhip_snap_dct_forward_steps(const rtmhip_shot *shot, const float *V2DT2,
                                   const float *coefs, const float *taper, const float *ricker,
-                                  float **P, float **PP, void *store, void *work, int k,
-                                  int keep_div, int it0, int nsteps);
+                                  float **P, float **PP, void *store, void *work,
+                                  const float *basis, int k, int keep_div, int it0, int nsteps);
 int rtmhip_snap_dct_backward_steps(const rtmhip_shot *shot, const float *V2DT2,
                                    const float *coefs, const float *taper, float **Pr,
                                    float **PPr, const void *store, int k, int keep_div,
@@ -270,8 +276,8 @@ int rtmhip_snap_dct_backward_steps(const rtmhip_shot *shot, const float *V2DT2,
  * cells of out (FP32 grid); *kept (host, optional) receives the number of
  * coefficients kept. */
 int rtmhip_dct_roundtrip(const void *field, int wave_format, int nxe, int nye, int nze,
-                         int blen, int keep_div, void *store, void *work, float *out,
-                         unsigned *kept);
+                         int blen, int keep_div, void *store, void *work, const float *basis,
+                         float *out, unsigned *kept);
 
 /* ================= batches of shots =================
  * No reference counterpart: the reference migrates one shot at a time per

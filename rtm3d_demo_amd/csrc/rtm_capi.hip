@@ -850,6 +850,7 @@ int snap_backward_loop(const rtmhip_shot *shot, const ShotCtx &c, const float *V
 struct DctSlots {
     DctLayout L;
     long long K;              // coefficient budget per snapshot
+    size_t base;              // byte offset of slot 0 (the basis precedes it)
     size_t entries, table, meta, slot;  // byte offsets in a slot, slot size
     size_t coef, hist, work;            // byte offsets in work, work size
 };
@@ -864,6 +865,7 @@ int dct_slots(const Grid &g, int blen, int keep_div, DctSlots *d)
     d->K = std::max(1LL, cells / keep_div);
     if (d->K >= (1LL << 31) || d->L.nbr * 3 >= (1LL << 31))
         return fail(RTMHIP_ERR_INVALID, "snapshot too large for 32-bit entry offsets");
+    d->base = align256(sizeof(BrickBasis)) + 4096;
     d->entries = 0;
     d->table = align256((size_t)d->K * 4);
     d->meta = d->table + align256((size_t)d->L.nbr * 12);
@@ -878,22 +880,49 @@ template <typename T>
 hipError_t dct_store(const T *f, const Grid &g, int b, const DctSlots &d, void *store, int j,
                      void *work)
 {
-    char *sl = static_cast<char *>(store) + (size_t)j * d.slot;
+    char *sl = static_cast<char *>(store) + d.base + (size_t)j * d.slot;
     char *wk = static_cast<char *>(work);
-    return launch_dct_encode(f, g, b, g.nz - b, d.L, d.K, (float *)(wk + d.coef),
-                             (unsigned *)(wk + d.hist), (unsigned *)(sl + d.meta),
-                             (unsigned *)(sl + d.entries), (unsigned *)(sl + d.table),
-                             g_state.stream);
+    return launch_dct_encode(f, g, b, g.nz - b, d.L, static_cast<const BrickBasis *>(store), d.K,
+                             (float *)(wk + d.coef), (unsigned *)(wk + d.hist),
+                             (unsigned *)(sl + d.meta), (unsigned *)(sl + d.entries),
+                             (unsigned *)(sl + d.table), g_state.stream);
 }
 
 template <typename T>
 hipError_t dct_image(const void *store, int j, const DctSlots &d, const Grid &g, int b,
                      const T *pr, float *img, float w)
 {
-    const char *sl = static_cast<const char *>(store) + (size_t)j * d.slot;
+    const char *sl = static_cast<const char *>(store) + d.base + (size_t)j * d.slot;
     return launch_dct_decode(reinterpret_cast<const unsigned *>(sl + d.entries),
-                             reinterpret_cast<const unsigned *>(sl + d.table), d.L, g, b,
-                             g.nz - b, pr, img, w, (float *)nullptr, g_state.stream);
+                             reinterpret_cast<const unsigned *>(sl + d.table), d.L,
+                             static_cast<const BrickBasis *>(store), g, b, g.nz - b, pr, img, w,
+                             (float *)nullptr, g_state.stream);
+}
+
+// The store's basis: the DCT-II (basis NULL) or the caller's (host: z[32][32],
+// x[8][8], y[8][8], rows orthonormal to 1e-4, checked here).
+int put_basis(void *store, const float *basis)
+{
+    if (!basis) {
+        HIPCHK(dct_default_basis(static_cast<BrickBasis *>(store), g_state.stream), "basis");
+        return RTMHIP_OK;
+    }
+    const float *m[3] = {basis, basis + 32 * 32, basis + 32 * 32 + 64};
+    const int n[3] = {32, 8, 8};
+    for (int a = 0; a < 3; ++a)
+        for (int u = 0; u < n[a]; ++u)
+            for (int v = 0; v < n[a]; ++v) {
+                double dot = 0.0;
+                for (int i = 0; i < n[a]; ++i)
+                    dot += (double)m[a][u * n[a] + i] * (double)m[a][v * n[a] + i];
+                if (std::fabs(dot - (u == v ? 1.0 : 0.0)) > 1e-4)
+                    return fail(RTMHIP_ERR_INVALID, "basis rows are not orthonormal");
+            }
+    HIPCHK(hipMemcpyAsync(store, basis, sizeof(BrickBasis), hipMemcpyHostToDevice,
+                          g_state.stream),
+           "hipMemcpyAsync");
+    HIPCHK(hipStreamSynchronize(g_state.stream), "hipStreamSynchronize");
+    return RTMHIP_OK;
 }
 
 int check_snap(const rtmhip_shot *shot, const void *snaps, int snap_format, int k)
@@ -1029,15 +1058,15 @@ int rtmhip_snap_dct_sizes(const rtmhip_shot *shot, int k, int keep_div, size_t *
     DctSlots d;
     if ((rc = dct_slots(dense_grid(shot->nxe, shot->nye, shot->nze), shot->blen, keep_div, &d)))
         return rc;
-    if (store_bytes) *store_bytes = (size_t)((shot->nt - 1) / k + 1) * d.slot;
+    if (store_bytes) *store_bytes = d.base + (size_t)((shot->nt - 1) / k + 1) * d.slot;
     if (work_bytes) *work_bytes = d.work;
     return RTMHIP_OK;
 }
 
 int rtmhip_snap_dct_forward_steps(const rtmhip_shot *shot, const float *V2DT2,
                                   const float *coefs, const float *taper, const float *ricker,
-                                  float **P, float **PP, void *store, void *work, int k,
-                                  int keep_div, int it0, int nsteps)
+                                  float **P, float **PP, void *store, void *work,
+                                  const float *basis, int k, int keep_div, int it0, int nsteps)
 {
     NEED_INIT();
     ShotCtx c;
@@ -1053,6 +1082,7 @@ int rtmhip_snap_dct_forward_steps(const rtmhip_shot *shot, const float *V2DT2,
     if (nsteps <= 0) return RTMHIP_OK;
     if (it0 < 0 || it0 + nsteps > shot->nt)
         return fail(RTMHIP_ERR_INVALID, "time range outside [0,%d)", shot->nt);
+    if ((rc = put_basis(store, basis))) return rc;
     return with_format(shot, [&](auto t) {
         using T = decltype(t);
         return snap_forward_run(shot, c, V2DT2, taper, ricker, as<T>(P), as<T>(PP), k, it0,
@@ -1095,8 +1125,8 @@ int rtmhip_snap_dct_backward_steps(const rtmhip_shot *shot, const float *V2DT2,
 }
 
 int rtmhip_dct_roundtrip(const void *field, int wave_format, int nxe, int nye, int nze,
-                         int blen, int keep_div, void *store, void *work, float *out,
-                         unsigned *kept)
+                         int blen, int keep_div, void *store, void *work, const float *basis,
+                         float *out, unsigned *kept)
 {
     NEED_INIT();
     if (!field || !store || !work || !out) return fail(RTMHIP_ERR_INVALID, "NULL pointer");
@@ -1106,7 +1136,8 @@ int rtmhip_dct_roundtrip(const void *field, int wave_format, int nxe, int nye, i
     const Grid g = dense_grid(nxe, nye, nze);
     DctSlots d;
     if ((rc = dct_slots(g, blen, keep_div, &d))) return rc;
-    const char *sl = static_cast<const char *>(store);
+    if ((rc = put_basis(store, basis))) return rc;
+    const char *sl = static_cast<const char *>(store) + d.base;
     hipError_t e;
     switch (wave_format) {
     case RTMHIP_WAVE_F32: e = dct_store((const float *)field, g, blen, d, store, 0, work); break;
@@ -1116,8 +1147,9 @@ int rtmhip_dct_roundtrip(const void *field, int wave_format, int nxe, int nye, i
     }
     HIPCHK(e, "transform coding");
     HIPCHK(launch_dct_decode(reinterpret_cast<const unsigned *>(sl + d.entries),
-                             reinterpret_cast<const unsigned *>(sl + d.table), d.L, g, blen,
-                             nze - blen, (const float *)nullptr, (float *)nullptr, 0.0f, out,
+                             reinterpret_cast<const unsigned *>(sl + d.table), d.L,
+                             static_cast<const BrickBasis *>(store), g, blen, nze - blen,
+                             (const float *)nullptr, (float *)nullptr, 0.0f, out,
                              g_state.stream),
            "decoding");
     if (kept) {

@@ -31,10 +31,11 @@ constexpr int kLine = 33;                   // padded z line in LDS
 constexpr int kTile = 8 * 8 * kLine;        // one brick in LDS
 constexpr int kCells = 8 * 8 * 32;          // 2048 cells / coefficients per brick
 
-// Orthonormal DCT-II matrices, D[u][i] = s(u) cos(pi (2i + 1) u / 2n),
-// s(0) = sqrt(1/n), s(u > 0) = sqrt(2/n), evaluated at compile time (cosine
-// by range reduction and its Taylor series in double) and held in constant
-// memory: the transforms index them wave-uniformly, so they are scalar loads.
+// The default basis, orthonormal DCT-II matrices D[u][i] = s(u) cos(pi (2i +
+// 1) u / 2n), s(0) = sqrt(1/n), s(u > 0) = sqrt(2/n), evaluated at compile
+// time (cosine by range reduction and its Taylor series in double).  The
+// transforms read the basis through a pointer with wave-uniform indices, so
+// its entries come through scalar loads.
 constexpr double kPi = 3.14159265358979323846;
 constexpr double ccos(double x)
 {
@@ -48,24 +49,29 @@ constexpr double ccos(double x)
     }
     return sum;
 }
-struct DctTab {
-    float d32[32][32];
-    float d8[8][8];
-};
-constexpr DctTab make_dct_tab()
+constexpr BrickBasis make_dct_basis()
 {
-    DctTab t{};
+    BrickBasis t{};
     for (int u = 0; u < 32; ++u)
         for (int i = 0; i < 32; ++i)
-            t.d32[u][i] = (float)((u ? 0.25 : 0.17677669529663688) *
-                                  ccos(kPi * (double)((2 * i + 1) * u) / 64.0));
+            t.z[u][i] = (float)((u ? 0.25 : 0.17677669529663688) *
+                                ccos(kPi * (double)((2 * i + 1) * u) / 64.0));
     for (int u = 0; u < 8; ++u)
-        for (int i = 0; i < 8; ++i)
-            t.d8[u][i] = (float)((u ? 0.5 : 0.35355339059327379) *
-                                 ccos(kPi * (double)((2 * i + 1) * u) / 16.0));
+        for (int i = 0; i < 8; ++i) {
+            t.x[u][i] = (float)((u ? 0.5 : 0.35355339059327379) *
+                                ccos(kPi * (double)((2 * i + 1) * u) / 16.0));
+            t.y[u][i] = t.x[u][i];
+        }
     return t;
 }
-__constant__ DctTab kDct = make_dct_tab();
+__constant__ BrickBasis kDct = make_dct_basis();
+
+__global__ void k_put_basis(BrickBasis *__restrict__ dst)
+{
+    const float *src = &kDct.z[0][0];
+    float *d = &dst->z[0][0];
+    for (int i = threadIdx.x; i < (int)(sizeof(BrickBasis) / 4); i += blockDim.x) d[i] = src[i];
+}
 
 __device__ __forceinline__ void brick_coords(long long br, const DctLayout &L, int &x0, int &y0,
                                              int &z0)
@@ -84,7 +90,8 @@ __device__ __forceinline__ void brick_coords(long long br, const DctLayout &L, i
 // pass's outputs stay in the thread's out (x index u = 0..7 at y = tid>>5,
 // z = tid&31).  z pass: wave q computes outputs 8q .. 8q+7 of the 64 lines
 // (lane = line), so every matrix index is wave-uniform.
-__device__ __forceinline__ void dct3_forward(float *t, float (&out)[8])
+__device__ __forceinline__ void dct3_forward(float *t, const BrickBasis *__restrict__ B,
+                                             float (&out)[8])
 {
     const int tid = threadIdx.x;
     {   // z
@@ -98,7 +105,7 @@ __device__ __forceinline__ void dct3_forward(float *t, float (&out)[8])
         for (int r = 0; r < 8; ++r) {
             float a = 0.0f;
 #pragma unroll
-            for (int k = 0; k < 32; ++k) a = __builtin_fmaf(kDct.d32[8 * q + r][k], in[k], a);
+            for (int k = 0; k < 32; ++k) a = __builtin_fmaf(B->z[8 * q + r][k], in[k], a);
             o[r] = a;
         }
         __syncthreads();
@@ -115,7 +122,7 @@ __device__ __forceinline__ void dct3_forward(float *t, float (&out)[8])
         for (int u = 0; u < 8; ++u) {
             float a = 0.0f;
 #pragma unroll
-            for (int j = 0; j < 8; ++j) a = __builtin_fmaf(kDct.d8[u][j], in[j], a);
+            for (int j = 0; j < 8; ++j) a = __builtin_fmaf(B->y[u][j], in[j], a);
             o[u] = a;
         }
         __syncthreads();
@@ -132,14 +139,14 @@ __device__ __forceinline__ void dct3_forward(float *t, float (&out)[8])
         for (int u = 0; u < 8; ++u) {
             float a = 0.0f;
 #pragma unroll
-            for (int i = 0; i < 8; ++i) a = __builtin_fmaf(kDct.d8[u][i], in[i], a);
+            for (int i = 0; i < 8; ++i) a = __builtin_fmaf(B->x[u][i], in[i], a);
             out[u] = a;
         }
     }
 }
 
 // Inverse transform in place (coefficients in t as dct3_forward leaves them).
-__device__ __forceinline__ void dct3_inverse(float *t)
+__device__ __forceinline__ void dct3_inverse(float *t, const BrickBasis *__restrict__ B)
 {
     const int tid = threadIdx.x;
     {   // x
@@ -151,7 +158,7 @@ __device__ __forceinline__ void dct3_inverse(float *t)
         for (int i = 0; i < 8; ++i) {
             float a = 0.0f;
 #pragma unroll
-            for (int u = 0; u < 8; ++u) a = __builtin_fmaf(kDct.d8[u][i], in[u], a);
+            for (int u = 0; u < 8; ++u) a = __builtin_fmaf(B->x[u][i], in[u], a);
             o[i] = a;
         }
         __syncthreads();
@@ -168,7 +175,7 @@ __device__ __forceinline__ void dct3_inverse(float *t)
         for (int j = 0; j < 8; ++j) {
             float a = 0.0f;
 #pragma unroll
-            for (int u = 0; u < 8; ++u) a = __builtin_fmaf(kDct.d8[u][j], in[u], a);
+            for (int u = 0; u < 8; ++u) a = __builtin_fmaf(B->y[u][j], in[u], a);
             o[j] = a;
         }
         __syncthreads();
@@ -187,7 +194,7 @@ __device__ __forceinline__ void dct3_inverse(float *t)
         for (int r = 0; r < 8; ++r) {
             float a = 0.0f;
 #pragma unroll
-            for (int w = 0; w < 32; ++w) a = __builtin_fmaf(kDct.d32[w][8 * q + r], in[w], a);
+            for (int w = 0; w < 32; ++w) a = __builtin_fmaf(B->z[w][8 * q + r], in[w], a);
             o[r] = a;
         }
         __syncthreads();
@@ -207,7 +214,8 @@ __device__ __forceinline__ int tile_index(int e) { return (e >> 5) * kLine + (e 
 template <typename T>
 __global__ void __launch_bounds__(kDctThreads)
     k_dct_encode(const T *__restrict__ F, Grid g, int z0, int z1, DctLayout L,
-                 float *__restrict__ coef, unsigned *__restrict__ hist)
+                 const BrickBasis *__restrict__ B, float *__restrict__ coef,
+                 unsigned *__restrict__ hist)
 {
     __shared__ float t[kTile];
     __shared__ unsigned h[256];
@@ -228,7 +236,7 @@ __global__ void __launch_bounds__(kDctThreads)
         }
         __syncthreads();
         float out[8];
-        dct3_forward(t, out);
+        dct3_forward(t, B, out);
         float *cb = coef + br * kCells;
         const int j = tid >> 5, k = tid & 31;
 #pragma unroll
@@ -346,8 +354,9 @@ __global__ void __launch_bounds__(kDctThreads)
 template <typename T, bool IMG>
 __global__ void __launch_bounds__(kDctThreads)
     k_dct_decode(const unsigned *__restrict__ entries, const unsigned *__restrict__ table,
-                 DctLayout L, Grid g, int z0, int z1, const T *__restrict__ PR,
-                 float *__restrict__ img, float w, float *__restrict__ out)
+                 DctLayout L, const BrickBasis *__restrict__ B, Grid g, int z0, int z1,
+                 const T *__restrict__ PR, float *__restrict__ img, float w,
+                 float *__restrict__ out)
 {
     __shared__ float t[kTile];
     const int tid = threadIdx.x;
@@ -388,7 +397,7 @@ __global__ void __launch_bounds__(kDctThreads)
             }
         }
         __syncthreads();
-        dct3_inverse(t);
+        dct3_inverse(t, B);
 #pragma unroll
         for (int m = 0; m < 8; ++m) {
             if (in[m]) {
@@ -415,6 +424,12 @@ inline unsigned dct_blocks(long long nbr) { return (unsigned)std::min<long long>
 
 }  // namespace
 
+hipError_t dct_default_basis(BrickBasis *dst, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_put_basis, dim3(1), dim3(256), 0, s, dst);
+    return hipGetLastError();
+}
+
 DctLayout dct_layout(const Grid &g)
 {
     DctLayout L{};
@@ -428,14 +443,14 @@ DctLayout dct_layout(const Grid &g)
 
 template <typename T>
 hipError_t launch_dct_encode(const T *F, const Grid &g, int z0, int z1, const DctLayout &L,
-                             long long K, float *coef, unsigned *hist, unsigned *meta,
-                             unsigned *entries, unsigned *table, hipStream_t s)
+                             const BrickBasis *B, long long K, float *coef, unsigned *hist,
+                             unsigned *meta, unsigned *entries, unsigned *table, hipStream_t s)
 {
     if (L.nbr <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_dct_clear, dim3(1), dim3(256), 0, s, hist);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_dct_encode<T>, dim3(L.nblk), dim3(kDctThreads), 0, s, F, g, z0, z1, L,
+    hipLaunchKernelGGL(k_dct_encode<T>, dim3(L.nblk), dim3(kDctThreads), 0, s, F, g, z0, z1, L, B,
                        coef, hist);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     hipLaunchKernelGGL(k_dct_select, dim3(1), dim3(kDctThreads), 0, s, hist, K, meta);
@@ -447,26 +462,28 @@ hipError_t launch_dct_encode(const T *F, const Grid &g, int z0, int z1, const Dc
 
 template <typename T>
 hipError_t launch_dct_decode(const unsigned *entries, const unsigned *table, const DctLayout &L,
-                             const Grid &g, int z0, int z1, const T *PR, float *img, float w,
-                             float *out, hipStream_t s)
+                             const BrickBasis *B, const Grid &g, int z0, int z1, const T *PR,
+                             float *img, float w, float *out, hipStream_t s)
 {
     if (L.nbr <= 0) return hipSuccess;
     if (PR)
         hipLaunchKernelGGL((k_dct_decode<T, true>), dim3(L.nblk), dim3(kDctThreads), 0, s,
-                           entries, table, L, g, z0, z1, PR, img, w, out);
+                           entries, table, L, B, g, z0, z1, PR, img, w, out);
     else
         hipLaunchKernelGGL((k_dct_decode<T, false>), dim3(L.nblk), dim3(kDctThreads), 0, s,
-                           entries, table, L, g, z0, z1, PR, img, w, out);
+                           entries, table, L, B, g, z0, z1, PR, img, w, out);
     return hipGetLastError();
 }
 
 #define RTM_INSTANTIATE_DCT(T)                                                                 \
     template hipError_t launch_dct_encode<T>(const T *, const Grid &, int, int,                \
-                                             const DctLayout &, long long, float *, unsigned *,  \
-                                             unsigned *, unsigned *, unsigned *, hipStream_t);   \
+                                             const DctLayout &, const BrickBasis *, long long,   \
+                                             float *, unsigned *, unsigned *, unsigned *,        \
+                                             unsigned *, hipStream_t);                           \
     template hipError_t launch_dct_decode<T>(const unsigned *, const unsigned *,               \
-                                             const DctLayout &, const Grid &, int, int,          \
-                                             const T *, float *, float, float *, hipStream_t);
+                                             const DctLayout &, const BrickBasis *,              \
+                                             const Grid &, int, int, const T *, float *, float,  \
+                                             float *, hipStream_t);
 RTM_INSTANTIATE_DCT(float)
 RTM_INSTANTIATE_DCT(_Float16)
 RTM_INSTANTIATE_DCT(__bf16)

@@ -135,6 +135,16 @@ hipError_t launch_snap_image(float *img, const S *snap, const T *PR, const Grid 
 // 3D DCT-II, one power-of-two threshold per snapshot keeping at most K
 // coefficients, entries {index in brick (11 bits), FP16 value / brick scale}
 // and a brick table {first entry, count, scale bits} (3 x 32 bits).
+// Separable orthonormal basis of a brick: rows are the basis vectors along
+// z (32), x (8) and y (8); forward coefficient (u, v, w) = sum over the
+// brick of x[u][i] y[v][j] z[w][k] f(i, j, k).  dct_default_basis writes the
+// DCT-II into device memory.
+struct BrickBasis {
+    float z[32][32];
+    float x[8][8];
+    float y[8][8];
+};
+hipError_t dct_default_basis(BrickBasis *dst, hipStream_t s);
 struct DctLayout {
     int nbx, nby, nbz;  // bricks along x, y, z
     int nblk;           // blocks per launch (each loops over bricks)
@@ -145,13 +155,13 @@ DctLayout dct_layout(const Grid &g);
 // (threshold bits, kept count, entry counter); the cells z in [z0, z1) enter.
 template <typename T>
 hipError_t launch_dct_encode(const T *F, const Grid &g, int z0, int z1, const DctLayout &L,
-                             long long K, float *coef, unsigned *hist, unsigned *meta,
-                             unsigned *entries, unsigned *table, hipStream_t s);
+                             const BrickBasis *B, long long K, float *coef, unsigned *hist,
+                             unsigned *meta, unsigned *entries, unsigned *table, hipStream_t s);
 // PR != NULL: img += w * (v * PR) on z in [z0, z1); else out = v there.
 template <typename T>
 hipError_t launch_dct_decode(const unsigned *entries, const unsigned *table, const DctLayout &L,
-                             const Grid &g, int z0, int z1, const T *PR, float *img, float w,
-                             float *out, hipStream_t s);
+                             const BrickBasis *B, const Grid &g, int z0, int z1, const T *PR,
+                             float *img, float w, float *out, hipStream_t s);
 // taperUpper3D of two grids of the same shape in one pass (Q may be NULL).
 template <typename T>
 hipError_t launch_taper_upper2(T *P, T *Q, const float *taper, const Grid &g, int b,

@@ -139,11 +139,11 @@ def load():
                                           i, fp, fp, i, i],
         "rtmhip_snap_dct_sizes": [C.POINTER(Shot), i, i, C.POINTER(C.c_size_t),
                                   C.POINTER(C.c_size_t)],
-        "rtmhip_snap_dct_forward_steps": [C.POINTER(Shot), fp, fp, fp, fp, pp, pp, vp, vp, i, i,
-                                          i, i],
+        "rtmhip_snap_dct_forward_steps": [C.POINTER(Shot), fp, fp, fp, fp, pp, pp, vp, vp, fp, i,
+                                          i, i, i],
         "rtmhip_snap_dct_backward_steps": [C.POINTER(Shot), fp, fp, fp, pp, pp, vp, i, i, fp, fp,
                                            i, i],
-        "rtmhip_dct_roundtrip": [vp, i, i, i, i, i, i, vp, vp, fp, C.POINTER(C.c_uint)],
+        "rtmhip_dct_roundtrip": [vp, i, i, i, i, i, i, vp, vp, fp, fp, C.POINTER(C.c_uint)],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -379,23 +379,34 @@ def snap_backward_steps(shot, V, coefs, g, rcv_fields, snaps, fmt, k, rcv_chunk,
                                             img.ptr, it_hi, nsteps))
 
 
+def _basis_ptr(basis):
+    if basis is None:
+        return None, None
+    b = np.ascontiguousarray(basis, np.float32).ravel()
+    assert b.size == 32 * 32 + 2 * 64, b.size
+    return b, _host_ptr(b)
+
+
 class DctSnapshots:
     """Transform-coded snapshots of one shot (rtmhip_snap_dct_*): the store of
     every snapshot and the work buffer; keep_div sets the coefficient budget
-    (nxe*nye*(nze-2*blen) / keep_div per snapshot)."""
+    (nxe*nye*(nze-2*blen) / keep_div per snapshot); basis: None for the
+    DCT-II, or a learned separable basis (basis.learn_brick_basis)."""
 
-    def __init__(self, shot, k, keep_div=32):
+    def __init__(self, shot, k, keep_div=32, basis=None):
         sb, wb = C.c_size_t(0), C.c_size_t(0)
         check(load().rtmhip_snap_dct_sizes(C.byref(shot), k, keep_div, C.byref(sb), C.byref(wb)))
-        self.shot, self.k, self.keep_div = shot, k, keep_div
+        self.shot, self.k, self.keep_div, self.basis = shot, k, keep_div, basis
         self.store = DeviceArray((sb.value,), np.uint8)
         self.work = DeviceArray((wb.value,), np.uint8)
 
     def forward_steps(self, V, coefs, g, ricker, fields, it0, nsteps):
         c = _f32(coefs)
+        keep, bp = _basis_ptr(self.basis)
         check(load().rtmhip_snap_dct_forward_steps(
             C.byref(self.shot), V.ptr, _host_ptr(c), g.ptr, ricker.ptr, C.byref(fields.p),
-            C.byref(fields.pp), self.store.ptr, self.work.ptr, self.k, self.keep_div, it0, nsteps))
+            C.byref(fields.pp), self.store.ptr, self.work.ptr, bp, self.k, self.keep_div, it0,
+            nsteps))
 
     def backward_steps(self, V, coefs, g, rcv_fields, rcv_chunk, img, it_hi, nsteps):
         c = _f32(coefs)
@@ -412,7 +423,7 @@ class DctSnapshots:
         self.work.free()
 
 
-def dct_roundtrip(field, fmt, blen, keep_div=32):
+def dct_roundtrip(field, fmt, blen, keep_div=32, basis=None):
     """Encodes the imaging range of a device field and decodes it: returns
     (decoded FP32 host array with the other cells 0, kept coefficients)."""
     nxe, nye, nze = field.shape
@@ -423,8 +434,9 @@ def dct_roundtrip(field, fmt, blen, keep_div=32):
     out = DeviceArray(field.shape).zero()
     kept = C.c_uint(0)
     try:
+        keep, bp = _basis_ptr(basis)
         check(load().rtmhip_dct_roundtrip(field.ptr, WAVE_FORMATS[fmt][0], nxe, nye, nze, blen,
-                                          keep_div, store.ptr, work.ptr, out.ptr,
+                                          keep_div, store.ptr, work.ptr, bp, out.ptr,
                                           C.byref(kept)))
         return out.download(), kept.value
     finally:

@@ -21,8 +21,15 @@ and the relative L2 of the coded image against the yardstick image over the
 first two held-out shots are also migrated with the untrained network (the
 seeded initial weights) to show what training gains.
 
+--codec basis evaluates the transform coder instead (rtmhip_snap_dct_*): its
+learned separable basis is trained on FP32 snapshots of the first
+--basis-train training shots (every 150th step after nt/3), and each held-out
+shot is migrated with the DCT-II and with the learned basis at each
+--keep-div budget, against the same FP16-snapshot yardstick.
+
     python3 rtm3d_demo_amd/learned_eval.py [--nt 3000] [--train 32] [--test 16]
-                                           [--iters 4000]
+                                           [--iters 4000] [--codec ae|basis]
+                                           [--basis-train 8] [--keep-div 32,64]
 Prints progress lines to stderr and one JSON line to stdout.
 """
 import argparse
@@ -55,6 +62,92 @@ def positions(seed, n, taken=()):
     return [cand[i] for i in idx]
 
 
+def basis_eval(c, keep_divs):
+    """Transform-coded snapshot RTM of the held-out shots with the DCT-II and
+    with a basis learned from the training shots' snapshots."""
+    from rtm3d_demo_amd.basis import BasisLearner
+    hip, shape, b, nt, k = c["hip"], c["shape"], c["b"], c["nt"], c["k"]
+    V, coefs, g, src = c["V"], c["coefs"], c["g"], c["src"]
+    kt = 150
+    slots = [j for j in range((nt - 1) // kt + 1) if nt - 1 - j * kt >= nt // 3]
+    learner = BasisLearner()
+    F, R = hip.FieldPair(shape), hip.FieldPair(shape)
+    t = time.perf_counter()
+    for i, pos in enumerate(c["train"]):
+        shot = c["mk"](pos)
+        for f in (F.a, F.b):
+            f.zero()
+        F.p.value, F.pp.value = F.a.ptr.value, F.b.ptr.value
+        snaps = hip.snap_buffer(shot, "f32", kt)
+        hip.snap_forward_steps(shot, V, coefs, g, src, F, snaps, "f32", kt, 0, nt)
+        for j in slots:
+            learner.add(snaps.slot(j).download(), b)
+        snaps.free()
+        log("basis training shot %d/%d at %s" % (i + 1, len(c["train"]), pos))
+    basis = learner.basis()
+    train_s = time.perf_counter() - t
+    img = hip.DeviceArray(shape).zero()
+    rcv = hip.DeviceArray(c["nr"] + (nt,))
+    fp16 = hip.snap_buffer(c["mk"](c["test"][0]), "f16", k)
+
+    def migrate(shot, div=None, bas=None):
+        for f in (F.a, F.b, R.a, R.b, img):
+            f.zero()
+        F.p.value, F.pp.value = F.a.ptr.value, F.b.ptr.value
+        R.p.value, R.pp.value = R.a.ptr.value, R.b.ptr.value
+        hip.synchronize()
+        t = time.perf_counter()
+        if div is None:
+            hip.snap_forward_steps(shot, V, coefs, g, src, F, fp16, "f16", k, 0, nt)
+            hip.snap_backward_steps(shot, V, coefs, g, R, fp16, "f16", k, rcv, img, nt - 1, nt)
+        else:
+            st = hip.DctSnapshots(shot, k, div, bas)
+            st.forward_steps(V, coefs, g, src, F, 0, nt)
+            st.backward_steps(V, coefs, g, R, rcv, img, nt - 1, nt)
+            st.free()
+        hip.synchronize()
+        s = time.perf_counter() - t
+        return img.download()[b:-b, b:-b, b:-b], s
+
+    results = []
+    for i, pos in enumerate(c["test"]):
+        shot = c["mk"](pos)
+        for f in (F.a, F.b):
+            f.zero()
+        F.p.value, F.pp.value = F.a.ptr.value, F.b.ptr.value
+        rcv.zero()
+        hip.model_steps(shot, c["V_abc"], coefs, g, src, F, rcv, 0, nt)
+        ref, s_ref = migrate(shot)
+        r = {"source": list(pos), "fp16_snapshot_rtm_s": round(s_ref, 3)}
+        for div in keep_divs:
+            for name, bas in (("dct", None), ("learned", basis)):
+                got, s = migrate(shot, div, bas)
+                r["%s_keep%d_image_rel_l2" % (name, div)] = rel(got, ref)
+                r["%s_keep%d_rtm_s" % (name, div)] = round(s, 3)
+        results.append(r)
+        log("held-out shot %d/%d: %s" % (i + 1, len(c["test"]), json.dumps(r)))
+    st = hip.DctSnapshots(c["mk"](c["test"][0]), k, keep_divs[0])
+    summary = {}
+    for key in results[0]:
+        if key.endswith("image_rel_l2"):
+            v = [r[key] for r in results]
+            summary[key] = {"mean": float(np.mean(v)), "max": float(np.max(v))}
+    out = {
+        "workload": "salt 512x512x256 at 12.5 m, HBC border 40, %d steps, snapshots every %d "
+                    "steps, transform-coded 8x8x32 bricks" % (nt, k),
+        "training": {"shots": len(c["train"]), "snapshots": len(c["train"]) * len(slots),
+                     "seconds": round(train_s, 1)},
+        "held_out": results, "summary": summary,
+        "coded_GB_per_shot_keep%d" % keep_divs[0]: round(st.nbytes() / 1e9, 2),
+        "fp16_snapshot_GB_per_shot": round(fp16.nbytes / 1e9, 2),
+        "yardstick": "snapshot RTM with FP16 snapshots at the same interval",
+    }
+    st.free()
+    for x in (F, R, img, rcv, fp16):
+        x.free()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--nt", type=int, default=3000)
@@ -62,6 +155,9 @@ def main():
     ap.add_argument("--test", type=int, default=16)
     ap.add_argument("--iters", type=int, default=4000)
     ap.add_argument("--k", type=int, default=4)
+    ap.add_argument("--codec", choices=("ae", "basis"), default="ae")
+    ap.add_argument("--basis-train", type=int, default=8)
+    ap.add_argument("--keep-div", default="32,64")
     args = ap.parse_args()
     from rtm3d_demo_amd import ae, hip, host, workloads
     hip.init(0)
@@ -90,6 +186,15 @@ def main():
     region = (0, 0, 0, shape[0], shape[1], lz)
     nb = ae.region_bricks(region)
     L = hip.load()
+    if args.codec == "basis":
+        ctx = dict(hip=hip, V=V, V_abc=V_abc, coefs=coefs, g=g, src=src, shape=shape, b=b,
+                   nt=nt, k=k, nr=nr, mk=mk, train=train[:args.basis_train], test=test)
+        out = basis_eval(ctx, [int(d) for d in args.keep_div.split(",")])
+        for x in (V_abc, V, src, g):
+            x.free()
+        hip.finalize()
+        print(json.dumps(out), flush=True)
+        return 0
 
     # ---- training pool
     kt, per = 150, 64

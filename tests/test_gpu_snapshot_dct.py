@@ -47,11 +47,15 @@ def _unbricks(br, shape, padded, b):
     return out
 
 
-def coder(f, b, keep_div):
-    """float64 restatement of the device coder: (decoded field, kept)."""
-    d8, d32 = _dct_matrix(8), _dct_matrix(32)
+def coder(f, b, keep_div, basis=None):
+    """float64 restatement of the device coder: (decoded field, kept).
+    basis: (z 32x32, x 8x8, y 8x8) rows, default the DCT-II."""
+    if basis is None:
+        dz, dx, dy = _dct_matrix(32), _dct_matrix(8), _dct_matrix(8)
+    else:
+        dz, dx, dy = basis
     br, padded = _bricks(np.asarray(f, np.float64), b)
-    c = np.einsum("ui,vj,wk,nijk->nuvw", d8, d8, d32, br)
+    c = np.einsum("ui,vj,wk,nijk->nuvw", dx, dy, dz, br)
     c32 = c.astype(np.float32)
     bits = c32.view(np.uint32) & 0x7FFFFFFF
     K = max(1, f.shape[0] * f.shape[1] * (f.shape[2] - 2 * b) // keep_div)
@@ -67,7 +71,7 @@ def coder(f, b, keep_div):
     safe = np.where(scale > 0, scale, np.float32(1.0))[:, None, None, None]
     q = (c32 / safe).astype(np.float16).astype(np.float64)
     q = np.where(keep, q * scale[:, None, None, None].astype(np.float64), 0.0)
-    rec = np.einsum("ui,vj,wk,nuvw->nijk", d8, d8, d32, q)
+    rec = np.einsum("ui,vj,wk,nuvw->nijk", dx, dy, dz, q)
     return _unbricks(rec, f.shape, padded, b), int(keep.sum())
 
 
@@ -109,6 +113,50 @@ def test_coder_matches_restatement(gpu, shape, b, keep_div):
     print("keep 1/%d: kept %d, field rel-L2 %.2e" % (keep_div, kept, err))
     if keep_div == 1:
         assert err < 2e-3
+
+
+def _random_basis(seed):
+    """A random orthonormal separable basis, packed (z, x, y rows)."""
+    rng = np.random.default_rng(seed)
+    qs = [np.linalg.qr(rng.standard_normal((n, n)))[0] for n in (32, 8, 8)]
+    return np.concatenate([q.ravel() for q in qs]).astype(np.float32)
+
+
+@pytest.mark.parametrize("keep_div", [1, 16])
+def test_coder_with_caller_basis_matches_restatement(gpu, keep_div):
+    """A caller's basis (random orthonormal rows, nothing like a cosine)
+    replaces the DCT in both transforms: the coder agrees with the
+    restatement using the same matrices, and with every coefficient kept
+    only FP16 quantisation remains."""
+    from rtm3d_demo_amd import basis as bb
+    shape, b = (48, 48, 96), 32 if keep_div == 1 else 11
+    f = _wavefield(shape, 4)
+    rb = _random_basis(11)
+    d = gpu.DeviceArray.from_host(f)
+    try:
+        got, kept = gpu.dct_roundtrip(d, "f32", b, keep_div, basis=rb)
+        dflt, _ = gpu.dct_roundtrip(d, "f32", b, keep_div)
+    finally:
+        d.free()
+    want, kept_np = coder(f, b, keep_div, bb.unpack(rb))
+    inside = np.s_[:, :, b:shape[2] - b]
+    assert abs(kept - kept_np) <= max(2, kept_np // 10000), (kept, kept_np)
+    assert _rel(got[inside], want[inside]) < 2e-4
+    # the default basis is not what ran
+    assert _rel(got[inside], dflt[inside]) > 1e-3 or keep_div == 1
+    if keep_div == 1:
+        assert _rel(got[inside], f[inside]) < 2e-3
+
+
+def test_coder_rejects_a_basis_that_is_not_orthonormal(gpu):
+    rb = _random_basis(12)
+    rb[32 * 32 + 3] += 0.01          # one x row entry
+    d = gpu.DeviceArray.from_host(_wavefield((40, 36, 90), 1))
+    try:
+        with pytest.raises(Exception, match="orthonormal"):
+            gpu.dct_roundtrip(d, "f32", 11, 16, basis=rb)
+    finally:
+        d.free()
 
 
 def test_coder_subnormal_bricks(gpu):
@@ -180,16 +228,18 @@ def _inputs(seed=7):
     return V, coefs, g, ricker, rcv
 
 
-def _run(gpu, wfmt, k, keep_div, splits_f, splits_b, coded=True):
+def _run(gpu, wfmt, k, keep_div, splits_f, splits_b, coded=True, basis=None, src=SRC,
+         snaps_out=None):
     V, coefs, g, ricker, rcv = _inputs()
     wd = gpu.WAVE_FORMATS[wfmt][1]
-    shot = gpu.make_shot(*GRID, B, 8, NT, NT, tuple(c + B for c in SRC), gpu.RcvGeom(*RCV), wfmt)
+    shot = gpu.make_shot(*GRID, B, 8, NT, NT, tuple(c + B for c in src), gpu.RcvGeom(*RCV), wfmt)
     dV, dg, dsrc, drcv = (gpu.DeviceArray.from_host(a) for a in (V, g, ricker, rcv))
     S, R = gpu.FieldPair(GRID, wd), gpu.FieldPair(GRID, wd)
     for f in (S.a, S.b, R.a, R.b):
         f.zero()
     img = gpu.DeviceArray(GRID).zero()
-    store = gpu.DctSnapshots(shot, k, keep_div) if coded else gpu.snap_buffer(shot, "f32", k)
+    store = (gpu.DctSnapshots(shot, k, keep_div, basis) if coded
+             else gpu.snap_buffer(shot, "f32", k))
     try:
         it0 = 0
         for n in splits_f:
@@ -198,6 +248,8 @@ def _run(gpu, wfmt, k, keep_div, splits_f, splits_b, coded=True):
             else:
                 gpu.snap_forward_steps(shot, dV, coefs, dg, dsrc, S, store, "f32", k, it0, n)
             it0 += n
+        if snaps_out is not None:
+            snaps_out.append(store.download())
         hi = NT - 1
         for n in splits_b:
             if coded:
@@ -225,3 +277,38 @@ def test_coded_snapshot_rtm_split_equals_whole_and_tracks_exact(gpu, wfmt):
     # the 16-bit runs carry their wavefield storage error as well (the
     # yardstick propagates in FP32); measured 6.9e-3 / 8.4e-3 / 2.2e-2
     assert err < {"f32": 1e-2, "f16": 2e-2, "bf16": 6e-2}[wfmt]
+
+
+def test_learned_basis_snapshot_rtm(gpu):
+    """A basis learned from another shot's FP32 snapshots (the principal
+    directions of their brick lines) codes this shot's snapshots: the image
+    stays within the DCT's bound of the exact-snapshot image, and the learned
+    basis compacts this shot's snapshot energy into the leading z
+    coefficients at least as well as the DCT (the property it is trained
+    for, measured on unseen data)."""
+    from rtm3d_demo_amd import basis as bb
+    other = []
+    _run(gpu, "f32", 2, 8, (NT,), (1,), coded=False, src=(8, 30, 6), snaps_out=other)
+    learner = bb.BasisLearner()
+    for snap in other[0][NT // 4:]:
+        learner.add(snap, B)
+    klt = learner.basis()
+    mine = []
+    exact = _run(gpu, "f32", 2, 8, (NT,), (NT,), coded=False, snaps_out=mine)
+    test = bb.BasisLearner()
+    for snap in mine[0][NT // 4:]:
+        test.add(snap, B)
+    cz = test.cz
+    dz, kz = bb.unpack(bb.dct_basis())[0], bb.unpack(klt)[0]
+    e_dct = np.cumsum(np.einsum("ui,ij,uj->u", dz, cz, dz))
+    e_klt = np.cumsum(np.einsum("ui,ij,uj->u", kz, cz, kz))
+    print("leading-8 z energy on the unseen shot: DCT %.4f, learned %.4f"
+          % (e_dct[7] / e_dct[-1], e_klt[7] / e_klt[-1]))
+    assert e_klt[7] >= 0.98 * e_dct[7]
+    errs = {}
+    for name, basis in (("dct", None), ("learned", klt)):
+        img = _run(gpu, "f32", 2, 8, (NT,), (NT,), basis=basis)
+        errs[name] = _rel(img[:, :, B:-B], exact[:, :, B:-B])
+    print("keep 1/8 image rel-L2 vs exact: DCT %.2e, learned %.2e" % (errs["dct"],
+                                                                       errs["learned"]))
+    assert errs["learned"] < 1e-2
