@@ -463,13 +463,15 @@ int rtmhip_get_tiling(int kernel_class, int *tz, int *by, int *ry, int *xchunk);
  * (tapered on load; a tapered kernel holds more registers), mode 0 runs them
  * as one in-place pass over both fields' upper slabs followed by the
  * untapered stencil, mode -1 restores the default (fused for F32, separate
- * for F16 and BF16).  Mode 2 also fuses the every-border tapers of
- * rtmhip_model_steps (taper3D of both fields before each step,
- * RTMFiniteDifferencesKernel.cpp:180-181; stencil order 8), which otherwise
- * run as one in-place pass (the default: on the 592x592x336 salt grid the
- * fused step is no faster).  Results are identical in every mode.  Batches
- * and fused = 2 always fuse the upper tapers.  rtmhip_get_taper_fusion
- * returns the mode (0, 1 or 2). */
+ * for F16 and BF16).  The every-border tapers of rtmhip_model_steps
+ * (taper3D of both fields before each step,
+ * RTMFiniteDifferencesKernel.cpp:180-181; stencil order 8) are applied to
+ * each step's output by default and with mode 3 (the step stores the tapered
+ * value its successor reads; used when the source and receivers lie off the
+ * borders, else the separate pass runs), fused on load with mode 2, and run
+ * as one in-place pass with modes 0 and 1.  Results are identical in every
+ * mode.  Batches and fused = 2 always fuse the upper tapers.
+ * rtmhip_get_taper_fusion returns the last mode set (-1 for the default). */
 int rtmhip_set_taper_fusion(int wave_format, int mode);
 int rtmhip_get_taper_fusion(int wave_format, int *fused);
 

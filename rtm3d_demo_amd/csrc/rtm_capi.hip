@@ -54,7 +54,8 @@ struct State {
     // in place by a separate pass (true) or fused into the stencil (false);
     // the modeling loop's every-border tapers fused (true) or separate
     bool taper_sep[3] = {false, true, true};
-    bool taper_all_fused[3] = {false, false, false};
+    int taper_all_mode[3] = {2, 2, 2};  // modeling taper3D: 0 separate, 1 on load, 2 on output
+    int taper_mode[3] = {-1, -1, -1};   // the last rtmhip_set_taper_fusion mode
 };
 constexpr bool kTaperSepDefault[3] = {false, true, true};
 
@@ -527,27 +528,64 @@ int rtmhip_stack(float *dst, const float *src, size_t count)
 
 namespace {
 
+// The source cell and every receiver cell lie at least blen from each face.
+bool off_borders(const rtmhip_shot *shot, const ShotCtx &c)
+{
+    const int b = shot->blen;
+    auto in = [&](long long v, int n) { return v >= b && v < n - b; };
+    if (!in(shot->sxe, c.g.nx) || !in(shot->sye, c.g.ny) || !in(shot->sze, c.g.nz)) return false;
+    const rtmhip_rcv_geom &r = shot->rcv;
+    if (r.count_x <= 0 || r.count_y <= 0) return true;
+    return in(c.rz, c.g.nz) && in(c.rx0, c.g.nx) && in(c.ry0, c.g.ny) &&
+           in(c.rx0 + (long long)(r.count_x - 1) * r.dist_x, c.g.nx) &&
+           in(c.ry0 + (long long)(r.count_y - 1) * r.dist_y, c.g.ny);
+}
+
 template <typename T>
 int model_loop(const rtmhip_shot *shot, const ShotCtx &c, const float *V2DT2, const float *taper,
                const float *ricker, T **P, T **PP, float *rcv_chunk, int it0, int nsteps)
 {
     hipStream_t s = g_state.stream;
-    // With the every-border fusion (rtmhip_set_taper_fusion mode 2, order 8)
-    // the two in-place taper3D passes of each step are fused into the
-    // stencil as for the HBC upper taper: P tapered on load, the centre PP
-    // `pp_taps` times, PP's frame (the cells the step does not update)
-    // tapered in place before the step; the taper still pending on P after
-    // the last step is applied in place, so the state between calls is the
-    // reference's.  Off by default: on the salt grid the fused step
-    // (one-z waves with a taper chain on every value) is no faster than the
-    // separate pass plus the two-z untapered step.
+    // taper3D of both fields before each step (order 8) runs in one of three
+    // forms, with identical results.  By default (and with
+    // rtmhip_set_taper_fusion mode 3) it is applied to each step's output:
+    // the step stores taper3D of its new value (the next step's P, so P is
+    // read as stored) and tapers the centre PP once; PP's x-y frame is
+    // tapered twice before the step (it is this step's PP and the next
+    // step's P), the z frame by the stencil, and the front-right edge is
+    // written after the step from the stored (b, y, z).  The first step of a
+    // call tapers P in place, the last tapers PP in place and runs the plain
+    // step, so the state between calls is the reference's.  The source and
+    // the receivers must lie off the borders, where taper3D leaves values
+    // unchanged, since they act on the stored (tapered) field; otherwise the
+    // separate pass runs.  Salt grid, 300 steps: FP32 0.140 s against 0.187 s
+    // for the separate pass, FP16 0.120 against 0.128.
+    // Mode 2 fuses it on load instead (P tapered on load, the centre PP
+    // `pp_taps` times, PP's frame tapered in place before the step, the taper
+    // still pending on P applied after the last step): a taper chain on every
+    // loaded value, 0.183 s.  Modes 0 and 1 run the separate in-place pass.
     const int b = shot->blen;
-    const bool fused = g_state.taper_all_fused[shot->wave_format] && c.radius == 4 && b > 0;
+    const int amode = (c.radius == 4 && b > 0) ? g_state.taper_all_mode[shot->wave_format] : 0;
+    const bool fused = amode == 1;
+    const bool out = amode == 2 && off_borders(shot, c);
     for (int it = it0; it < it0 + nsteps; ++it) {
         T *t = *P;
         *P = *PP;
         *PP = t;
-        if (fused) {
+        const bool last = it == it0 + nsteps - 1;
+        if (out && !last) {
+            if (it == it0) HIPCHK(launch_taper(*P, taper, c.g, b, false, s), "taper");
+            const TaperFuse tf{taper, b, 1, 2};
+            HIPCHK(launch_taper_frame_xy(*PP, taper, c.g, b, c.radius, 2, s), "frame taper");
+            HIPCHK(stencil(c.radius, *P, *PP, V2DT2, c.cf, c.g, nullptr, &tf), "stencil");
+            HIPCHK(launch_taper_quirk_out(*PP, taper, c.g, b, s), "front-right taper");
+        } else if (out) {
+            if (it == it0)
+                HIPCHK(launch_taper_all2(*P, *PP, taper, c.g, b, s), "taper");
+            else
+                HIPCHK(launch_taper(*PP, taper, c.g, b, false, s), "taper");
+            HIPCHK(stencil(c.radius, *P, *PP, V2DT2, c.cf, c.g, nullptr), "stencil");
+        } else if (fused) {
             const TaperFuse tf{taper, b, it == it0 ? 1 : 2, 1};
             HIPCHK(launch_taper_frame_all(*PP, taper, c.g, b, c.radius, tf.pp_taps, s),
                    "frame taper");
@@ -1210,12 +1248,13 @@ int rtmhip_set_taper_fusion(int wave_format, int mode)
     NEED_INIT();
     if (wave_format < RTMHIP_WAVE_F32 || wave_format > RTMHIP_WAVE_BF16)
         return fail(RTMHIP_ERR_INVALID, "unknown wave_format %d", wave_format);
-    if (mode < -1 || mode > 2)
+    if (mode < -1 || mode > 3)
         return fail(RTMHIP_ERR_INVALID,
-                    "taper fusion mode %d (2 every taper fused, 1 upper tapers fused, "
-                    "0 separate, -1 default)", mode);
+                    "taper fusion mode %d (3 every taper fused on output, 2 every taper fused "
+                    "on load, 1 upper tapers fused, 0 separate, -1 default)", mode);
+    g_state.taper_mode[wave_format] = mode;
     g_state.taper_sep[wave_format] = mode < 0 ? kTaperSepDefault[wave_format] : mode == 0;
-    g_state.taper_all_fused[wave_format] = mode == 2;
+    g_state.taper_all_mode[wave_format] = (mode < 0 || mode == 3) ? 2 : (mode == 2 ? 1 : 0);
     return RTMHIP_OK;
 }
 
@@ -1224,7 +1263,7 @@ int rtmhip_get_taper_fusion(int wave_format, int *fused)
     NEED_INIT();
     if (wave_format < RTMHIP_WAVE_F32 || wave_format > RTMHIP_WAVE_BF16 || !fused)
         return fail(RTMHIP_ERR_INVALID, "bad arguments");
-    *fused = g_state.taper_all_fused[wave_format] ? 2 : (g_state.taper_sep[wave_format] ? 0 : 1);
+    *fused = g_state.taper_mode[wave_format];
     return RTMHIP_OK;
 }
 

@@ -70,7 +70,8 @@ struct TaperFuse {
     const float *t;  // device table g[b]
     int b;
     int pp_taps;     // 1 or 2
-    int all;         // 0: the upper taper (taperUpper3D); 1: every border (taper3D)
+    int all;         // 0: the upper taper (taperUpper3D); 1: every border (taper3D) on
+                     // load; 2: every border on the step's output (P stored tapered)
 };
 
 // Tile configuration of the stencil kernel.
@@ -182,6 +183,15 @@ hipError_t launch_taper_frame(T *P, const float *taper, const Grid &g, int b, in
 template <typename T>
 hipError_t launch_taper_frame_all(T *P, const float *taper, const Grid &g, int b, int R, int taps,
                                   hipStream_t s);
+// taper3D on a step's output (TaperFuse all 2): the cells with x or y within
+// R of a face, `taps` times in place, front-right edge cells excepted
+// (launch_taper_frame_xy, before the step); then the front-right edge cells
+// from the stored (b, y, z) (launch_taper_quirk_out, after the step).
+template <typename T>
+hipError_t launch_taper_frame_xy(T *P, const float *taper, const Grid &g, int b, int R, int taps,
+                                 hipStream_t s);
+template <typename T>
+hipError_t launch_taper_quirk_out(T *P, const float *taper, const Grid &g, int b, hipStream_t s);
 template <typename T>
 hipError_t launch_source(T *PP, long long offset, const float *table, int it, float value,
                          hipStream_t s);

@@ -425,6 +425,31 @@ __global__ void __launch_bounds__(kThreads)
     P[o] = wst<T>(all_taper<T>(wld(P[o]), t, g, b, x, y, z, taps));
 }
 
+// The front-right edge of taper3D applied to a step's output (the step
+// fused with TAPER 3 stored every other cell tapered): cells x < b, y >=
+// ny-b, z in [b, nz-b) take the face-tapered value of (b, y, z), which the
+// step stored as (v*gy) rounded, then ((.)*gx)*gy.  Runs after the step and
+// after the frame pass.
+template <typename T>
+__global__ void __launch_bounds__(kThreads)
+    k_taper_quirk_out(T *__restrict__ P, const float *__restrict__ t, Grid g, int b, FastDiv dlz,
+                      FastDiv db, int n)
+{
+    const int i = (int)(blockIdx.x * kThreads + threadIdx.x);  // n < 2^31 (launcher)
+    if (i >= n) return;
+    const int lz = g.nz - 2 * b;
+    const int r = fdiv(i, dlz);
+    const int z = b + (i - r * lz);
+    const int x = fdiv(r, db);
+    const int iy = r - x * b;
+    const int y = g.ny - b + iy;
+    const float gy = t[iy];
+    float v = wld(P[goff(g, b, y, z)]);
+    v *= t[b - 1 - x];
+    v *= gy;
+    P[goff(g, x, y, z)] = wst<T>(v);
+}
+
 // The front-right edge cells of that frame (x < R, or y >= ny-R), from the
 // cell (b, y, z) before this step's tapers: the face pass's (v*gy) rounded,
 // `taps` times, then (v*gx)*gy.  Runs before k_taper_frame_all, which
@@ -1115,6 +1140,33 @@ hipError_t launch_taper_frame_all(T *P, const float *taper, const Grid &g, int b
 }
 
 template <typename T>
+hipError_t launch_taper_frame_xy(T *P, const float *taper, const Grid &g, int b, int R, int taps,
+                                 hipStream_t s)
+{
+    if (b <= 0 || taps <= 0) return hipSuccess;
+    if (b < R || 2 * b >= g.nx || 2 * b >= g.ny || 2 * b >= g.nz) return hipErrorInvalidValue;
+    const long long nX = (long long)2 * R * g.ny * g.nz;
+    const long long n = nX + (long long)(g.nx - 2 * R) * 2 * R * g.nz;
+    if (n + kThreads >= (1LL << 31)) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_taper_frame_all<T>, dim3(nblocks(n)), dim3(kThreads), 0, s, P, taper, g,
+                       b, R, taps, (int)nX, (int)n);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_taper_quirk_out(T *P, const float *taper, const Grid &g, int b, hipStream_t s)
+{
+    if (b <= 0) return hipSuccess;
+    if (2 * b >= g.nx || 2 * b >= g.ny || 2 * b >= g.nz) return hipErrorInvalidValue;
+    const long long lz = g.nz - 2 * b;
+    const long long n = (long long)b * b * lz;
+    if (n + kThreads >= (1LL << 31)) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_taper_quirk_out<T>, dim3(nblocks(n)), dim3(kThreads), 0, s, P, taper, g,
+                       b, fast_div((uint32_t)lz), fast_div((uint32_t)b), (int)n);
+    return hipGetLastError();
+}
+
+template <typename T>
 hipError_t launch_source(T *PP, long long offset, const float *table, int it, float value,
                          hipStream_t s)
 {
@@ -1295,6 +1347,10 @@ hipError_t launch_tick(int *p, int d, bool set, hipStream_t s)
                                               hipStream_t);                                     \
     template hipError_t launch_taper_frame_all<T>(T *, const float *, const Grid &, int, int,   \
                                                   int, hipStream_t);                            \
+    template hipError_t launch_taper_frame_xy<T>(T *, const float *, const Grid &, int, int,    \
+                                                 int, hipStream_t);                             \
+    template hipError_t launch_taper_quirk_out<T>(T *, const float *, const Grid &, int,        \
+                                                  hipStream_t);                                 \
     template hipError_t launch_source<T>(T *, long long, const float *, int, float, hipStream_t); \
     template hipError_t launch_receivers<T>(T *, float *, const Grid &, int, int, int, int, int, \
                                             int, int, int, int, bool, hipStream_t);             \

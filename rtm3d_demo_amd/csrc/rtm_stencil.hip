@@ -659,6 +659,11 @@ struct WaveGrid {
 // b (a uniform byte offset on their lane offset) and apply
 // ((v*fy) rounded *fx)*fy.  With TAPER 2 the wave also stores the tapered
 // PP of its z-frame lanes (z < R, z >= nz-R), which the step does not update.
+// TAPER 3 applies taper3D to the step's output instead: P is read as stored
+// (the previous step stored it tapered), the centre PP is tapered once and
+// the new value is tapered before it is stored; the front-right cells and
+// the frame are written by separate kernels, so this form needs no plane-b
+// loads (the pointwise chain only).
 template <typename T, int R, int RY, bool IMG, int TAPER>
 __device__ __forceinline__ void wave_tile(const T *__restrict__ P, T *__restrict__ PP,
                                           const float *__restrict__ V, const Coefs &cf,
@@ -686,14 +691,17 @@ __device__ __forceinline__ void wave_tile(const T *__restrict__ P, T *__restrict
         return j >= 0 ? tab[max(j, 0)] : 1.0f;
     };
     constexpr bool ALL = TAPER == 2;
-    // the wave holds a tapered value: some z < b (halo included), or any (ALL)
-    const bool wtap = TAPER && (ALL || z0 - R < tf.b);
+    constexpr bool OUT = TAPER == 3;
+    constexpr bool EVERY = ALL || OUT;   // every border (taper3D)
+    constexpr bool LOADTAP = TAPER == 1 || TAPER == 2;  // P tapered on load
+    // the wave holds a tapered value: some z < b (halo included), or any (EVERY)
+    const bool wtap = TAPER && (EVERY || z0 - R < tf.b);
     const int zq = (int)zc;
-    const bool ztap = wtap && (ALL ? (zq < tf.b || zq >= g.nz - tf.b) : (z < tf.b));
+    const bool ztap = wtap && (EVERY ? (zq < tf.b || zq >= g.nz - tf.b) : (z < tf.b));
     float fzl = 1.0f;
-    if (ztap) fzl = ALL ? lat(zq, g.nz) : tab[tf.b - 1 - z];
+    if (ztap) fzl = EVERY ? lat(zq, g.nz) : tab[tf.b - 1 - z];
     const bool zmid = ALL && !ztap;  // front-right lanes (on the rows and planes concerned)
-    const bool zframe = ALL && z < g.nz && !zcomp;
+    const bool zframe = EVERY && z < g.nz && !zcomp;
 
     // own rows (uniform)
     unsigned rowb[RY];
@@ -724,7 +732,7 @@ __device__ __forceinline__ void wave_tile(const T *__restrict__ P, T *__restrict
         yhb[h] = (unsigned)gy * (unsigned)g.sy * ES;
         if (ALL && gy >= g.ny - tf.b) qhmask |= 1u << h;
         yhf[h] = 1.0f;
-        if (wtap) yhf[h] = __builtin_bit_cast(
+        if (LOADTAP && wtap) yhf[h] = __builtin_bit_cast(
             float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, lat(gy, g.ny))));
     }
     // this lane's z-halo cell: row zr, column zcol of the LDS row
@@ -735,7 +743,8 @@ __device__ __forceinline__ void wave_tile(const T *__restrict__ P, T *__restrict
     const int zcol = zk < R ? zk : 64 + zk;
     const int zhy = min(y0 + zr, g.ny - 1);
     const unsigned zhoff = ((unsigned)zhy * (unsigned)g.sy + (unsigned)zhzc) * ES;
-    const bool zhtap = wtap && (ALL ? (zhzc < tf.b || zhzc >= g.nz - tf.b) : zhzc < tf.b);
+    const bool zhtap =
+        LOADTAP && wtap && (ALL ? (zhzc < tf.b || zhzc >= g.nz - tf.b) : zhzc < tf.b);
     float zhfy = 1.0f, zhfz = 1.0f;
     if (zhtap || (ALL && wtap)) zhfy = lat(zhy, g.ny);
     if (zhtap) zhfz = ALL ? lat(zhzc, g.nz) : tab[tf.b - 1 - zhzc];
@@ -758,7 +767,7 @@ __device__ __forceinline__ void wave_tile(const T *__restrict__ P, T *__restrict
             a = fy;
             c = fx;
             d = fz;
-        } else if (ALL) {
+        } else if (EVERY) {
             a = q ? fy : fx;
             c = q ? fx : fy;
             d = q ? fy : 1.0f;
@@ -787,7 +796,7 @@ __device__ __forceinline__ void wave_tile(const T *__restrict__ P, T *__restrict
         }
     };
     auto taper_ring = [&](int x, float(&dst)[RY]) {
-        if (TAPER && wtap) {
+        if (LOADTAP && wtap) {
             const float fx = plane_fx(x);
             const bool qx = ALL && x < tf.b;
 #pragma unroll
@@ -811,7 +820,7 @@ __device__ __forceinline__ void wave_tile(const T *__restrict__ P, T *__restrict
         }
     };
     auto taper_halo = [&](int x, int set) {
-        if (TAPER && wtap) {
+        if (LOADTAP && wtap) {
             const float fx = plane_fx(x);
             const bool qx = ALL && x < tf.b;
 #pragma unroll
@@ -900,7 +909,9 @@ __device__ __forceinline__ void wave_tile(const T *__restrict__ P, T *__restrict
         for (int r = 0; r < RY; ++r) {
             const float p = q[ctr][r];
             float pp = c_pp[r];
-            if (TAPER && wtap) {
+            if (OUT) {
+                pp = tapv(pp, fxc, fyr[r], ztap, fzl, false);
+            } else if (TAPER && wtap) {
                 const bool qk = qxc && ((qmask >> r) & 1u) && zmid;
                 // a front-right PP after two passes: plane b's face value
                 // of the first pass, then the second pass's chain
@@ -909,9 +920,15 @@ __device__ __forceinline__ void wave_tile(const T *__restrict__ P, T *__restrict
                 if (tf.pp_taps > 1 && !qk) pp = tap(pp, fxc, fyr[r]);
             }
             const float npp = 2.f * p - pp + c_v[r] * lapl[r];
-            if (ALL && zframe && ((ymask >> r) & 1u)) st_buf<T>(rpp, rowb[r], zoff, pp);
+            // z frame (not updated): PP tapered, and with OUT once more as
+            // the next step's P
+            if (EVERY && zframe && ((ymask >> r) & 1u))
+                st_buf<T>(rpp, rowb[r], zoff,
+                          OUT ? tapv(pp, fxc, fyr[r], ztap, fzl, false) : pp);
             if (((ymask >> r) & 1u) && zcomp) {
-                st_buf<T>(rpp, rowb[r], zoff, npp);
+                // OUT: the value the step stores (rounded), then taper3D
+                st_buf<T>(rpp, rowb[r], zoff,
+                          OUT ? tapv(wrnd<T>(npp), fxc, fyr[r], ztap, fzl, false) : npp);
                 bad |= (npp != npp);
                 if (IMG && (z >= fz.z0) && (z < fz.z1) &&
                     !(rcvx && ((rmask >> r) & 1u) && z == fz.rz))
@@ -1028,7 +1045,11 @@ constexpr int wave2_min_occupancy()
     return IMG ? 2 : 3;
 }
 
-template <typename T, int R, int RY, bool IMG, bool TAPER>
+// TAPER 1: the upper taper fused on load; TAPER 3: taper3D applied to the
+// output, as in wave_tile (pointwise chains on the centre PP, the new value
+// and the z frame; the frame's x-y part and the front-right edge are written
+// by separate kernels).
+template <typename T, int R, int RY, bool IMG, int TAPER>
 __device__ __forceinline__ void wave_tile2(const T *__restrict__ P, T *__restrict__ PP,
                                            const float *__restrict__ V, const Coefs &cf,
                                            const Grid &g, int z0, int yt, int xb, int xe,
@@ -1056,15 +1077,19 @@ __device__ __forceinline__ void wave_tile2(const T *__restrict__ P, T *__restric
         const int j = i < tf.b ? tf.b - 1 - i : (i >= n - tf.b ? i - (n - tf.b) : -1);
         return j >= 0 ? tab[max(j, 0)] : 1.0f;
     };
-    const bool wtap = TAPER && (z0 - R < tf.b);
+    constexpr bool OUT = TAPER == 3;
+    constexpr bool LOADTAP = TAPER == 1;
+    const bool wtap = TAPER && (OUT || z0 - R < tf.b);
     bool ztap[2];
     float fzl[2];
 #pragma unroll
     for (int a = 0; a < 2; ++a) {
-        ztap[a] = wtap && (z + a < tf.b);
+        const int za = min(z + a, g.nz - 1);
+        ztap[a] = wtap && (OUT ? (za < tf.b || za >= g.nz - tf.b) : (z + a < tf.b));
         fzl[a] = 1.0f;
-        if (ztap[a]) fzl[a] = tab[tf.b - 1 - (z + a)];
+        if (ztap[a]) fzl[a] = OUT ? lat(za, g.nz) : tab[tf.b - 1 - (z + a)];
     }
+    const bool zframe = OUT && z < g.nz && !zcomp;
 
     unsigned rowb[RY];
     unsigned ymask = 0, rmask = 0;
@@ -1090,7 +1115,7 @@ __device__ __forceinline__ void wave_tile2(const T *__restrict__ P, T *__restric
         const int gy = min(h < R ? y0 - R + h : y0 + RY + (h - R), g.ny - 1);
         yhb[h] = (unsigned)gy * (unsigned)g.sy * ES;
         yhf[h] = 1.0f;
-        if (wtap) yhf[h] = __builtin_bit_cast(
+        if (LOADTAP && wtap) yhf[h] = __builtin_bit_cast(
             float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, lat(gy, g.ny))));
     }
     // this lane's z-halo cell (one 16- or 32-bit load): row zr, LDS column zcol
@@ -1101,7 +1126,7 @@ __device__ __forceinline__ void wave_tile2(const T *__restrict__ P, T *__restric
     const int zcol = zk < R ? zk : 128 + zk;
     const int zhy = min(y0 + zr, g.ny - 1);
     const unsigned zhoff = ((unsigned)zhy * (unsigned)g.sy + (unsigned)zhzc) * ES;
-    const bool zhtap = wtap && zhzc < tf.b;
+    const bool zhtap = LOADTAP && wtap && zhzc < tf.b;
     float zhfy = 1.0f, zhfz = 1.0f;
     if (zhtap) {
         zhfy = lat(zhy, g.ny);
@@ -1119,13 +1144,26 @@ __device__ __forceinline__ void wave_tile2(const T *__restrict__ P, T *__restric
     auto tap = [&](float v, int a, float fx, float fy) -> float {
         return wrnd<T>(taper1(v, ztap[a] ? fx : 1.0f, ztap[a] ? fy : 1.0f, fzl[a]));
     };
+    // taper3D at a value (OUT): ((v*fy)*fx)*fz on the z border, (v*fx)*fy off
+    // it, factors exactly 1 off their border
+    auto tapo = [&](float v, int a, float fx, float fy) -> float {
+        if (ztap[a]) {
+            v *= fy;
+            v *= fx;
+            v *= fzl[a];
+        } else {
+            v *= fx;
+            v *= fy;
+        }
+        return wrnd<T>(v);
+    };
     auto load_ring = [&](int x, float(&dst)[RY][2]) {
         const auto rs = plane_rsrc(P + (size_t)x * (size_t)g.sx);
 #pragma unroll
         for (int r = 0; r < RY; ++r) ld_buf2<T>(rs, rowb[r], zoff, dst[r]);
     };
     auto taper_ring = [&](int x, float(&dst)[RY][2]) {
-        if (TAPER && wtap) {
+        if (LOADTAP && wtap) {
             const float fx = plane_fx(x);
 #pragma unroll
             for (int r = 0; r < RY; ++r)
@@ -1140,7 +1178,7 @@ __device__ __forceinline__ void wave_tile2(const T *__restrict__ P, T *__restric
         zh[set] = ld_buf<T>(rs, 0u, zhoff);
     };
     auto taper_halo = [&](int x, int set) {
-        if (TAPER && wtap) {
+        if (LOADTAP && wtap) {
             const float fx = plane_fx(x);
 #pragma unroll
             for (int h = 0; h < 2 * R; ++h)
@@ -1239,18 +1277,29 @@ __device__ __forceinline__ void wave_tile2(const T *__restrict__ P, T *__restric
         }
 #pragma unroll
         for (int r = 0; r < RY; ++r) {
-            float npp[2];
+            float npp[2], ppt[2];
 #pragma unroll
             for (int a = 0; a < 2; ++a) {
                 float pp = c_pp[r][a];
-                if (TAPER && wtap) {
+                if (OUT) {
+                    pp = tapo(pp, a, fxc, fyr[r]);
+                } else if (TAPER && wtap) {
                     pp = tap(pp, a, fxc, fyr[r]);
                     if (tf.pp_taps > 1) pp = tap(pp, a, fxc, fyr[r]);
                 }
+                ppt[a] = pp;
                 npp[a] = 2.f * q[ctr][r][a] - pp + c_v[r][a] * lapl[r][a];
             }
+            // z frame (not updated): PP tapered once more as the next step's P
+            if (OUT && zframe && ((ymask >> r) & 1u))
+                st_buf2<T>(rpp, rowb[r], zoff, tapo(ppt[0], 0, fxc, fyr[r]),
+                           tapo(ppt[1], 1, fxc, fyr[r]));
             if (((ymask >> r) & 1u) && zcomp) {
-                st_buf2<T>(rpp, rowb[r], zoff, npp[0], npp[1]);
+                if (OUT)
+                    st_buf2<T>(rpp, rowb[r], zoff, tapo(wrnd<T>(npp[0]), 0, fxc, fyr[r]),
+                               tapo(wrnd<T>(npp[1]), 1, fxc, fyr[r]));
+                else
+                    st_buf2<T>(rpp, rowb[r], zoff, npp[0], npp[1]);
                 bad |= (npp[0] != npp[0]) || (npp[1] != npp[1]);
                 if (IMG) {
                     bool im[2];
@@ -1279,7 +1328,7 @@ __device__ __forceinline__ void wave_tile2(const T *__restrict__ P, T *__restric
     if (bad) atomicOr(flag, 1);
 }
 
-template <typename T, int R, int RY, int WPB, bool IMG, bool TAPER>
+template <typename T, int R, int RY, int WPB, bool IMG, int TAPER>
 __global__ void __launch_bounds__(64 * WPB, (wave2_min_occupancy<IMG>()))
     k_stencil_wave2(const T *__restrict__ P, T *__restrict__ PP, const float *__restrict__ V,
                     const Coefs cf, const Grid g, const WaveGrid wg, const ImageFuse fz,
@@ -2221,7 +2270,7 @@ hipError_t launch_wave(const T *P, T *PP, const float *V, const Coefs &cf, const
 
 // Two-cells-per-lane waves, one launch over the z tiles [zt0, zt0 + nzt) of
 // 128 cells, tile t starting at z = zb + 128 t (zb even).
-template <typename T, int R, int RY, int WPB, bool IMG, bool TAPER>
+template <typename T, int R, int RY, int WPB, bool IMG, int TAPER>
 hipError_t launch_wave2_cfg(const T *P, T *PP, const float *V, const Coefs &cf, const Grid &g,
                             const ImageFuse &fz, const TaperFuse &tf, int *flag, hipStream_t s,
                             int xchunk, int zb, int zt0, int nzt)
@@ -2285,6 +2334,27 @@ hipError_t launch_wave2(const T *P, T *PP, const float *V, const Coefs &cf, cons
     });
 }
 
+// The step with taper3D applied to its output (TAPER 3, no imaging): two-z
+// 128 x 1 x 4 waves where cell pairs fit (even nz and strides), else one-z
+// 64 x 1 x RY waves, over every z tile.
+template <typename T, int R, int RY, int WPB>
+hipError_t launch_wave_out(const T *P, T *PP, const float *V, const Coefs &cf, const Grid &g,
+                           const ImageFuse *fuse, const TaperFuse *taper, int *flag,
+                           hipStream_t s, int xchunk)
+{
+    if (fuse || !taper || taper->b < R || taper->b > 8192) return hipErrorInvalidValue;
+    if (2 * taper->b >= g.nx || 2 * taper->b >= g.ny || 2 * taper->b >= g.nz)
+        return hipErrorInvalidValue;
+    ImageFuse fz;
+    TaperFuse tf;
+    fuse_args(nullptr, taper, fz, tf);
+    if (wave2_fits(P, PP, V, nullptr, g))
+        return launch_wave2_cfg<T, R, 4, WPB, false, 3>(P, PP, V, cf, g, fz, tf, flag, s, xchunk,
+                                                        0, 0, (g.nz + 127) / 128);
+    return launch_wave_cfg<T, R, RY, WPB, false, 3>(P, PP, V, cf, g, fz, tf, flag, s, xchunk, 0,
+                                                    (g.nz + 63) / 64);
+}
+
 // The 64-cell z tiles that the upper taper reaches (zt * 64 - R < b) on
 // one-z waves of RY rows with the taper fused, every deeper cell on
 // untapered two-z waves starting at the first untapered tile (even z).
@@ -2341,8 +2411,11 @@ hipError_t launch_radius(const T *P, T *PP, const float *V, const Coefs &cf, con
     const int xchunk = t ? t->xchunk : 0;
     if (taper && taper->all) {
         // taper3D fused: 64 x 1 x 8 one-z waves whatever the class tiling
-        if constexpr (R == 4)
+        if constexpr (R == 4) {
+            if (taper->all == 2)
+                return launch_wave_out<T, R, 8, 4>(P, PP, V, cf, g, fuse, taper, flag, s, xchunk);
             return launch_wave_all<T, R, 8, 4>(P, PP, V, cf, g, fuse, taper, flag, s, xchunk);
+        }
         return hipErrorNotSupported;
     }
     if constexpr (R == 4 && std::is_same<T, float>::value) {

@@ -450,8 +450,11 @@ def set_tiling(kernel_class, tz=0, by=0, ry=0, xchunk=0):
 
 def set_taper_fusion(fmt, mode):
     """mode 1: upper tapers fused into the stencil; 2: also the modeling
-    loop's every-border tapers; 0: separate in-place passes; -1: the
-    format's default."""
+    loop's every-border tapers, applied as values are loaded; 3: those
+    applied to each step's output instead (needs the source and receivers
+    off the borders, else the separate pass runs); 0: separate in-place
+    passes; -1: the format's default (upper tapers fused for F32, separate
+    for 16-bit formats; every-border tapers on the output)."""
     check(load().rtmhip_set_taper_fusion(WAVE_FORMATS[fmt][0], mode))
 
 

@@ -1,8 +1,15 @@
 """Modeling steps (rtmhip_model_steps) with taper3D of both fields fused into
-the stencil (taper fusion mode 2), or applied by the separate in-place pass
-(the default), against the reference's operator sequence on the oracle
-(swap, taper3D(P), taper3D(PP), step, source, save receivers), bit for bit,
-for every wavefield format.
+the stencil (taper fusion mode 3 and the default: applied to each step's
+output; mode 2: applied on load), or applied by the separate in-place pass
+(mode 0),
+against the reference's operator sequence on the oracle (swap, taper3D(P),
+taper3D(PP), step, source, save receivers), bit for bit, for every wavefield
+format.
+
+The output form needs the source and the receivers off the borders; the
+wide receiver line reaches the x border of the narrower grids, where the
+default falls back to the separate pass, and the inner line keeps the output
+form on every grid.  Even nz runs it on two-z waves, odd nz on one-z waves.
 
 The fields start non-zero everywhere, so the front-right edge of taper3D
 (cells x < blen, y >= ny-blen, z inside take the face-tapered value of
@@ -19,6 +26,7 @@ pytestmark = pytest.mark.gpu
 
 GRIDS = [(40, 44, 90), (32, 36, 340), (30, 34, 91)]
 RCV = (1, 2, 3, 3, 4, 4, 5)      # start x, y, depth, dist x, y, count x, y
+RCV_IN = (1, 2, 3, 2, 2, 3, 4)   # inside the borders of every grid below
 
 
 @pytest.fixture(params=[((0, 0, 0), 0), ((64, 1, 8), 3), ((128, 1, 4), 0)],
@@ -32,13 +40,15 @@ def tiling(request, gpu):
         gpu.set_tiling(cls, 0)
 
 
-@pytest.mark.parametrize("fusion", [2, -1], ids=["tfused", "tdefault"])
+@pytest.mark.parametrize("fusion, rcvg", [(3, RCV_IN), (-1, RCV), (2, RCV), (0, RCV)],
+                         ids=["tout-rin", "tdefault", "tload", "tsep"])
 @pytest.mark.parametrize("splits", [(5,), (2, 1, 2)])
 @pytest.mark.parametrize("b", [12, 11])
 @pytest.mark.parametrize("shape", GRIDS)
 @pytest.mark.parametrize("fmt", ["f32", "f16", "bf16"])
-def test_model_steps_match_operator_sequence(gpu, oracle, fmt, shape, b, splits, fusion, tiling):
-    print("model fusion", fmt, shape, b, splits, fusion, tiling, flush=True)
+def test_model_steps_match_operator_sequence(gpu, oracle, fmt, shape, b, splits, fusion, rcvg,
+                                             tiling):
+    print("model fusion", fmt, shape, b, splits, fusion, rcvg, tiling, flush=True)
     order, nt = 8, 5
     rng = np.random.default_rng(41)
     conv = (lambda a: a) if fmt == "f32" else (lambda a: oracle.to_format(a, fmt))
@@ -50,9 +60,9 @@ def test_model_steps_match_operator_sequence(gpu, oracle, fmt, shape, b, splits,
     coefs = oracle.stencil_coefs(order, 10.0, 11.0, 9.0)
     src = rng.uniform(-1, 1, nt).astype(np.float32)
     sxe = (shape[0] // 2, shape[1] // 2, b + 2)
-    ogeom = oracle.RcvGeom(*RCV)
+    ogeom = oracle.RcvGeom(*rcvg)
     P, PP = P0.copy(), PP0.copy()
-    rec = np.zeros((RCV[5], RCV[6], nt), np.float32)
+    rec = np.zeros((rcvg[5], rcvg[6], nt), np.float32)
     for it in range(nt):
         P, PP = PP, P
         oracle.taper(P, g, b, False, *ofmt)
@@ -67,7 +77,7 @@ def test_model_steps_match_operator_sequence(gpu, oracle, fmt, shape, b, splits,
     F.b.upload(PP0)
     V, dg, ds = (gpu.DeviceArray.from_host(a) for a in (v2, g, src))
     drec = gpu.DeviceArray(rec.shape).zero()
-    shot = gpu.make_shot(*shape, b, order, nt, nt, sxe, gpu.RcvGeom(*RCV), fmt)
+    shot = gpu.make_shot(*shape, b, order, nt, nt, sxe, gpu.RcvGeom(*rcvg), fmt)
     try:
         it0 = 0
         for n in splits:
