@@ -456,21 +456,25 @@ int rtmhip_ae_gemm_test(const void *A, long long sam, long long sak, int a_half,
 int rtmhip_set_tiling(int kernel_class, int tz, int by, int ry, int xchunk);
 int rtmhip_get_tiling(int kernel_class, int *tz, int *by, int *ry, int *xchunk);
 
-/* How the HBC upper tapers (taperUpper3D of both fields before each step,
- * RTMFiniteDifferencesKernel_HBC.cpp:160-165 and :311-316) reach the single-shot
+/* How the tapers reach the loops, per wave format; results are identical in
+ * every mode, only the speed differs.
+ * HBC upper tapers (taperUpper3D of both fields before each step,
+ * RTMFiniteDifferencesKernel_HBC.cpp:160-165 and :311-316) in the single-shot
  * loops (rtmhip_forward_steps, rtmhip_backward_steps with fused = 1,
- * rtmhip_snap_*) for one wave format: mode 1 fuses them into the stencil
- * (tapered on load; a tapered kernel holds more registers), mode 0 runs them
- * as one in-place pass over both fields' upper slabs followed by the
- * untapered stencil, mode -1 restores the default (fused for F32, separate
- * for F16 and BF16).  The every-border tapers of rtmhip_model_steps
- * (taper3D of both fields before each step,
- * RTMFiniteDifferencesKernel.cpp:180-181; stencil order 8) are applied to
- * each step's output by default and with mode 3 (the step stores the tapered
- * value its successor reads; used when the source and receivers lie off the
- * borders, else the separate pass runs), fused on load with mode 2, and run
- * as one in-place pass with modes 0 and 1.  Results are identical in every
- * mode.  Batches and fused = 2 always fuse the upper tapers.
+ * rtmhip_snap_*): mode 1 fuses them into the stencil (tapered on load);
+ * mode 0 runs them as one in-place pass over both fields' upper slabs
+ * followed by the untapered stencil; mode 4 applies the forward loops' tapers
+ * to each step's output (the step stores the tapered value its successor
+ * reads as P and tapers only the centre PP; used when the source lies at
+ * z >= blen) and fuses the backward loops' on load.
+ * Every-border tapers of rtmhip_model_steps (taper3D of both fields before
+ * each step, RTMFiniteDifferencesKernel.cpp:180-181; stencil order 8): on
+ * each step's output with modes 3 and 4 (used when the source and receivers
+ * lie off the borders, else the separate pass runs), fused on load with
+ * mode 2 (upper tapers as mode 1), one in-place pass with modes 0 and 1.
+ * Mode -1 restores the default: mode 4 for F32; for F16 and BF16 the upper
+ * tapers in separate passes and the every-border tapers on the output.
+ * Batches and fused = 2 always fuse the upper tapers on load.
  * rtmhip_get_taper_fusion returns the last mode set (-1 for the default). */
 int rtmhip_set_taper_fusion(int wave_format, int mode);
 int rtmhip_get_taper_fusion(int wave_format, int *fused);

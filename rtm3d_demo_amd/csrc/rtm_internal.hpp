@@ -70,8 +70,11 @@ struct TaperFuse {
     const float *t;  // device table g[b]
     int b;
     int pp_taps;     // 1 or 2
-    int all;         // 0: the upper taper (taperUpper3D); 1: every border (taper3D) on
-                     // load; 2: every border on the step's output (P stored tapered)
+    int all;         // 0: the upper taper (taperUpper3D) on load; 1: every border
+                     // (taper3D) on load; 2: every border on the step's output (P stored
+                     // tapered); 3: the upper taper on the step's output
+    int hold;        // all 3: planes z in [b - hold, b) stored untapered (the upper
+                     // border history is saved from them, then they are tapered)
 };
 
 // Tile configuration of the stencil kernel.
@@ -173,9 +176,10 @@ hipError_t launch_taper_all2(T *P, T *Q, const float *taper, const Grid &g, int 
                              hipStream_t s);
 // Upper taper (applied `taps` times) of the slab cells the stencil never
 // updates (z < R or the x / y frame of width R), in place.
+// With hold > 0 the planes z in [b - hold, b) get one application fewer.
 template <typename T>
 hipError_t launch_taper_frame(T *P, const float *taper, const Grid &g, int b, int R, int taps,
-                              hipStream_t s);
+                              hipStream_t s, int hold = 0);
 // taper3D's counterpart of launch_taper_frame (fused every-border taper):
 // the cells within R of any grid face, `taps` times in place, the front-right
 // edge's cells from the cell (b, y, z) as taper3D reads it.  Runs before the
@@ -206,6 +210,12 @@ hipError_t launch_receivers_image(T *PP, const float *rcv, const Grid &g, int rx
 template <typename T>
 hipError_t launch_upb(T *PP, T *upb, const Grid &g, int blen, int hf, long long lt, bool restore,
                       hipStream_t s);
+// The upper-border history saved from PP's planes [blen-hf, blen), then the
+// upper taper applied to those planes in place (a step that stores the upper
+// taper of its output left them untapered).
+template <typename T>
+hipError_t launch_upb_save_taper(T *PP, T *upb, const float *taper, const Grid &g, int blen,
+                                 int hf, long long lt, hipStream_t s);
 template <typename T>
 hipError_t launch_image(float *img, const T *PS, const T *PR, const Grid &g, int z0, int z1,
                         hipStream_t s);

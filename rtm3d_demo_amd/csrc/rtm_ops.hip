@@ -359,13 +359,13 @@ inline FrameDivs frame_divs(int R, int lz, int ny)
 template <typename T>
 __global__ void __launch_bounds__(kThreads)
     k_taper_frame(T *__restrict__ P, const float *__restrict__ t, Grid g, int b, int R, int taps,
-                  FrameDivs dv)
+                  FrameDivs dv, int zhold)
 {
     const int i = (int)(blockIdx.x * kThreads + threadIdx.x);  // count < 2^31 (launcher)
     int x, y, z;
     if (!frame_cell_fd(g, b, R, i, dv, x, y, z)) return;
     const long long o = goff(g, x, y, z);
-    P[o] = wst<T>(frame_taper<T>(wld(P[o]), t, g, b, x, y, z, taps));
+    P[o] = wst<T>(frame_taper<T>(wld(P[o]), t, g, b, x, y, z, z >= zhold ? taps - 1 : taps));
 }
 
 // taper3D at one cell, `taps` times (rounded to T between applications):
@@ -767,9 +767,12 @@ __global__ void __launch_bounds__(kThreads)
 
 // rtmSaveUpperBorder / rtmRestoreUpperBorder (RTMCPUPlatform.cpp:153-189):
 // upb[lt][x][y][k] <-> PP[x][y][blen-hf+k], 64-bit offsets.
+// mode 0 save, 1 restore, 2 save and then apply the upper taper (table t)
+// to the saved cell in place.
 template <typename T>
 __global__ void __launch_bounds__(kThreads)
-    k_upb(T *__restrict__ PP, T *__restrict__ upb, Grid g, int b, int hf, long long lt, int restore)
+    k_upb(T *__restrict__ PP, T *__restrict__ upb, Grid g, int b, int hf, long long lt, int mode,
+          const float *__restrict__ t)
 {
     const long long n = (long long)g.nx * g.ny * hf;
     const long long i = (long long)blockIdx.x * kThreads + threadIdx.x;
@@ -779,9 +782,15 @@ __global__ void __launch_bounds__(kThreads)
     const int y = (int)(r % g.ny);
     const int x = (int)(r / g.ny);
     const long long u = lt * n + i;
-    const long long o = goff(g, x, y, b - hf + k);
-    if (restore) PP[o] = upb[u];
-    else upb[u] = PP[o];
+    const int z = b - hf + k;
+    const long long o = goff(g, x, y, z);
+    if (mode == 1) {
+        PP[o] = upb[u];
+    } else {
+        const T v = PP[o];
+        upb[u] = v;
+        if (mode == 2) PP[o] = wst<T>(frame_taper<T>(wld(v), t, g, b, x, y, z, 1));
+    }
 }
 
 // rtmImageCondition (RTMCPUPlatform.cpp:513-540): z in [z0, z1), all x, y.
@@ -1104,15 +1113,16 @@ hipError_t launch_taper_all2(T *P, T *Q, const float *taper, const Grid &g, int 
 
 template <typename T>
 hipError_t launch_taper_frame(T *P, const float *taper, const Grid &g, int b, int R, int taps,
-                              hipStream_t s)
+                              hipStream_t s, int hold)
 {
     if (b <= 0 || taps <= 0) return hipSuccess;
+    if (hold < 0 || hold > b) return hipErrorInvalidValue;
     const long long lz = b - R;
     const long long n = (long long)g.nx * g.ny * R + (long long)2 * R * g.ny * lz +
                         (long long)(g.nx - 2 * R) * 2 * R * lz;
     if (n + kThreads >= (1LL << 31)) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_taper_frame<T>, dim3(nblocks(n)), dim3(kThreads), 0, s, P, taper, g, b, R,
-                       taps, frame_divs(R, b - R, g.ny));
+                       taps, frame_divs(R, b - R, g.ny), b - hold);
     return hipGetLastError();
 }
 
@@ -1210,7 +1220,18 @@ hipError_t launch_upb(T *PP, T *upb, const Grid &g, int b, int hf, long long lt,
 {
     const long long n = (long long)g.nx * g.ny * hf;
     hipLaunchKernelGGL(k_upb<T>, dim3(nblocks(n)), dim3(kThreads), 0, s, PP, upb, g, b, hf, lt,
-                       restore ? 1 : 0);
+                       restore ? 1 : 0, (const float *)nullptr);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_upb_save_taper(T *PP, T *upb, const float *taper, const Grid &g, int b, int hf,
+                                 long long lt, hipStream_t s)
+{
+    if (!taper || hf <= 0 || hf > b) return hipErrorInvalidValue;
+    const long long n = (long long)g.nx * g.ny * hf;
+    hipLaunchKernelGGL(k_upb<T>, dim3(nblocks(n)), dim3(kThreads), 0, s, PP, upb, g, b, hf, lt, 2,
+                       taper);
     return hipGetLastError();
 }
 
@@ -1344,7 +1365,7 @@ hipError_t launch_tick(int *p, int d, bool set, hipStream_t s)
     template hipError_t launch_taper_upper2<T>(T *, T *, const float *, const Grid &, int,      \
                                                hipStream_t);                                    \
     template hipError_t launch_taper_frame<T>(T *, const float *, const Grid &, int, int, int,  \
-                                              hipStream_t);                                     \
+                                              hipStream_t, int);                                \
     template hipError_t launch_taper_frame_all<T>(T *, const float *, const Grid &, int, int,   \
                                                   int, hipStream_t);                            \
     template hipError_t launch_taper_frame_xy<T>(T *, const float *, const Grid &, int, int,    \
@@ -1359,6 +1380,8 @@ hipError_t launch_tick(int *p, int d, bool set, hipStream_t s)
                                                   const T *, int, int, int, hipStream_t);       \
     template hipError_t launch_upb<T>(T *, T *, const Grid &, int, int, long long, bool,        \
                                       hipStream_t);                                             \
+    template hipError_t launch_upb_save_taper<T>(T *, T *, const float *, const Grid &, int,    \
+                                                 int, long long, hipStream_t);                  \
     template hipError_t launch_image<T>(float *, const T *, const T *, const Grid &, int, int,  \
                                         hipStream_t);                                           \
     template hipError_t launch_tail_fwd<T>(const BatchShot *, int, int, int, const float *,    \

@@ -663,7 +663,8 @@ struct WaveGrid {
 // (the previous step stored it tapered), the centre PP is tapered once and
 // the new value is tapered before it is stored; the front-right cells and
 // the frame are written by separate kernels, so this form needs no plane-b
-// loads (the pointwise chain only).
+// loads (the pointwise chain only).  TAPER 4 does the same with the upper
+// taper, leaving the planes [b - hold, b) of the output untapered.
 template <typename T, int R, int RY, bool IMG, int TAPER>
 __device__ __forceinline__ void wave_tile(const T *__restrict__ P, T *__restrict__ PP,
                                           const float *__restrict__ V, const Coefs &cf,
@@ -694,8 +695,10 @@ __device__ __forceinline__ void wave_tile(const T *__restrict__ P, T *__restrict
     constexpr bool OUT = TAPER == 3;
     constexpr bool EVERY = ALL || OUT;   // every border (taper3D)
     constexpr bool LOADTAP = TAPER == 1 || TAPER == 2;  // P tapered on load
-    // the wave holds a tapered value: some z < b (halo included), or any (EVERY)
-    const bool wtap = TAPER && (EVERY || z0 - R < tf.b);
+    constexpr bool OUTU = TAPER == 4;   // the upper taper on the output
+    // the wave holds a tapered value: some z < b (halo included, or only
+    // its own cells with OUTU), or any (EVERY)
+    const bool wtap = TAPER && (EVERY || (OUTU ? z0 < tf.b : z0 - R < tf.b));
     const int zq = (int)zc;
     const bool ztap = wtap && (EVERY ? (zq < tf.b || zq >= g.nz - tf.b) : (z < tf.b));
     float fzl = 1.0f;
@@ -911,6 +914,8 @@ __device__ __forceinline__ void wave_tile(const T *__restrict__ P, T *__restrict
             float pp = c_pp[r];
             if (OUT) {
                 pp = tapv(pp, fxc, fyr[r], ztap, fzl, false);
+            } else if (OUTU) {
+                if (wtap) pp = tapv(pp, fxc, fyr[r], ztap, fzl, false);
             } else if (TAPER && wtap) {
                 const bool qk = qxc && ((qmask >> r) & 1u) && zmid;
                 // a front-right PP after two passes: plane b's face value
@@ -927,8 +932,9 @@ __device__ __forceinline__ void wave_tile(const T *__restrict__ P, T *__restrict
                           OUT ? tapv(pp, fxc, fyr[r], ztap, fzl, false) : pp);
             if (((ymask >> r) & 1u) && zcomp) {
                 // OUT: the value the step stores (rounded), then taper3D
+                const bool otap = OUT || (OUTU && ztap && z < tf.b - tf.hold);
                 st_buf<T>(rpp, rowb[r], zoff,
-                          OUT ? tapv(wrnd<T>(npp), fxc, fyr[r], ztap, fzl, false) : npp);
+                          otap ? tapv(wrnd<T>(npp), fxc, fyr[r], ztap, fzl, false) : npp);
                 bad |= (npp != npp);
                 if (IMG && (z >= fz.z0) && (z < fz.z1) &&
                     !(rcvx && ((rmask >> r) & 1u) && z == fz.rz))
@@ -1048,7 +1054,9 @@ constexpr int wave2_min_occupancy()
 // TAPER 1: the upper taper fused on load; TAPER 3: taper3D applied to the
 // output, as in wave_tile (pointwise chains on the centre PP, the new value
 // and the z frame; the frame's x-y part and the front-right edge are written
-// by separate kernels).
+// by separate kernels); TAPER 4: the upper taper applied to the output (the
+// centre PP and the new value of cells z < b - hold; the frame by
+// k_taper_frame).
 template <typename T, int R, int RY, bool IMG, int TAPER>
 __device__ __forceinline__ void wave_tile2(const T *__restrict__ P, T *__restrict__ PP,
                                            const float *__restrict__ V, const Coefs &cf,
@@ -1078,8 +1086,9 @@ __device__ __forceinline__ void wave_tile2(const T *__restrict__ P, T *__restric
         return j >= 0 ? tab[max(j, 0)] : 1.0f;
     };
     constexpr bool OUT = TAPER == 3;
+    constexpr bool OUTU = TAPER == 4;   // the upper taper on the output
     constexpr bool LOADTAP = TAPER == 1;
-    const bool wtap = TAPER && (OUT || z0 - R < tf.b);
+    const bool wtap = TAPER && (OUT || (OUTU ? z0 < tf.b : z0 - R < tf.b));
     bool ztap[2];
     float fzl[2];
 #pragma unroll
@@ -1283,6 +1292,8 @@ __device__ __forceinline__ void wave_tile2(const T *__restrict__ P, T *__restric
                 float pp = c_pp[r][a];
                 if (OUT) {
                     pp = tapo(pp, a, fxc, fyr[r]);
+                } else if (OUTU) {
+                    if (wtap) pp = tap(pp, a, fxc, fyr[r]);
                 } else if (TAPER && wtap) {
                     pp = tap(pp, a, fxc, fyr[r]);
                     if (tf.pp_taps > 1) pp = tap(pp, a, fxc, fyr[r]);
@@ -1295,11 +1306,21 @@ __device__ __forceinline__ void wave_tile2(const T *__restrict__ P, T *__restric
                 st_buf2<T>(rpp, rowb[r], zoff, tapo(ppt[0], 0, fxc, fyr[r]),
                            tapo(ppt[1], 1, fxc, fyr[r]));
             if (((ymask >> r) & 1u) && zcomp) {
-                if (OUT)
+                if (OUT) {
                     st_buf2<T>(rpp, rowb[r], zoff, tapo(wrnd<T>(npp[0]), 0, fxc, fyr[r]),
                                tapo(wrnd<T>(npp[1]), 1, fxc, fyr[r]));
-                else
+                } else if (OUTU && wtap) {
+                    // cells z < b - hold: the value stored, then the upper taper
+                    float o[2];
+#pragma unroll
+                    for (int a = 0; a < 2; ++a)
+                        o[a] = (ztap[a] && z + a < tf.b - tf.hold)
+                                   ? tap(wrnd<T>(npp[a]), a, fxc, fyr[r])
+                                   : npp[a];
+                    st_buf2<T>(rpp, rowb[r], zoff, o[0], o[1]);
+                } else {
                     st_buf2<T>(rpp, rowb[r], zoff, npp[0], npp[1]);
+                }
                 bad |= (npp[0] != npp[0]) || (npp[1] != npp[1]);
                 if (IMG) {
                     bool im[2];
@@ -2355,6 +2376,44 @@ hipError_t launch_wave_out(const T *P, T *PP, const float *V, const Coefs &cf, c
                                                     (g.nz + 63) / 64);
 }
 
+// The step with the upper taper applied to its output (TAPER 4), with or
+// without imaging: one-z 64 x 1 x ry waves when the class tiling asks for
+// them (ry 6 or 8), else two-z 128 x 1 x 4 waves where cell pairs fit, else
+// one-z waves (64 x 1 x 6 with imaging, 64 x 1 x 8 without).
+template <typename T, int R>
+hipError_t launch_wave_outu(const T *P, T *PP, const float *V, const Coefs &cf, const Grid &g,
+                            const ImageFuse *fuse, const TaperFuse *taper, int *flag,
+                            hipStream_t s, const StencilTiling *t)
+{
+    if (!taper || taper->b < 1 || taper->b > 8192 || taper->hold < 0 || taper->hold > taper->b)
+        return hipErrorInvalidValue;
+    ImageFuse fz;
+    TaperFuse tf;
+    fuse_args(fuse, taper, fz, tf);
+    const int xchunk = t ? t->xchunk : 0;
+    const bool one = t && t->tz == 64 && t->by == 1 && (t->ry == 6 || t->ry == 8);
+    const bool two = !one && wave2_fits(P, PP, V, fuse, g);
+    const int ry = one ? t->ry : (fuse ? 6 : 8);
+    if (fuse) {
+        if (two)
+            return launch_wave2_cfg<T, R, 4, 4, true, 4>(P, PP, V, cf, g, fz, tf, flag, s, xchunk,
+                                                         0, 0, (g.nz + 127) / 128);
+        if (ry == 8)
+            return launch_wave_cfg<T, R, 8, 4, true, 4>(P, PP, V, cf, g, fz, tf, flag, s, xchunk,
+                                                        0, (g.nz + 63) / 64);
+        return launch_wave_cfg<T, R, 6, 4, true, 4>(P, PP, V, cf, g, fz, tf, flag, s, xchunk, 0,
+                                                    (g.nz + 63) / 64);
+    }
+    if (two)
+        return launch_wave2_cfg<T, R, 4, 4, false, 4>(P, PP, V, cf, g, fz, tf, flag, s, xchunk, 0,
+                                                      0, (g.nz + 127) / 128);
+    if (ry == 6)
+        return launch_wave_cfg<T, R, 6, 4, false, 4>(P, PP, V, cf, g, fz, tf, flag, s, xchunk, 0,
+                                                     (g.nz + 63) / 64);
+    return launch_wave_cfg<T, R, 8, 4, false, 4>(P, PP, V, cf, g, fz, tf, flag, s, xchunk, 0,
+                                                 (g.nz + 63) / 64);
+}
+
 // The 64-cell z tiles that the upper taper reaches (zt * 64 - R < b) on
 // one-z waves of RY rows with the taper fused, every deeper cell on
 // untapered two-z waves starting at the first untapered tile (even z).
@@ -2412,6 +2471,8 @@ hipError_t launch_radius(const T *P, T *PP, const float *V, const Coefs &cf, con
     if (taper && taper->all) {
         // taper3D fused: 64 x 1 x 8 one-z waves whatever the class tiling
         if constexpr (R == 4) {
+            if (taper->all == 3)
+                return launch_wave_outu<T, R>(P, PP, V, cf, g, fuse, taper, flag, s, t);
             if (taper->all == 2)
                 return launch_wave_out<T, R, 8, 4>(P, PP, V, cf, g, fuse, taper, flag, s, xchunk);
             return launch_wave_all<T, R, 8, 4>(P, PP, V, cf, g, fuse, taper, flag, s, xchunk);

@@ -452,9 +452,11 @@ def set_taper_fusion(fmt, mode):
     """mode 1: upper tapers fused into the stencil; 2: also the modeling
     loop's every-border tapers, applied as values are loaded; 3: those
     applied to each step's output instead (needs the source and receivers
-    off the borders, else the separate pass runs); 0: separate in-place
-    passes; -1: the format's default (upper tapers fused for F32, separate
-    for 16-bit formats; every-border tapers on the output)."""
+    off the borders, else the separate pass runs); 4: also the forward
+    loops' upper tapers applied to each step's output (backward loops fused
+    on load); 0: separate in-place passes; -1: the format's default (F32:
+    mode 4; 16-bit formats: upper tapers in separate passes, every-border
+    tapers on the output)."""
     check(load().rtmhip_set_taper_fusion(WAVE_FORMATS[fmt][0], mode))
 
 

@@ -56,10 +56,10 @@ def shape16(request, gpu):
         gpu.set_tiling(cls, 0)
 
 
-@pytest.fixture(params=[-1, 1], ids=["tdefault", "tfused"])
+@pytest.fixture(params=[-1, 1, 4], ids=["tdefault", "tfused", "tout"])
 def fusion16(request, gpu):
-    """The 16-bit default (upper tapers in a separate in-place pass) or the
-    tapers fused into the stencil."""
+    """The 16-bit default (upper tapers in a separate in-place pass), the
+    tapers fused into the stencil, or applied to each step's output."""
     for fmt in FMT16:
         gpu.set_taper_fusion(fmt, request.param)
     yield request.param

@@ -203,10 +203,10 @@ def xchunk(request, gpu):
 GRIDS = [(40, 44, 90), (32, 36, 340), (30, 32, 420), (30, 34, 91)]
 
 
-@pytest.fixture(params=[1, 0], ids=["tfused", "tsep"])
+@pytest.fixture(params=[1, 0, 4], ids=["tfused", "tsep", "tout"])
 def fusion32(request, gpu):
-    """Upper tapers fused into the FP32 stencil (the default) or applied by
-    a separate in-place pass."""
+    """Upper tapers fused into the FP32 stencil (the default), applied by a
+    separate in-place pass, or applied to each step's output."""
     gpu.set_taper_fusion("f32", request.param)
     yield request.param
     gpu.set_taper_fusion("f32", -1)
@@ -256,8 +256,8 @@ def test_fused_taper_forward_matches_operator_sequence(gpu, oracle, splits, xchu
 
 
 @pytest.mark.parametrize("b", [12, 11])
-@pytest.mark.parametrize("mode, fusion", [(1, 1), (1, 0), (2, 1)],
-                         ids=["m1-tfused", "m1-tsep", "m2"])
+@pytest.mark.parametrize("mode, fusion", [(1, 1), (1, 0), (1, 4), (2, 1)],
+                         ids=["m1-tfused", "m1-tsep", "m1-tout", "m2"])
 @pytest.mark.parametrize("shape", GRIDS)
 @pytest.mark.parametrize("splits", [(3,), (1, 2)])
 def test_fused_backward_matches_operator_sequence(gpu, oracle, splits, xchunk, shape, mode,
