@@ -992,10 +992,19 @@ def main():
              2: ("tapered forward step", 16.0),
              3: ("dual backward step (source + receiver + image)", 36.0)}
 
+    # FP32 default (and mode 4): the forward step stores the upper taper of
+    # its output, on two-z 128x1x4 waves unless the class tiling asks for
+    # one-z 64x1x6 / 64x1x8 waves
+    outu = hip.get_taper_fusion("f32") in (-1, 4)
+    if outu:
+        names[2] = ("tapered forward step (upper taper on the output)", 16.0)
+
     def cls_stats(c):
         (name, bpc), (n, ms) = names[c], cls_t[c]
         avg = ms / n * 1e-3 if n else None
         tz, by, ry, _ = hip.get_tiling(c)
+        if c == 2 and outu and not (tz == 64 and by == 1 and ry in (6, 8)):
+            tz, by, ry = 128, 1, 4
         return {"kernel": name, "tile": "%dx%dx%d" % (tz, by, ry) if tz else "default",
                 "launches": n, "avg_ms": round(avg * 1e3, 4) if n else None,
                 "achieved_GBs": round(bpc * sh.U / avg / 1e9, 1) if n else None,

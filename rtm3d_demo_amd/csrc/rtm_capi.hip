@@ -886,8 +886,22 @@ int snap_backward_run(const rtmhip_shot *shot, const ShotCtx &c, const float *V2
     hipStream_t s = g_state.stream;
     const int b = shot->blen;
     const bool sep = g_state.taper_sep[shot->wave_format];
+    // the receiver-only steps take the upper taper on the output like the
+    // forward loops (the receivers and the imaged range lie at z >= blen)
+    const bool outu = g_state.taper_outu[shot->wave_format] && c.radius == 4 && c.rz >= b;
     for (int it = it_hi; it > it_hi - nsteps; --it) {
-        if (sep) {
+        if (outu && it > it_hi - nsteps + 1) {
+            if (it == it_hi) HIPCHK(launch_taper(*Pr, taper, c.g, b, true, s), "taper");
+            const TaperFuse tf{taper, b, 1, 3, 0};
+            HIPCHK(stencil(c.radius, *Pr, *PPr, V2DT2, c.cf, c.g, nullptr, &tf), "stencil");
+            HIPCHK(launch_taper_frame(*PPr, taper, c.g, b, c.radius, 2, s), "frame taper");
+        } else if (outu) {
+            if (it == it_hi)
+                HIPCHK(launch_taper_upper2(*Pr, *PPr, taper, c.g, b, s), "taper");
+            else
+                HIPCHK(launch_taper(*PPr, taper, c.g, b, true, s), "taper");
+            HIPCHK(stencil(c.radius, *Pr, *PPr, V2DT2, c.cf, c.g, nullptr), "stencil");
+        } else if (sep) {
             HIPCHK(launch_taper_upper2(*Pr, *PPr, taper, c.g, b, s), "taper");
             HIPCHK(stencil(c.radius, *Pr, *PPr, V2DT2, c.cf, c.g, nullptr), "stencil");
         } else {
@@ -909,7 +923,7 @@ int snap_backward_run(const rtmhip_shot *shot, const ShotCtx &c, const float *V2
         *Pr = *PPr;
         *PPr = t;
     }
-    if (!sep) HIPCHK(launch_taper(*PPr, taper, c.g, b, true, s), "taper");
+    if (!sep && !outu) HIPCHK(launch_taper(*PPr, taper, c.g, b, true, s), "taper");
     return RTMHIP_OK;
 }
 
