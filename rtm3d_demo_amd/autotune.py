@@ -37,6 +37,14 @@ def tune(hip, shot, V, coefs, g, src, F, R, upb, rcv, img, steps=6, rounds=2, lo
                 log("tile %s class %d: %.4f ms" % (tiling, cls, avg))
             if cls not in best or avg < best[cls][1]:
                 best[cls] = (tiling, avg)
+    # With the forward upper taper on the output (the FP32 default) every
+    # class-2 shape but the one-z 64x1x6 / 64x1x8 waves launches the same
+    # two-z 128x1x4 kernel: record that shape
+    fmt = {0: "f32", 1: "f16", 2: "bf16"}.get(getattr(shot, "wave_format", 0), "f32")
+    if 2 in best and hip.get_taper_fusion(fmt) in (-1, 4) and fmt == "f32":
+        t2 = tuple(best[2][0])
+        if not (t2[0] == 64 and t2[1] == 1 and t2[2] in (6, 8)):
+            best[2] = ((128, 1, 4), best[2][1])
     for cls in range(3):
         if cls in best:
             hip.set_tiling(cls, *best[cls][0], 0)

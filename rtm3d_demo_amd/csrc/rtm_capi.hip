@@ -1399,6 +1399,7 @@ struct rtmhip_batch {
     int cs = 0, cr = 0;             // P = f[c], PP = f[c ^ 1]
     int mode[3] = {0, 0, 0};        // per stencil class: 0 per-shot launches, 6/8 batched
     int graph_steps = 8;            // steps per captured graph (0: no graphs)
+    bool fout = false;              // forward steps: upper taper on the output (this call)
     std::map<int, hipGraphExec_t> graphs;
     int tiling_gen = 0;             // g_state.tiling_gen when the graphs were captured
 };
@@ -1407,15 +1408,57 @@ namespace {
 
 int batch_tab(const rtmhip_batch *b, int cls) { return b->mode[cls] == 6 ? 0 : 1; }
 
+// The forward steps of a batch store the upper taper of their output (as
+// forward_loop) when the format asks for it, the forward stencil runs as
+// per-shot launches, and every source lies at z >= blen.
+bool batch_fwd_out(const rtmhip_batch *b)
+{
+    if (!g_state.taper_outu[b->wave_format] || b->mode[2] || b->radius != 4) return false;
+    for (int i = 0; i < b->n; ++i) {
+        const BatchShot &h = b->hs[0][i];
+        if (h.src_off % h.g.sy < b->blen) return false;
+    }
+    return true;
+}
+
 // One forward step (HBC) of every shot at time it: with itp != nullptr the
-// small operators read it from *itp (plus itoff).
+// small operators read it from *itp (plus itoff).  form 0: tapers on load
+// (taps as in forward_loop); 1: the upper taper on the output (first: P
+// tapered in place before the step); 2: the last step of a call with form 1
+// (PP tapered in place, plain step; first: P as well).
 template <typename T>
-int batch_fwd_step(rtmhip_batch *b, int taps, const int *itp, int itoff)
+int batch_fwd_step(rtmhip_batch *b, int taps, const int *itp, int itoff, int form = 0,
+                   bool first = false)
 {
     hipStream_t s = g_state.stream;
     b->cs ^= 1;
     const TaperFuse tf{b->taper, b->blen, taps};
     const int cls = 2;
+    if (form) {
+        for (int i = 0; i < b->n; ++i) {
+            const BatchShot &h = b->hs[0][i];
+            T *P = (T *)h.fs[b->cs], *PP = (T *)h.fs[b->cs ^ 1];
+            if (first && form == 2)
+                HIPCHK(launch_taper_upper2<T>(P, PP, b->taper, h.g, b->blen, s), "taper");
+            else if (first)
+                HIPCHK(launch_taper<T>(P, b->taper, h.g, b->blen, true, s), "taper");
+            else if (form == 2)
+                HIPCHK(launch_taper<T>(PP, b->taper, h.g, b->blen, true, s), "taper");
+        }
+        const TaperFuse to{b->taper, b->blen, 1, 3, b->hf};
+        for (int i = 0; i < b->n; ++i) {
+            const BatchShot &h = b->hs[0][i];
+            HIPCHK(launch_stencil<T>(b->radius, (const T *)h.fs[b->cs], (T *)h.fs[b->cs ^ 1], h.V,
+                                     b->cf, h.g, nullptr, form == 1 ? &to : nullptr,
+                                     g_state.flag, s, class_tiling(form == 1 ? cls : 0)),
+                   "stencil");
+        }
+        HIPCHK(launch_tail_fwd<T>(b->ds[0], b->n, b->fwd_blocks, b->cs, b->taper, b->blen,
+                                  b->radius, b->hf, form == 1 ? 2 : 0, itp, itoff, s,
+                                  form == 1),
+               "forward tail");
+        return RTMHIP_OK;
+    }
     if (b->mode[cls]) {
         const int k = batch_tab(b, cls);
         HIPCHK(launch_stencil_batch<T>(b->mode[cls], b->ds[k], b->hs[k], b->n, b->nblk[k],
@@ -1504,7 +1547,7 @@ int batch_graph(rtmhip_batch *b, int dir, hipGraphExec_t *out)
         b->graphs.clear();
         b->tiling_gen = g_state.tiling_gen;
     }
-    const int key = (dir > 0 ? 0 : 4) | (b->cs << 1) | b->cr;
+    const int key = (dir > 0 ? 0 : 4) | (b->cs << 1) | b->cr | (dir > 0 && b->fout ? 8 : 0);
     auto f = b->graphs.find(key);
     if (f != b->graphs.end()) {
         *out = f->second;
@@ -1520,7 +1563,8 @@ int batch_graph(rtmhip_batch *b, int dir, hipGraphExec_t *out)
     capture_guard(true);
     int rc = RTMHIP_OK;
     for (int k = 0; k < b->graph_steps && rc == RTMHIP_OK; ++k)
-        rc = dir > 0 ? batch_fwd_step<T>(b, 2, b->ctr, k) : batch_bwd_step<T>(b, 2, b->ctr, -k);
+        rc = dir > 0 ? batch_fwd_step<T>(b, 2, b->ctr, k, b->fout ? 1 : 0)
+                     : batch_bwd_step<T>(b, 2, b->ctr, -k);
     hipError_t e = rc == RTMHIP_OK ? launch_tick(b->ctr, dir * b->graph_steps, false, s)
                                    : hipSuccess;
     capture_guard(false);
@@ -1553,25 +1597,32 @@ template <typename T>
 int batch_run(rtmhip_batch *b, int dir, int it0, int nsteps)
 {
     hipStream_t s = g_state.stream;
-    auto step = [&](int taps, int it) {
+    // forward steps with the upper taper on the output: the first tapers P
+    // in place, the last (never inside a graph) runs the reference's order
+    b->fout = dir > 0 && batch_fwd_out(b);
+    auto step = [&](int taps, int it, int k) {
+        if (dir > 0 && b->fout)
+            return batch_fwd_step<T>(b, taps, nullptr, it, k == nsteps - 1 ? 2 : 1, k == 0);
         return dir > 0 ? batch_fwd_step<T>(b, taps, nullptr, it)
                        : batch_bwd_step<T>(b, taps, nullptr, it);
     };
-    int rc = step(1, it0);
+    int rc = step(1, it0, 0);
     if (rc) return rc;
     int done = 1;
     const int G = b->graph_steps;
-    if (G > 0 && nsteps - done >= G) {
+    const int keep = b->fout ? 1 : 0;
+    if (G > 0 && nsteps - keep - done >= G) {
         hipGraphExec_t exec;
         if ((rc = batch_graph<T>(b, dir, &exec))) return rc;
         HIPCHK(launch_tick(b->ctr, it0 + dir * done, true, s), "time index");
-        while (nsteps - done >= G) {
+        while (nsteps - keep - done >= G) {
             HIPCHK(hipGraphLaunch(exec, s), "hipGraphLaunch");
             done += G;  // parities unchanged: G is even
         }
     }
     for (; done < nsteps; ++done)
-        if ((rc = step(2, it0 + dir * done))) return rc;
+        if ((rc = step(2, it0 + dir * done, done))) return rc;
+    if (dir > 0 && b->fout) return RTMHIP_OK;
     // the pending in-place taper: P of the source field (forward) or the
     // older receiver field (backward), as the per-shot loops leave them
     for (int i = 0; i < b->n; ++i) {

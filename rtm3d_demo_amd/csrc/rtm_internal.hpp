@@ -234,14 +234,17 @@ hipError_t launch_stencil_batch(int ry, const BatchShot *shots, const BatchShot 
 // a batched stencil launch, written into hs.
 long long plan_stencil_batch(int ry, BatchShot *hs, int n, int R, int blen);
 // Forward tail of one HBC step on every shot: frame taper (taps times) of
-// PP = fs[par ^ 1], source injection, UPB save at time it.
+// PP = fs[par ^ 1], source injection, UPB save at time it; with out (a step
+// that stored the upper taper of its output outside the UPB layer) the
+// layer is saved and then tapered once, its frame cells tapered taps - 1
+// times before the save.
 // Backward tail: UPB restore into fs[spar ^ 1], frame taper of
 // fr[rpar ^ 1], receiver injection and the imaging condition off the
 // stencil's cells.  it = (itp ? *itp : 0) + itoff.
 template <typename T>
 hipError_t launch_tail_fwd(const BatchShot *shots, int n, int blocks, int par,
                            const float *taper, int b, int R, int hf, int taps, const int *itp,
-                           int itoff, hipStream_t s);
+                           int itoff, hipStream_t s, bool out = false);
 template <typename T>
 hipError_t launch_tail_bwd(const BatchShot *shots, int n, int blocks, int spar, int rpar,
                            const float *taper, int b, int R, int hf, int taps, int nt,

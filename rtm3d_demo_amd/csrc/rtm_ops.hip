@@ -922,7 +922,7 @@ __device__ __forceinline__ int batch_shot(const BatchShot *__restrict__ shots, i
 template <typename T>
 __global__ void __launch_bounds__(kThreads)
     k_tail_fwd(const BatchShot *__restrict__ shots, int n, int par, const float *__restrict__ t,
-               int b, int R, int hf, int taps, const int *__restrict__ itp, int itoff)
+               int b, int R, int hf, int taps, const int *__restrict__ itp, int itoff, int out)
 {
     const BatchShot &sh = shots[batch_shot<&BatchShot::fwd0>(shots, n, (int)blockIdx.x)];
     int i = ((int)blockIdx.x - sh.fwd0) * kThreads + (int)threadIdx.x;
@@ -957,14 +957,21 @@ __global__ void __launch_bounds__(kThreads)
     const bool fr = in_frame(g, b, R, x, y, z), src = o == sh.src_off;
     if (!fr && !src && !save) return;
     T tv = PP[o];
+    // out: the step stored the upper taper of its output except in the
+    // layer, which is saved first and then tapered once (its frame cells
+    // carry one taper fewer until then)
+    const int ftaps = (out && save) ? taps - 1 : taps;
     if (fr || src) {
         float v = wld(tv);
-        if (fr) v = wrnd<T>(frame_taper<T>(v, t, g, b, x, y, z, taps));
+        if (fr) v = wrnd<T>(frame_taper<T>(v, t, g, b, x, y, z, ftaps));
         if (src) v = v + sh.ricker[it];
         tv = wst<T>(v);
         PP[o] = tv;
     }
-    if (save) static_cast<T *>(sh.upb)[(long long)it * nl + i] = tv;
+    if (save) {
+        static_cast<T *>(sh.upb)[(long long)it * nl + i] = tv;
+        if (out) PP[o] = wst<T>(frame_taper<T>(wld(tv), t, g, b, x, y, z, 1));
+    }
 }
 
 // Backward tail of an HBC step: UPB restore of step `it` into the new source
@@ -1332,11 +1339,11 @@ int tail_blocks(long long items) { return (int)((items + kThreads - 1) / kThread
 template <typename T>
 hipError_t launch_tail_fwd(const BatchShot *shots, int n, int blocks, int par,
                            const float *taper, int b, int R, int hf, int taps, const int *itp,
-                           int itoff, hipStream_t s)
+                           int itoff, hipStream_t s, bool out)
 {
     if (blocks <= 0 || n <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_tail_fwd<T>, dim3(blocks), dim3(kThreads), 0, s, shots, n, par, taper, b,
-                       R, hf, taps, itp, itoff);
+                       R, hf, taps, itp, itoff, out ? 1 : 0);
     return hipGetLastError();
 }
 
@@ -1385,7 +1392,8 @@ hipError_t launch_tick(int *p, int d, bool set, hipStream_t s)
     template hipError_t launch_image<T>(float *, const T *, const T *, const Grid &, int, int,  \
                                         hipStream_t);                                           \
     template hipError_t launch_tail_fwd<T>(const BatchShot *, int, int, int, const float *,    \
-                                           int, int, int, int, const int *, int, hipStream_t);  \
+                                           int, int, int, int, const int *, int, hipStream_t,   \
+                                           bool);                                               \
     template hipError_t launch_tail_bwd<T>(const BatchShot *, int, int, int, int,               \
                                            const float *, int, int, int, int, int, const int *, \
                                            int, hipStream_t);
