@@ -1018,8 +1018,12 @@ int put_basis(void *store, const float *basis)
                 if (std::fabs(dot - (u == v ? 1.0 : 0.0)) > 1e-4)
                     return fail(RTMHIP_ERR_INVALID, "basis rows are not orthonormal");
             }
-    HIPCHK(hipMemcpyAsync(store, basis, sizeof(BrickBasis), hipMemcpyHostToDevice,
-                          g_state.stream),
+    BrickBasis hb{};
+    std::memcpy(&hb.z[0][0], basis, sizeof(hb.z));
+    std::memcpy(&hb.x[0][0], basis + 32 * 32, sizeof(hb.x));
+    std::memcpy(&hb.y[0][0], basis + 32 * 32 + 64, sizeof(hb.y));
+    hb.learned = 1;
+    HIPCHK(hipMemcpyAsync(store, &hb, sizeof(BrickBasis), hipMemcpyHostToDevice, g_state.stream),
            "hipMemcpyAsync");
     HIPCHK(hipStreamSynchronize(g_state.stream), "hipStreamSynchronize");
     return RTMHIP_OK;

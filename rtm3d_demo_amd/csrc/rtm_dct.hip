@@ -66,10 +66,32 @@ constexpr BrickBasis make_dct_basis()
 }
 __constant__ BrickBasis kDct = make_dct_basis();
 
+// Basis entries: the caller's (L, read through B with wave-uniform indices,
+// so they come through scalar loads) or the compile-time DCT-II table, whose
+// entries at constant indices the compiler folds into the instructions.
+template <bool L>
+__device__ __forceinline__ float bz(const BrickBasis *__restrict__ B, int u, int i)
+{
+    if constexpr (L) return B->z[u][i];
+    else return kDct.z[u][i];
+}
+template <bool L>
+__device__ __forceinline__ float bx(const BrickBasis *__restrict__ B, int u, int i)
+{
+    if constexpr (L) return B->x[u][i];
+    else return kDct.x[u][i];
+}
+template <bool L>
+__device__ __forceinline__ float by(const BrickBasis *__restrict__ B, int u, int i)
+{
+    if constexpr (L) return B->y[u][i];
+    else return kDct.y[u][i];
+}
+
 __global__ void k_put_basis(BrickBasis *__restrict__ dst)
 {
-    const float *src = &kDct.z[0][0];
-    float *d = &dst->z[0][0];
+    const int *src = reinterpret_cast<const int *>(&kDct);
+    int *d = reinterpret_cast<int *>(dst);
     for (int i = threadIdx.x; i < (int)(sizeof(BrickBasis) / 4); i += blockDim.x) d[i] = src[i];
 }
 
@@ -90,6 +112,7 @@ __device__ __forceinline__ void brick_coords(long long br, const DctLayout &L, i
 // pass's outputs stay in the thread's out (x index u = 0..7 at y = tid>>5,
 // z = tid&31).  z pass: wave q computes outputs 8q .. 8q+7 of the 64 lines
 // (lane = line), so every matrix index is wave-uniform.
+template <bool L>
 __device__ __forceinline__ void dct3_forward(float *t, const BrickBasis *__restrict__ B,
                                              float (&out)[8])
 {
@@ -105,7 +128,7 @@ __device__ __forceinline__ void dct3_forward(float *t, const BrickBasis *__restr
         for (int r = 0; r < 8; ++r) {
             float a = 0.0f;
 #pragma unroll
-            for (int k = 0; k < 32; ++k) a = __builtin_fmaf(B->z[8 * q + r][k], in[k], a);
+            for (int k = 0; k < 32; ++k) a = __builtin_fmaf(bz<L>(B, 8 * q + r, k), in[k], a);
             o[r] = a;
         }
         __syncthreads();
@@ -122,7 +145,7 @@ __device__ __forceinline__ void dct3_forward(float *t, const BrickBasis *__restr
         for (int u = 0; u < 8; ++u) {
             float a = 0.0f;
 #pragma unroll
-            for (int j = 0; j < 8; ++j) a = __builtin_fmaf(B->y[u][j], in[j], a);
+            for (int j = 0; j < 8; ++j) a = __builtin_fmaf(by<L>(B, u, j), in[j], a);
             o[u] = a;
         }
         __syncthreads();
@@ -139,13 +162,14 @@ __device__ __forceinline__ void dct3_forward(float *t, const BrickBasis *__restr
         for (int u = 0; u < 8; ++u) {
             float a = 0.0f;
 #pragma unroll
-            for (int i = 0; i < 8; ++i) a = __builtin_fmaf(B->x[u][i], in[i], a);
+            for (int i = 0; i < 8; ++i) a = __builtin_fmaf(bx<L>(B, u, i), in[i], a);
             out[u] = a;
         }
     }
 }
 
 // Inverse transform in place (coefficients in t as dct3_forward leaves them).
+template <bool L>
 __device__ __forceinline__ void dct3_inverse(float *t, const BrickBasis *__restrict__ B)
 {
     const int tid = threadIdx.x;
@@ -158,7 +182,7 @@ __device__ __forceinline__ void dct3_inverse(float *t, const BrickBasis *__restr
         for (int i = 0; i < 8; ++i) {
             float a = 0.0f;
 #pragma unroll
-            for (int u = 0; u < 8; ++u) a = __builtin_fmaf(B->x[u][i], in[u], a);
+            for (int u = 0; u < 8; ++u) a = __builtin_fmaf(bx<L>(B, u, i), in[u], a);
             o[i] = a;
         }
         __syncthreads();
@@ -175,7 +199,7 @@ __device__ __forceinline__ void dct3_inverse(float *t, const BrickBasis *__restr
         for (int j = 0; j < 8; ++j) {
             float a = 0.0f;
 #pragma unroll
-            for (int u = 0; u < 8; ++u) a = __builtin_fmaf(B->y[u][j], in[u], a);
+            for (int u = 0; u < 8; ++u) a = __builtin_fmaf(by<L>(B, u, j), in[u], a);
             o[j] = a;
         }
         __syncthreads();
@@ -194,7 +218,7 @@ __device__ __forceinline__ void dct3_inverse(float *t, const BrickBasis *__restr
         for (int r = 0; r < 8; ++r) {
             float a = 0.0f;
 #pragma unroll
-            for (int w = 0; w < 32; ++w) a = __builtin_fmaf(B->z[w][8 * q + r], in[w], a);
+            for (int w = 0; w < 32; ++w) a = __builtin_fmaf(bz<L>(B, w, 8 * q + r), in[w], a);
             o[r] = a;
         }
         __syncthreads();
@@ -222,6 +246,7 @@ __global__ void __launch_bounds__(kDctThreads)
     h[threadIdx.x] = 0;
     __syncthreads();
     const int tid = threadIdx.x;
+    const bool learned = B->learned != 0;
     for (long long br = blockIdx.x; br < L.nbr; br += gridDim.x) {
         int bx0, by0, bz0;
         brick_coords(br, L, bx0, by0, bz0);
@@ -236,7 +261,10 @@ __global__ void __launch_bounds__(kDctThreads)
         }
         __syncthreads();
         float out[8];
-        dct3_forward(t, B, out);
+        if (learned)
+            dct3_forward<true>(t, B, out);
+        else
+            dct3_forward<false>(t, B, out);
         float *cb = coef + br * kCells;
         const int j = tid >> 5, k = tid & 31;
 #pragma unroll
@@ -360,6 +388,7 @@ __global__ void __launch_bounds__(kDctThreads)
 {
     __shared__ float t[kTile];
     const int tid = threadIdx.x;
+    const bool learned = B->learned != 0;
     for (long long br = blockIdx.x; br < L.nbr; br += gridDim.x) {
         for (int e = tid; e < kTile; e += kDctThreads) t[e] = 0.0f;
         __syncthreads();
@@ -397,7 +426,10 @@ __global__ void __launch_bounds__(kDctThreads)
             }
         }
         __syncthreads();
-        dct3_inverse(t, B);
+        if (learned)
+            dct3_inverse<true>(t, B);
+        else
+            dct3_inverse<false>(t, B);
 #pragma unroll
         for (int m = 0; m < 8; ++m) {
             if (in[m]) {

@@ -147,6 +147,7 @@ struct BrickBasis {
     float z[32][32];
     float x[8][8];
     float y[8][8];
+    int learned;  // 0: the DCT-II (the kernels use their compile-time copy)
 };
 hipError_t dct_default_basis(BrickBasis *dst, hipStream_t s);
 struct DctLayout {

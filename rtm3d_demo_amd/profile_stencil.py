@@ -144,8 +144,8 @@ def summarize(fetch_dir, write_dir, tilings=None, wave_format="f32"):
         if short.startswith("k_stencil_dual"):
             cls = 3
         elif short.startswith("k_stencil"):
-            tail = short.rsplit(",", 2)[-2:]  # the IMG and TAPER flags
-            img, tap = (t.strip(" >") == "true" for t in tail)
+            tail = short.rsplit(",", 2)[-2:]  # the IMG and TAPER arguments
+            img, tap = (t.strip(" >") not in ("false", "0") for t in tail)
             cls = 1 if img else (2 if tap else 0)
         if cls is not None and cls not in first:
             first[cls] = (short, r + w)
