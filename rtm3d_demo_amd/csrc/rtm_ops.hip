@@ -769,19 +769,20 @@ __global__ void __launch_bounds__(kThreads)
 // upb[lt][x][y][k] <-> PP[x][y][blen-hf+k], 64-bit offsets.
 // mode 0 save, 1 restore, 2 save and then apply the upper taper (table t)
 // to the saved cell in place.
+// 32-bit cell index (n = nx * ny * hf < 2^31, launcher) with divisions by
+// launch constants through a multiply-high; the history offset is 64-bit.
 template <typename T>
 __global__ void __launch_bounds__(kThreads)
     k_upb(T *__restrict__ PP, T *__restrict__ upb, Grid g, int b, int hf, long long lt, int mode,
-          const float *__restrict__ t)
+          const float *__restrict__ t, int n, FastDiv dhf, FastDiv dny)
 {
-    const long long n = (long long)g.nx * g.ny * hf;
-    const long long i = (long long)blockIdx.x * kThreads + threadIdx.x;
+    const int i = (int)(blockIdx.x * kThreads + threadIdx.x);
     if (i >= n) return;
-    const int k = (int)(i % hf);
-    const long long r = i / hf;
-    const int y = (int)(r % g.ny);
-    const int x = (int)(r / g.ny);
-    const long long u = lt * n + i;
+    const int r = fdiv(i, dhf);
+    const int k = i - r * hf;
+    const int x = fdiv(r, dny);
+    const int y = r - x * g.ny;
+    const long long u = lt * (long long)n + i;
     const int z = b - hf + k;
     const long long o = goff(g, x, y, z);
     if (mode == 1) {
@@ -1226,8 +1227,10 @@ hipError_t launch_upb(T *PP, T *upb, const Grid &g, int b, int hf, long long lt,
                       bool restore, hipStream_t s)
 {
     const long long n = (long long)g.nx * g.ny * hf;
+    if (hf <= 0 || n + kThreads >= (1LL << 31)) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_upb<T>, dim3(nblocks(n)), dim3(kThreads), 0, s, PP, upb, g, b, hf, lt,
-                       restore ? 1 : 0, (const float *)nullptr);
+                       restore ? 1 : 0, (const float *)nullptr, (int)n, fast_div((uint32_t)hf),
+                       fast_div((uint32_t)g.ny));
     return hipGetLastError();
 }
 
@@ -1237,8 +1240,9 @@ hipError_t launch_upb_save_taper(T *PP, T *upb, const float *taper, const Grid &
 {
     if (!taper || hf <= 0 || hf > b) return hipErrorInvalidValue;
     const long long n = (long long)g.nx * g.ny * hf;
+    if (n + kThreads >= (1LL << 31)) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_upb<T>, dim3(nblocks(n)), dim3(kThreads), 0, s, PP, upb, g, b, hf, lt, 2,
-                       taper);
+                       taper, (int)n, fast_div((uint32_t)hf), fast_div((uint32_t)g.ny));
     return hipGetLastError();
 }
 
