@@ -1,9 +1,11 @@
 """Parity at the benchmark's full grid sizes: one complete backward step of
 HBC migration (source reconstruction with upper-border restore, receiver
 tapers, fused receiver step + injection + imaging) on the 512x512x256+2*40
-grid, one stencil step on the 800x800x400+2*40 grid, and two fused HBC
-forward steps with FP16 and with BF16 wavefields on that grid, bit-exact
-against the oracle on the same random fields."""
+grid, four HBC forward steps and four modeling steps there with the FP32
+defaults (tapers applied to each step's output), one stencil step on the
+800x800x400+2*40 grid, and two fused HBC forward steps with FP16 and with
+BF16 wavefields on that grid, bit-exact against the oracle on the same
+random fields."""
 import numpy as np
 import pytest
 
@@ -62,6 +64,85 @@ def test_backward_step_full_grid(gpu, oracle):
     assert _bits_equal(d["img"].download(), img)
     for x in d.values():
         x.free()
+
+
+SALT = (592, 592, 336)
+
+
+def test_forward_steps_output_taper_full_grid(gpu, oracle):
+    """Four HBC forward steps with the FP32 default, the upper taper applied
+    to each step's output (the first step tapers P in place, the middle two
+    run the output-taper kernel, the last the plain step), against the
+    reference's sequence (taper, taper, step, source, save UPB)."""
+    shape, b, order, nt = SALT, 40, 8, 4
+    print("full-size forward steps", shape, flush=True)
+    gpu.set_taper_fusion("f32", -1)
+    P0, PP0 = _rand(shape, 31), _rand(shape, 32)
+    v2 = _rand(shape, 33, 0.0, 0.2)
+    g = host.taper_table(b, 0.9)
+    coefs = host.stencil_coefs(order, 12.5, 12.5, 12.5)
+    src = np.array([0.75, -0.5, 0.25, 1.0], np.float32)
+    F = gpu.FieldPair(shape)
+    F.a.upload(P0)
+    F.b.upload(PP0)
+    V, dg, ds = (gpu.DeviceArray.from_host(a) for a in (v2, g, src))
+    dupb = gpu.DeviceArray((nt, shape[0], shape[1], 4)).zero()
+    shot = gpu.make_shot(*shape, b, order, nt, nt, (296, 296, b + 2),
+                         gpu.RcvGeom(0, 0, 0, 1, 1, 1, 1))
+    gpu.forward_steps(shot, True, V, coefs, dg, ds, F, dupb, 0, nt)
+    got_P, got_PP, got_upb = F.P.download(), F.PP.download(), dupb.download()
+    for d in (F, V, dg, ds, dupb):
+        d.free()
+    P, PP = P0, PP0
+    upb = np.zeros((nt, shape[0], shape[1], 4), np.float32)
+    for it in range(nt):
+        P, PP = PP, P
+        oracle.taper(P, g, b, True)
+        oracle.taper(PP, g, b, True)
+        oracle.step(P, PP, v2, coefs, order)
+        oracle.apply_source(PP, 296, 296, b + 2, src[it])
+        oracle.upb(PP, upb, b, order, it, False)
+    assert _bits_equal(got_PP, PP)
+    assert _bits_equal(got_P, P)
+    assert _bits_equal(got_upb, upb)
+
+
+def test_model_steps_output_taper_full_grid(gpu, oracle):
+    """Four modeling steps with the default, taper3D of both fields applied
+    to each step's output (x-y frame and front-right edge by their own
+    kernels), with the salt shot's 128 x 128 receivers, against the
+    reference's sequence (taper3D, taper3D, step, source, save receivers)."""
+    shape, b, order, nt = SALT, 40, 8, 4
+    print("full-size modeling steps", shape, flush=True)
+    P0, PP0 = _rand(shape, 41), _rand(shape, 42)
+    v2 = _rand(shape, 43, 0.0, 0.2)
+    g = host.taper_table(b, 0.9)
+    coefs = host.stencil_coefs(order, 12.5, 12.5, 12.5)
+    src = np.array([0.75, -0.5, 0.25, 1.0], np.float32)
+    rgeom = (0, 0, 0, 4, 4, 128, 128)
+    F = gpu.FieldPair(shape)
+    F.a.upload(P0)
+    F.b.upload(PP0)
+    V, dg, ds = (gpu.DeviceArray.from_host(a) for a in (v2, g, src))
+    drec = gpu.DeviceArray((128, 128, nt)).zero()
+    shot = gpu.make_shot(*shape, b, order, nt, nt, (296, 296, b + 2), gpu.RcvGeom(*rgeom))
+    gpu.model_steps(shot, V, coefs, dg, ds, F, drec, 0, nt)
+    got_P, got_PP, got_rec = F.P.download(), F.PP.download(), drec.download()
+    for d in (F, V, dg, ds, drec):
+        d.free()
+    P, PP = P0, PP0
+    rec = np.zeros((128, 128, nt), np.float32)
+    og = oracle.RcvGeom(*rgeom)
+    for it in range(nt):
+        P, PP = PP, P
+        oracle.taper(P, g, b, False)
+        oracle.taper(PP, g, b, False)
+        oracle.step(P, PP, v2, coefs, order)
+        oracle.apply_source(PP, 296, 296, b + 2, src[it])
+        oracle.receivers(PP, rec, b, og, nt, it, False)
+    assert _bits_equal(got_rec, rec)
+    assert _bits_equal(got_PP, PP)
+    assert _bits_equal(got_P, P)
 
 
 def test_stencil_step_800_grid(gpu, oracle):
