@@ -159,6 +159,24 @@ int rtmhip_freq_image(float *img, const float *PS, const float *PR, float *sre, 
                       const float *w2, int nxe, int nye, int nze, int blen, int nwstep, int iw0,
                       int nw, int nt, int st);
 
+/* The frequency-domain migration loop of one frequency group
+ * (RTMFreqDomainKernel::rtmFreqDomainPropagation,
+ * RTMFiniteDifferencesKernel_FreqDomain.cpp:220-357): for rt = it_hi down to
+ * it_hi - nsteps + 1, with st = nt-1-rt and lt = rt % ntstep, both fields
+ * stepped (source Ps/PPs, receiver Pr/PPr), taper3D of all four grids, the
+ * Ricker sample st (device ricker[nt]) added to PPs, the receivers of
+ * rcv_chunk at lt set into PPr, the contributions and image of the group as
+ * rtmhip_freq_image at st, then the swaps (the pointers are swapped in
+ * place).  FP32 wavefields.  Results equal that sequence of operator calls
+ * bit for bit; the tapers are applied to the new fields' output when the
+ * source and receivers lie off the borders (the default every-border taper
+ * form, rtmhip_set_taper_fusion).  Calls may split the steps. */
+int rtmhip_freq_steps(const rtmhip_shot *shot, const float *V2DT2, const float *coefs,
+                      const float *taper, const float *ricker, float **Ps, float **PPs,
+                      float **Pr, float **PPr, const float *rcv_chunk, float *img, float *sre,
+                      float *sim, float *rre, float *rim, const float *kre, const float *kim,
+                      const float *w2, int nwstep, int iw0, int nw, int it_hi, int nsteps);
+
 /* Image post-processing of the controller (RTMAcousticController.cpp:244-249,
  * RTMGrid.cpp:129-166 and :226-300): order-2 Laplacian filter with the
  * [3][3] plane fcoefs (host) followed by border removal; out is the

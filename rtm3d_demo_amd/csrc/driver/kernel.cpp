@@ -330,15 +330,18 @@ void RTMKernel::rtmMigrate(const ShotPos &s, const DeviceGrid &v2dt2)
 void RTMKernel::rtmMigrateFreq(const ShotPos &s, const DeviceGrid &v2dt2)
 {
     const double t0 = now_s();
-    const int nt = (int)p_.nt, ntstep = (int)p_.ntstep, b = d_.blen;
-    const int nxe = d_.nxe, nye = d_.nye, nze = d_.nze;
+    const int nt = (int)p_.nt, ntstep = (int)p_.ntstep;
     const std::vector<float> src = ricker(nt, p_.dt, p_.fpeak);
-    const rtmhip_rcv_geom g = geom();
+    ricker_.createDeviceBuffer(src.size());
+    ricker_.moveToDevice(src.data());
+    rtmhip_shot sh = make_shot(s);
+    sh.wave_format = RTMHIP_WAVE_F32;  // the frequency-domain path keeps FP32 fields
     std::string snapfile;
     if (p_.save_snapshots) {
         snapfile = p_.tmpdir + "/RTMSNAPSHOT_S" + std::to_string(s.sx) + "x" +
                    std::to_string(s.sy) + "_" + std::to_string(nt / (int)p_.snapshot_step) +
-                   "x" + std::to_string(nye) + "x" + std::to_string(nze) + ".snapshot.FREQIMG";
+                   "x" + std::to_string(d_.nye) + "x" + std::to_string(d_.nze) +
+                   ".snapshot.FREQIMG";
         std::ofstream(snapfile, std::ios::out | std::ios::binary);
     }
     for (int gw = 0; gw < freq_.nw; gw += nwstep_) {
@@ -352,30 +355,29 @@ void RTMKernel::rtmMigrateFreq(const ShotPos &s, const DeviceGrid &v2dt2)
             hipcheck(rtmhip_copy_to_device(rcv_.ptr(), host_rcv_.data(),
                                            host_rcv_.size() * sizeof(float)),
                      "receiver chunk to device");
-            for (int lt = ntstep - 1; lt >= 0; --lt) {
-                const int rt = itstep + lt, st = nt - 1 - rt;
-                hipcheck(rtmhip_step(Ps, PPs, v2dt2.ptr(), coefs_.data(), nxe, nye, nze,
-                                     d_.order),
-                         "source step");
-                hipcheck(rtmhip_step(Pr, PPr, v2dt2.ptr(), coefs_.data(), nxe, nye, nze,
-                                     d_.order),
-                         "receiver step");
-                for (float *f : {Ps, PPs, Pr, PPr})
-                    hipcheck(rtmhip_taper(f, taper_.ptr(), nxe, nye, nze, b, 0), "taper");
-                hipcheck(rtmhip_apply_source(PPs, nxe, nye, nze, s.sx + b, s.sy + b, s.sz + b,
-                                             src[st]),
-                         "source");
-                hipcheck(rtmhip_receivers(PPr, rcv_.ptr(), nxe, nye, nze, b, &g, ntstep, lt, 1),
-                         "receivers");
-                hipcheck(rtmhip_freq_image(img_.ptr(), PPs, PPr, acc_[0].ptr(), acc_[1].ptr(),
+            // steps lt = ntstep-1 .. 0 of the chunk through the library's
+            // loop (rtmhip_freq_steps: the same operator sequence per step),
+            // split after each step whose image the snapshot option saves
+            int lt = ntstep - 1;
+            while (lt >= 0) {
+                int n = lt + 1;
+                if (p_.save_snapshots) {
+                    for (int j = lt; j >= 0; --j)
+                        if ((nt - 1 - (itstep + j)) % (int)p_.snapshot_step == 0) {
+                            n = lt - j + 1;
+                            break;
+                        }
+                }
+                hipcheck(rtmhip_freq_steps(&sh, v2dt2.ptr(), coefs_.data(), taper_.ptr(),
+                                           ricker_.ptr(), &Ps, &PPs, &Pr, &PPr, rcv_.ptr(),
+                                           img_.ptr(), acc_[0].ptr(), acc_[1].ptr(),
                                            acc_[2].ptr(), acc_[3].ptr(), kre_.ptr(), kim_.ptr(),
-                                           w2_.ptr(), nxe, nye, nze, b, nwstep_, gw, freq_.nw,
-                                           nt, st),
-                         "frequency contributions");
+                                           w2_.ptr(), nwstep_, gw, freq_.nw, itstep + lt, n),
+                         "frequency-domain steps");
+                lt -= n;
+                const int st = nt - 1 - (itstep + lt + 1);
                 if (p_.save_snapshots && st % (int)p_.snapshot_step == 0)
                     snapshot(img_.ptr(), s.sx, snapfile);
-                std::swap(Ps, PPs);
-                std::swap(Pr, PPr);
             }
             progress("+ FMIG", s, nt - itstep, t0);
         }
@@ -384,7 +386,8 @@ void RTMKernel::rtmMigrateFreq(const ShotPos &s, const DeviceGrid &v2dt2)
     const double t1 = now_s();
     report_.migration_s += t1 - t0;
     report_.migration_n++;
-    const double cells = (double)(nxe - d_.order) * (nye - d_.order) * (nze - d_.order);
+    const double cells =
+        (double)(d_.nxe - d_.order) * (d_.nye - d_.order) * (d_.nze - d_.order);
     report_.cell_updates += cells * 2.0 * nt * (double)((freq_.nw + nwstep_ - 1) / nwstep_);
 }
 

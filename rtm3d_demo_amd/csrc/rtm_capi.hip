@@ -182,6 +182,7 @@ hipError_t stencil(int radius, const T *P, T *PP, const float *V, const Coefs &c
     });
 }
 
+
 int check_dispersion()
 {
     int h = 0;
@@ -231,6 +232,19 @@ int prepare_shot(const rtmhip_shot *s, const float *coefs, ShotCtx *c)
     c->ry0 = s->rcv.start_y + s->blen;
     c->rz = s->rcv.depth_z + s->blen;
     return RTMHIP_OK;
+}
+
+// The source cell and every receiver cell lie at least blen from each face.
+bool off_borders(const rtmhip_shot *shot, const ShotCtx &c)
+{
+    const int b = shot->blen;
+    auto in = [&](long long v, int n) { return v >= b && v < n - b; };
+    if (!in(shot->sxe, c.g.nx) || !in(shot->sye, c.g.ny) || !in(shot->sze, c.g.nz)) return false;
+    const rtmhip_rcv_geom &r = shot->rcv;
+    if (r.count_x <= 0 || r.count_y <= 0) return true;
+    return in(c.rz, c.g.nz) && in(c.rx0, c.g.nx) && in(c.ry0, c.g.ny) &&
+           in(c.rx0 + (long long)(r.count_x - 1) * r.dist_x, c.g.nx) &&
+           in(c.ry0 + (long long)(r.count_y - 1) * r.dist_y, c.g.ny);
 }
 
 int check_format(const rtmhip_shot *s)
@@ -506,6 +520,75 @@ int rtmhip_freq_image(float *img, const float *PS, const float *PR, float *sre, 
     return RTMHIP_OK;
 }
 
+int rtmhip_freq_steps(const rtmhip_shot *shot, const float *V2DT2, const float *coefs,
+                      const float *taper, const float *ricker, float **Ps, float **PPs,
+                      float **Pr, float **PPr, const float *rcv_chunk, float *img, float *sre,
+                      float *sim, float *rre, float *rim, const float *kre, const float *kim,
+                      const float *w2, int nwstep, int iw0, int nw, int it_hi, int nsteps)
+{
+    NEED_INIT();
+    ShotCtx c;
+    int rc;
+    if ((rc = prepare_shot(shot, coefs, &c))) return rc;
+    if (shot->wave_format != RTMHIP_WAVE_F32)
+        return fail(RTMHIP_ERR_UNSUPPORTED, "frequency-domain steps take FP32 wavefields");
+    if (!V2DT2 || !taper || !ricker || !Ps || !PPs || !Pr || !PPr || !*Ps || !*PPs || !*Pr ||
+        !*PPr || !rcv_chunk || !img || !sre || !sim || !rre || !rim || !kre || !kim || !w2)
+        return fail(RTMHIP_ERR_INVALID, "NULL pointer");
+    const int nt = shot->nt, b = shot->blen;
+    if (nsteps < 0 || it_hi >= nt || it_hi - nsteps + 1 < 0)
+        return fail(RTMHIP_ERR_INVALID, "steps %d..%d outside [0, %d)", it_hi - nsteps + 1, it_hi,
+                    nt);
+    if (nwstep < 1 || nw < 0 || iw0 < 0)
+        return fail(RTMHIP_ERR_INVALID, "bad frequency group");
+    hipStream_t s = g_state.stream;
+    // taper3D of the four grids after the steps, as the reference orders it,
+    // or applied to the new fields' output (the modeling loop's output form,
+    // when the every-border taper takes it for FP32 and the source and the
+    // receivers lie off the borders): each new field is stored tapered (with
+    // the x-y frame and the front-right edge by their kernels), the older
+    // field, which the reference tapers in place after the step, is tapered
+    // at the next step's centre load instead (not on the call's first step,
+    // whose older fields memory already holds tapered), and in place after
+    // the call's last step.
+    const bool out = c.radius == 4 && b > 0 && g_state.taper_all_mode[RTMHIP_WAVE_F32] == 2 &&
+                     off_borders(shot, c);
+    for (int k = 0; k < nsteps; ++k) {
+        const int rt = it_hi - k, st = nt - 1 - rt, lt = rt % shot->ntstep;
+        if (out) {
+            const TaperFuse tf{taper, b, k == 0 ? 0 : 1, 2};
+            float **fields[2][2] = {{Ps, PPs}, {Pr, PPr}};
+            for (auto &f : fields) {
+                HIPCHK(launch_taper_frame_xy(*f[1], taper, c.g, b, c.radius, k == 0 ? 1 : 2, s),
+                       "frame taper");
+                HIPCHK(stencil(c.radius, (const float *)*f[0], *f[1], V2DT2, c.cf, c.g, nullptr,
+                               &tf),
+                       "stencil");
+                HIPCHK(launch_taper_quirk_out(*f[1], taper, c.g, b, s), "front-right taper");
+            }
+        } else {
+            HIPCHK(stencil(c.radius, (const float *)*Ps, *PPs, V2DT2, c.cf, c.g, nullptr),
+                   "source stencil");
+            HIPCHK(stencil(c.radius, (const float *)*Pr, *PPr, V2DT2, c.cf, c.g, nullptr),
+                   "receiver stencil");
+            HIPCHK(launch_taper_all2(*Ps, *PPs, taper, c.g, b, s), "taper");
+            HIPCHK(launch_taper_all2(*Pr, *PPr, taper, c.g, b, s), "taper");
+        }
+        HIPCHK(launch_source(*PPs, c.src_off, ricker, st, 0.f, s), "source");
+        HIPCHK(launch_receivers(*PPr, const_cast<float *>(rcv_chunk), c.g, c.rx0, c.ry0, c.rz,
+                                shot->rcv.dist_x, shot->rcv.dist_y, shot->rcv.count_x,
+                                shot->rcv.count_y, shot->ntstep, lt, true, s),
+               "receivers");
+        HIPCHK(launch_freq_image(img, *PPs, *PPr, sre, sim, rre, rim, kre, kim, w2, c.g, b, nwstep,
+                                 iw0, nw, nt, st, s),
+               "frequency contributions");
+        std::swap(*Ps, *PPs);
+        std::swap(*Pr, *PPr);
+    }
+    if (out && nsteps > 0) HIPCHK(launch_taper_all2(*PPs, *PPr, taper, c.g, b, s), "taper");
+    return check_dispersion();
+}
+
 int rtmhip_image_finish(const float *img, float *out, int nxe, int nye, int nze, int blen,
                         const float *fcoefs)
 {
@@ -533,18 +616,6 @@ int rtmhip_stack(float *dst, const float *src, size_t count)
 
 namespace {
 
-// The source cell and every receiver cell lie at least blen from each face.
-bool off_borders(const rtmhip_shot *shot, const ShotCtx &c)
-{
-    const int b = shot->blen;
-    auto in = [&](long long v, int n) { return v >= b && v < n - b; };
-    if (!in(shot->sxe, c.g.nx) || !in(shot->sye, c.g.ny) || !in(shot->sze, c.g.nz)) return false;
-    const rtmhip_rcv_geom &r = shot->rcv;
-    if (r.count_x <= 0 || r.count_y <= 0) return true;
-    return in(c.rz, c.g.nz) && in(c.rx0, c.g.nx) && in(c.ry0, c.g.ny) &&
-           in(c.rx0 + (long long)(r.count_x - 1) * r.dist_x, c.g.nx) &&
-           in(c.ry0 + (long long)(r.count_y - 1) * r.dist_y, c.g.ny);
-}
 
 template <typename T>
 int model_loop(const rtmhip_shot *shot, const ShotCtx &c, const float *V2DT2, const float *taper,

@@ -69,7 +69,7 @@ struct ImageFuse {
 struct TaperFuse {
     const float *t;  // device table g[b]
     int b;
-    int pp_taps;     // 1 or 2
+    int pp_taps;     // 1 or 2 (all 2: 0 or 1)
     int all;         // 0: the upper taper (taperUpper3D) on load; 1: every border
                      // (taper3D) on load; 2: every border on the step's output (P stored
                      // tapered); 3: the upper taper on the step's output

@@ -660,7 +660,8 @@ struct WaveGrid {
 // ((v*fy) rounded *fx)*fy.  With TAPER 2 the wave also stores the tapered
 // PP of its z-frame lanes (z < R, z >= nz-R), which the step does not update.
 // TAPER 3 applies taper3D to the step's output instead: P is read as stored
-// (the previous step stored it tapered), the centre PP is tapered once and
+// (the previous step stored it tapered), the centre PP is tapered once
+// (pp_taps 1; pp_taps 0 when memory already holds it tapered) and
 // the new value is tapered before it is stored; the front-right cells and
 // the frame are written by separate kernels, so this form needs no plane-b
 // loads (the pointwise chain only).  TAPER 4 does the same with the upper
@@ -913,7 +914,7 @@ __device__ __forceinline__ void wave_tile(const T *__restrict__ P, T *__restrict
             const float p = q[ctr][r];
             float pp = c_pp[r];
             if (OUT) {
-                pp = tapv(pp, fxc, fyr[r], ztap, fzl, false);
+                if (tf.pp_taps > 0) pp = tapv(pp, fxc, fyr[r], ztap, fzl, false);
             } else if (OUTU) {
                 if (wtap) pp = tapv(pp, fxc, fyr[r], ztap, fzl, false);
             } else if (TAPER && wtap) {
@@ -1291,7 +1292,7 @@ __device__ __forceinline__ void wave_tile2(const T *__restrict__ P, T *__restric
             for (int a = 0; a < 2; ++a) {
                 float pp = c_pp[r][a];
                 if (OUT) {
-                    pp = tapo(pp, a, fxc, fyr[r]);
+                    if (tf.pp_taps > 0) pp = tapo(pp, a, fxc, fyr[r]);
                 } else if (OUTU) {
                     if (wtap) pp = tap(pp, a, fxc, fyr[r]);
                 } else if (TAPER && wtap) {

@@ -30,6 +30,7 @@ EXPORTED = [
     "rtmhip_ae_param_count", "rtmhip_ae_create", "rtmhip_ae_destroy", "rtmhip_ae_get_params",
     "rtmhip_ae_set_params", "rtmhip_ae_gradient", "rtmhip_ae_train", "rtmhip_ae_bricks",
     "rtmhip_ae_encode", "rtmhip_ae_decode", "rtmhip_ae_gemm_test", "rtmhip_freq_image",
+    "rtmhip_freq_steps",
     "rtmhip_snap_ae_sizes", "rtmhip_snap_ae_forward_steps", "rtmhip_snap_ae_backward_steps",
     "rtmhip_snap_dct_sizes", "rtmhip_snap_dct_forward_steps", "rtmhip_snap_dct_backward_steps",
     "rtmhip_dct_roundtrip",
@@ -117,6 +118,8 @@ def load():
         "rtmhip_batch_set_graph": [vp, i],
         "rtmhip_batch_steps_noalloc": [C.POINTER(BatchShotDesc), i, fp, fp, i, i, i],
         "rtmhip_freq_image": [fp] * 10 + [i] * 9,
+        "rtmhip_freq_steps": [C.POINTER(Shot), fp, fp, fp, fp, pp, pp, pp, pp, fp, fp, fp, fp,
+                              fp, fp, fp, fp, fp, i, i, i, i, i],
         "rtmhip_ae_param_count": [],
         "rtmhip_ae_create": [pp, i, C.c_ulonglong],
         "rtmhip_ae_destroy": [vp],
@@ -284,6 +287,18 @@ def freq_image(img, PS, PR, acc, kre, kim, w2, blen, nwstep, iw0, nw, nt, st):
     imaging condition (acc: four DeviceArrays sRe, sIm, rRe, rIm)."""
     check(load().rtmhip_freq_image(img.ptr, PS.ptr, PR.ptr, *[a.ptr for a in acc], kre.ptr,
                                    kim.ptr, w2.ptr, *img.shape, blen, nwstep, iw0, nw, nt, st))
+
+
+def freq_steps(shot, V, coefs, g, ricker, S, R, rcv_chunk, img, acc, kre, kim, w2, nwstep, iw0,
+               nw, it_hi, nsteps):
+    """rtmhip_freq_steps: the frequency-domain migration loop of one group
+    (S, R: FieldPairs of the source and receiver fields, swapped in place;
+    acc: four DeviceArrays sRe, sIm, rRe, rIm)."""
+    c = _f32(coefs)
+    check(load().rtmhip_freq_steps(C.byref(shot), V.ptr, _host_ptr(c), g.ptr, ricker.ptr,
+                                   C.byref(S.p), C.byref(S.pp), C.byref(R.p), C.byref(R.pp),
+                                   rcv_chunk.ptr, img.ptr, *[a.ptr for a in acc], kre.ptr,
+                                   kim.ptr, w2.ptr, nwstep, iw0, nw, it_hi, nsteps))
 
 
 def stack(dst, src):
