@@ -385,6 +385,57 @@ def three_layer_gpu(hip, host, workloads, reps=3):
                                    "not an HBM roofline case"}
 
 
+# --------------------------------------------------- frequency-domain leg ---
+def freq_leg(hip, host, sh, nsteps=60, nwstep=4):
+    """Frequency-domain migration steps (rtmhip_freq_steps) on the salt grid:
+    per time step both fields stepped, taper3D of the four grids, source,
+    receivers (a synthetic record) and the contributions and image of a group
+    of nwstep frequencies -- with the tapers on the new fields' output (the
+    default) and as the reference's in-place passes (mode 0); the images of
+    the two runs are compared bit for bit."""
+    p, nt = sh.p, sh.nt
+    w2, kre, kim = host.freq_tables(nt, np.float32(p["dt"]), np.float32(2.0), np.float32(40.0))
+    nw = len(w2)
+    inner = tuple(n - 2 * sh.b for n in sh.shape)
+    acc = [hip.DeviceArray((nwstep,) + inner) for _ in range(4)]
+    dk = [hip.DeviceArray.from_host(np.ascontiguousarray(a)) for a in (kre, kim, w2)]
+    rng = np.random.default_rng(3)
+    rec = hip.DeviceArray.from_host(
+        (rng.standard_normal((p["receiver_count_x"], p["receiver_count_y"], nt)) * 1e-3)
+        .astype(np.float32))
+    img = hip.DeviceArray(sh.shape)
+    out, ref = {}, None
+    try:
+        for name, mode in (("output_taper", -1), ("in_place_tapers", 0)):
+            hip.set_taper_fusion("f32", mode)
+            for f in (sh.F.a, sh.F.b, sh.R.a, sh.R.b, img, *acc):
+                f.zero()
+            for fp in (sh.F, sh.R):
+                fp.p.value, fp.pp.value = fp.a.ptr.value, fp.b.ptr.value
+            hip.synchronize()
+            t = time.perf_counter()
+            hip.freq_steps(sh.shot, sh.V, sh.coefs, sh.g, sh.src, sh.F, sh.R, rec, img, acc,
+                           dk[0], dk[1], dk[2], nwstep, 0, nw, nt - 1, nsteps)
+            hip.synchronize()
+            dt = time.perf_counter() - t
+            im = img.download()
+            same = ref is None or np.array_equal(im.view(np.uint32), ref.view(np.uint32))
+            ref = im if ref is None else ref
+            out[name] = {"ms_per_step": round(dt / nsteps * 1e3, 3), "images_bit_identical": same}
+        out["speedup"] = round(out["in_place_tapers"]["ms_per_step"]
+                               / out["output_taper"]["ms_per_step"], 3)
+        out.update(workload="salt 512x512x256 + HBC 40, %d time steps of both fields" % nsteps,
+                   frequencies_per_group=nwstep, frequencies_total=nw,
+                   note="the reference runs nw/nwstep groups of nt steps each")
+        log("frequency-domain steps: %.3f ms per step (%.3f with in-place tapers)"
+            % (out["output_taper"]["ms_per_step"], out["in_place_tapers"]["ms_per_step"]))
+    finally:
+        hip.set_taper_fusion("f32", -1)
+        for x in acc + dk + [rec, img]:
+            x.free()
+    return out
+
+
 # ------------------------------------------------------------ snapshot leg ---
 def snapshot_leg(hip, sh, f32_shot_s, k=4):
     """Compressed-snapshot RTM of the salt shot against the FP32 HBC shot."""
@@ -935,8 +986,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--skip", default="",
-                    help="comma-separated legs to leave out: cpu, snapshot, three_layer, "
-                         "accuracy, precision, batched")
+                    help="comma-separated legs to leave out: cpu, snapshot, "
+                         "frequency_domain, three_layer, accuracy, precision, batched")
     ap.add_argument("--batch-nt", type=int, default=300)
     ap.add_argument("--budget", type=float, default=540.0,
                     help="seconds the whole run may take; optional legs that would exceed "
@@ -1058,6 +1109,8 @@ def main():
     snapshot_ok = budget.allows("snapshot", 20.0)
     if snapshot_ok:
         line["snapshot"] = snapshot_leg(hip, sh, shot_s)
+    if budget.allows("frequency_domain", 10.0):
+        line["frequency_domain"] = freq_leg(hip, host, sh)
     sh.free()
     gpu_rec = None
     if budget.allows("three_layer", 5.0):
