@@ -170,7 +170,8 @@ int rtmhip_freq_image(float *img, const float *PS, const float *PR, float *sre, 
  * place).  FP32 wavefields.  Results equal that sequence of operator calls
  * bit for bit; the tapers are applied to the new fields' output when the
  * source and receivers lie off the borders (the default every-border taper
- * form, rtmhip_set_taper_fusion).  Calls may split the steps. */
+ * form, rtmhip_set_taper_fusion).  Calls may split the steps; the steps of
+ * one call lie in one receiver chunk (else RTMHIP_ERR_INVALID). */
 int rtmhip_freq_steps(const rtmhip_shot *shot, const float *V2DT2, const float *coefs,
                       const float *taper, const float *ricker, float **Ps, float **PPs,
                       float **Pr, float **PPr, const float *rcv_chunk, float *img, float *sre,

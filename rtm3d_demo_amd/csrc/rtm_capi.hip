@@ -541,6 +541,9 @@ int rtmhip_freq_steps(const rtmhip_shot *shot, const float *V2DT2, const float *
                     nt);
     if (nwstep < 1 || nw < 0 || iw0 < 0)
         return fail(RTMHIP_ERR_INVALID, "bad frequency group");
+    if (nsteps > 0 && it_hi / shot->ntstep != (it_hi - nsteps + 1) / shot->ntstep)
+        return fail(RTMHIP_ERR_INVALID, "steps %d..%d span two receiver chunks of %d",
+                    it_hi - nsteps + 1, it_hi, shot->ntstep);
     hipStream_t s = g_state.stream;
     // taper3D of the four grids after the steps, as the reference orders it,
     // or applied to the new fields' output (the modeling loop's output form,

@@ -86,3 +86,24 @@ def test_freq_steps_match_operator_sequence(gpu, oracle, shape, kind, fusion, sp
     want = [Ps, PPs, Pr, PPr, img] + acc
     for k, (a, w) in enumerate(zip(got, want)):
         assert np.array_equal(a.view(np.uint32), w.view(np.uint32)), k
+
+
+def test_freq_steps_reject_a_call_across_receiver_chunks(gpu, oracle):
+    shape = (40, 44, 36)
+    S, R = gpu.FieldPair(shape), gpu.FieldPair(shape)
+    inner = tuple(n - 2 * B for n in shape)
+    w2, kre, kim = oracle.freq_tables(NT, np.float32(0.004), np.float32(2.0), np.float32(40.0))
+    d = [gpu.DeviceArray.from_host(np.ascontiguousarray(a)) for a in (kre, kim, w2)]
+    acc = [gpu.DeviceArray((1,) + inner) for _ in range(4)]
+    V, g, ric = gpu.DeviceArray(shape), gpu.DeviceArray((B,)), gpu.DeviceArray((NT,))
+    rcv, img = gpu.DeviceArray((3, 4, NTSTEP)), gpu.DeviceArray(shape)
+    shot = gpu.make_shot(*shape, B, 8, NT, NTSTEP, (20, 22, B + 2),
+                         gpu.RcvGeom(1, 2, 3, 3, 2, 3, 4))
+    coefs = oracle.stencil_coefs(8, 10.0, 11.0, 9.0)
+    try:
+        with pytest.raises(Exception, match="receiver chunks"):
+            gpu.freq_steps(shot, V, coefs, g, ric, S, R, rcv, img, acc, d[0], d[1], d[2], 1, 0,
+                           len(w2), 7, 4)
+    finally:
+        for x in [S, R, V, g, ric, rcv, img] + acc + d:
+            x.free()
