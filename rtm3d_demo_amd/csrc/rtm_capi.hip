@@ -690,23 +690,23 @@ int forward_loop(const rtmhip_shot *shot, const ShotCtx &c, int hbc, const float
                  int nsteps)
 {
     hipStream_t s = g_state.stream;
-    // HBC: the two in-place upper tapers of each step are fused into the
-    // stencil (P tapered on load, the centre PP tapered twice -- once on the
-    // first step of a call, whose PP already carries one taper).  The taper
-    // still pending on P is applied in place after the last step, so the
-    // state between calls is the reference's.
-    // With the separate-taper option the two tapers run in place (one pass
-    // over both fields) before a plain step, as the reference orders them.
-    // With the upper taper on the output (the default, and
-    // rtmhip_set_taper_fusion mode 4) each step stores the upper taper of its
-    // new value, the next step's P,
-    // so P is read as stored and only the centre PP is tapered on load; the
-    // planes [blen-hf, blen) are stored untapered, the upper-border history
-    // is saved from them and they are tapered in place after the source;
-    // PP's frame is tapered twice (once in those planes).  The first step of
-    // a call tapers P in place, the last tapers PP in place and runs the
-    // plain step, so the state between calls is the reference's.  Needs the
-    // source at z >= blen (it acts on the stored field).
+    // HBC, three forms of the two in-place upper tapers of each step:
+    // * on the output (the F32 default, rtmhip_set_taper_fusion mode 4): each
+    //   step stores the upper taper of its new value, the next step's P, so
+    //   P is read as stored and only the centre PP is tapered on load; the
+    //   planes [blen-hf, blen) are stored untapered, the upper-border history
+    //   is saved from them and they are tapered in place after the source;
+    //   PP's frame is tapered twice (once in those planes).  The first step
+    //   of a call tapers P in place, the last tapers PP in place and runs the
+    //   plain step, so the state between calls is the reference's.  Needs the
+    //   source at z >= blen (it acts on the stored field).
+    // * fused on load (mode 1): P tapered on load, the centre PP tapered
+    //   twice -- once on the first step of a call, whose PP already carries
+    //   one taper; the taper still pending on P is applied in place after the
+    //   last step, so the state between calls is the reference's.
+    // * separate (mode 0, the 16-bit default): the two tapers run in place
+    //   (one pass over both fields) before a plain step, as the reference
+    //   orders them.
     const bool sep = g_state.taper_sep[shot->wave_format];
     const int b = shot->blen;
     const bool outu = hbc && g_state.taper_outu[shot->wave_format] && c.radius == 4 &&
