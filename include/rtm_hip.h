@@ -205,7 +205,14 @@ int rtmhip_model_steps(const rtmhip_shot *shot, const float *V2DT2, const float 
 
 /* Forward propagation of migration, steps it0 .. it0+nsteps-1:
  * hbc != 0: rtmHBCForward (RTMFiniteDifferencesKernel_HBC.cpp:140-185):
- *   swap, taperUpper x2, step, source, save UPB at absolute index it.
+ *   swap, taperUpper x2, step, source, save the UPB layer of step it.
+ *   hbc == 1: upb holds every step's layer (nt layers of nxe*nye*order/2
+ *   elements; step it in layer it).  hbc == 2: upb holds one receiver
+ *   chunk's layers (ntstep layers; step it in layer it % ntstep), the
+ *   reference's per-chunk upper-border grid (RTMFiniteDifferencesKernel_HBC.cpp
+ *   :34, :168; spilled to a file after each chunk, :188-195, and loaded back
+ *   before the backward pass reaches that chunk, :261-277); the steps of one
+ *   call must then lie in one receiver chunk.
  * hbc == 0: rtmRBCForward (RTMFiniteDifferencesKernel_RBC.cpp:96-126). */
 int rtmhip_forward_steps(const rtmhip_shot *shot, int hbc, const float *V2DT2,
                          const float *coefs, const float *taper, const float *ricker, float **P,
@@ -220,7 +227,9 @@ int rtmhip_forward_steps(const rtmhip_shot *shot, int hbc, const float *V2DT2,
  * receiver injection and imaging condition run as one pass; with fused == 2
  * (FP32 wavefields, order 8) the source field's backward step joins that
  * pass for every it < nt-2, so V is read once and the image takes the new
- * source value from registers.  Results are identical in all modes. */
+ * source value from registers.  Results are identical in all modes.  hbc
+ * selects the UPB layout as for rtmhip_forward_steps (2: upb holds the
+ * layers of the receiver chunk being processed). */
 int rtmhip_backward_steps(const rtmhip_shot *shot, int hbc, const float *V2DT2,
                           const float *coefs, const float *taper, float **Ps, float **PPs,
                           float **Pr, float **PPr, const float *upb, const float *rcv_chunk,

@@ -47,6 +47,11 @@ RTMKernel::RTMKernel(const RTMParam &p, RTMHIPPlatform &plat) : p_(p), plat_(pla
 //                       (FP32 arithmetic; order 8 only; no x-section snapshots)
 //   RTM3D_BACKWARD_MODE 1 (default): fused receiver step; 2: source and
 //                       receiver steps in one pass; 0: unfused operators
+//   RTM3D_UPB_SPILL     1: keep one receiver chunk of the HBC upper-border
+//                       history on the device and the rest in host memory,
+//                       as the reference keeps one chunk and spills the rest
+//                       to files (the default holds every step on the device
+//                       and spills only when that allocation fails)
 void RTMKernel::read_options()
 {
     if (const char *f = std::getenv("RTM3D_WAVE_FORMAT")) {
@@ -63,6 +68,11 @@ void RTMKernel::read_options()
         backward_mode_ = std::atoi(m);
         if (backward_mode_ < 0 || backward_mode_ > 2)
             throw std::runtime_error("RTM3D_BACKWARD_MODE must be 0, 1 or 2");
+    }
+    if (const char *u = std::getenv("RTM3D_UPB_SPILL")) {
+        if (std::strcmp(u, "0") && std::strcmp(u, "1"))
+            throw std::runtime_error("RTM3D_UPB_SPILL must be 0 or 1");
+        upb_spill_ = !std::strcmp(u, "1");
     }
 }
 
@@ -119,7 +129,26 @@ void RTMKernel::initKernel()
                 for (auto &a : acc_) a.createDeviceBuffer(vol * (size_t)nwstep_);
             }
         } else if (hbc_) {
-            upb_.createDeviceBuffer(wave_slots((size_t)p_.nt * d_.nxe * d_.nye * (d_.order / 2)));
+            // the reference holds one receiver chunk of the history and
+            // spills each chunk to a file (RTMFiniteDifferencesKernel_HBC.cpp
+            // :34, :188-195, :261-277); here the whole history stays on the
+            // device unless it does not fit or a spill is asked for, and the
+            // spilled chunks stay in host memory
+            const size_t layer = (size_t)d_.nxe * d_.nye * (d_.order / 2);
+            if (!upb_spill_) {
+                try {
+                    upb_.createDeviceBuffer(wave_slots((size_t)p_.nt * layer));
+                } catch (const RTMException &) {
+                    upb_.removeDeviceBuffer();
+                    upb_spill_ = true;
+                }
+            }
+            if (upb_spill_) {
+                const size_t es = wave_format_ == RTMHIP_WAVE_F32 ? 4 : 2;
+                upb_chunk_bytes_ = (size_t)p_.ntstep * layer * es;
+                upb_.createDeviceBuffer(wave_slots((size_t)p_.ntstep * layer));
+                host_upb_.resize(upb_chunk_bytes_ * (size_t)(p_.nt / p_.ntstep));
+            }
         }
     }
 }
@@ -241,6 +270,7 @@ void RTMKernel::rtmMigrate(const ShotPos &s, const DeviceGrid &v2dt2)
     float *P = f_[0].ptr(), *PP = f_[1].ptr();
     const int nt = (int)p_.nt, ntstep = (int)p_.ntstep;
     const char *fwdtag = hbc_ ? "HBC_FWD" : "RBC_FWD";
+    const int upb_mode = !hbc_ ? 0 : upb_spill_ ? 2 : 1;  // history layout
     const char *bwdtag = hbc_ ? "HBC_BWD" : "RBC_BWD";
     auto snapname = [&](const char *tag) {
         return p_.tmpdir + "/RTMSNAPSHOT_S" + std::to_string(s.sx) + "x" + std::to_string(s.sy) +
@@ -262,7 +292,7 @@ void RTMKernel::rtmMigrate(const ShotPos &s, const DeviceGrid &v2dt2)
                 const int next = ((it + k - 1) / k) * k;
                 if (next < it + n) n = next - it + 1;
             }
-            hipcheck(rtmhip_forward_steps(&sh, hbc_ ? 1 : 0, v2dt2.ptr(), coefs_.data(),
+            hipcheck(rtmhip_forward_steps(&sh, upb_mode, v2dt2.ptr(), coefs_.data(),
                                           taper_.ptr(), ricker_.ptr(), &P, &PP,
                                           hbc_ ? upb_.ptr() : nullptr, it, n),
                      "forward steps");
@@ -270,6 +300,10 @@ void RTMKernel::rtmMigrate(const ShotPos &s, const DeviceGrid &v2dt2)
             if (p_.save_snapshots && (it - 1) % (int)p_.snapshot_step == 0)
                 snapshot(PP, s.sx, snapfile);
         }
+        if (upb_spill_)
+            hipcheck(rtmhip_copy_from_device(host_upb_.data() + upb_chunk_bytes_ * (itstep / ntstep),
+                                             upb_.ptr(), upb_chunk_bytes_),
+                     "upper-border chunk to host");
         progress((std::string("+[P0] ") + fwdtag).c_str(), s, itstep + ntstep, t0);
     }
     plat_.synchronize();
@@ -288,6 +322,11 @@ void RTMKernel::rtmMigrate(const ShotPos &s, const DeviceGrid &v2dt2)
         hipcheck(rtmhip_copy_to_device(rcv_.ptr(), host_rcv_.data(),
                                        host_rcv_.size() * sizeof(float)),
                  "receiver chunk to device");
+        if (upb_spill_)
+            hipcheck(rtmhip_copy_to_device(upb_.ptr(),
+                                           host_upb_.data() + upb_chunk_bytes_ * (itstep / ntstep),
+                                           upb_chunk_bytes_),
+                     "upper-border chunk to device");
         int it = itstep + ntstep - 1;
         while (it >= itstep) {
             int n = it - itstep + 1;
@@ -296,7 +335,7 @@ void RTMKernel::rtmMigrate(const ShotPos &s, const DeviceGrid &v2dt2)
                 const int prev = (it / k) * k;  // next snapshot time going down
                 if (prev >= itstep) n = it - prev + 1;
             }
-            hipcheck(rtmhip_backward_steps(&sh, hbc_ ? 1 : 0, v2dt2.ptr(), coefs_.data(),
+            hipcheck(rtmhip_backward_steps(&sh, upb_mode, v2dt2.ptr(), coefs_.data(),
                                            taper_.ptr(), &Ps, &PPs, &Pr, &PPr,
                                            hbc_ ? upb_.ptr() : nullptr, rcv_.ptr(), img_.ptr(),
                                            it, n, backward_mode_),

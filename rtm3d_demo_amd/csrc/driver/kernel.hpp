@@ -49,6 +49,11 @@ private:
     bool hbc_ = true;
     int wave_format_ = RTMHIP_WAVE_F32;  // RTM3D_WAVE_FORMAT
     int backward_mode_ = 1;              // RTM3D_BACKWARD_MODE
+    // HBC upper-border history: every step on the device, or (spill) one
+    // receiver chunk on the device and the whole history in host memory
+    bool upb_spill_ = false;             // RTM3D_UPB_SPILL, or no room on the device
+    size_t upb_chunk_bytes_ = 0;         // one chunk's layers
+    std::vector<unsigned char> host_upb_;
     std::vector<float> coefs_;
     DeviceGrid taper_, ricker_, rcv_, upb_, img_;
     DeviceGrid f_[4];  // two source and two receiver wavefields

@@ -354,7 +354,14 @@ int rtmhip_alloc(void **dptr, size_t bytes)
 {
     NEED_INIT();
     if (!dptr) return fail(RTMHIP_ERR_INVALID, "dptr is NULL");
-    HIPCHK(hipMalloc(dptr, bytes ? bytes : 4), "hipMalloc");
+    const hipError_t e = hipMalloc(dptr, bytes ? bytes : 4);
+    if (e != hipSuccess) {
+        // reported here; cleared so that the next launch check does not see
+        // it (callers may fall back to a smaller allocation)
+        (void)hipGetLastError();
+        *dptr = nullptr;
+        return hip_fail(e, "hipMalloc");
+    }
     return RTMHIP_OK;
 }
 
@@ -712,6 +719,8 @@ int forward_loop(const rtmhip_shot *shot, const ShotCtx &c, int hbc, const float
     const bool outu = hbc && g_state.taper_outu[shot->wave_format] && c.radius == 4 &&
                       shot->sze >= b;
     for (int it = it0; it < it0 + nsteps; ++it) {
+        // history layer: every step held (hbc 1) or one receiver chunk (hbc 2)
+        const int ul = hbc == 2 ? it % shot->ntstep : it;
         T *t = *P;
         *P = *PP;
         *PP = t;
@@ -723,7 +732,7 @@ int forward_loop(const rtmhip_shot *shot, const ShotCtx &c, int hbc, const float
                 HIPCHK(launch_taper_frame(*PP, taper, c.g, b, c.radius, 2, s, c.hf),
                        "frame taper");
                 HIPCHK(launch_source(*PP, c.src_off, ricker, it, 0.f, s), "source");
-                HIPCHK(launch_upb_save_taper(*PP, upb, taper, c.g, b, c.hf, it, s), "upb save");
+                HIPCHK(launch_upb_save_taper(*PP, upb, taper, c.g, b, c.hf, ul, s), "upb save");
                 continue;
             }
             if (it == it0)
@@ -743,7 +752,7 @@ int forward_loop(const rtmhip_shot *shot, const ShotCtx &c, int hbc, const float
             HIPCHK(stencil(c.radius, *P, *PP, V2DT2, c.cf, c.g, nullptr), "stencil");
         }
         HIPCHK(launch_source(*PP, c.src_off, ricker, it, 0.f, s), "source");
-        if (hbc) HIPCHK(launch_upb(*PP, upb, c.g, shot->blen, c.hf, it, false, s), "upb save");
+        if (hbc) HIPCHK(launch_upb(*PP, upb, c.g, shot->blen, c.hf, ul, false, s), "upb save");
     }
     if (hbc && !sep && !outu) HIPCHK(launch_taper(*P, taper, c.g, shot->blen, true, s), "taper");
     return RTMHIP_OK;
@@ -764,6 +773,7 @@ int backward_loop(const rtmhip_shot *shot, const ShotCtx &c, int hbc, const floa
     const bool sep = hbc && fused && !dual && g_state.taper_sep[shot->wave_format];
     for (int it = it_hi; it > it_hi - nsteps; --it) {
         const int lt = it % shot->ntstep;
+        const int ul = hbc == 2 ? lt : it;  // history layer, as in forward_loop
         if constexpr (std::is_same<T, float>::value) {
             if (dual && it < nt - 2) {
                 // both backward steps in one pass (class 3), then the UPB
@@ -788,7 +798,7 @@ int backward_loop(const rtmhip_shot *shot, const ShotCtx &c, int hbc, const floa
                        }),
                        "dual stencil");
                 if (hbc)
-                    HIPCHK(launch_upb(*PPs, const_cast<T *>(upb), c.g, b, c.hf, it, true, s),
+                    HIPCHK(launch_upb(*PPs, const_cast<T *>(upb), c.g, b, c.hf, ul, true, s),
                            "upb restore");
                 T *t = *Ps;
                 *Ps = *PPs;
@@ -817,7 +827,7 @@ int backward_loop(const rtmhip_shot *shot, const ShotCtx &c, int hbc, const floa
         } else {
             HIPCHK(stencil(c.radius, *Ps, *PPs, V2DT2, c.cf, c.g, nullptr), "stencil");
             if (hbc)
-                HIPCHK(launch_upb(*PPs, const_cast<T *>(upb), c.g, b, c.hf, it, true, s),
+                HIPCHK(launch_upb(*PPs, const_cast<T *>(upb), c.g, b, c.hf, ul, true, s),
                        "upb restore");
             T *t = *Ps;
             *Ps = *PPs;
@@ -1150,9 +1160,12 @@ int rtmhip_forward_steps(const rtmhip_shot *shot, int hbc, const float *V2DT2,
     if ((rc = check_format(shot))) return rc;
     if (!V2DT2 || !ricker || !P || !PP || !*P || !*PP || (hbc && (!taper || !upb)))
         return fail(RTMHIP_ERR_INVALID, "NULL pointer");
+    if (hbc < 0 || hbc > 2) return fail(RTMHIP_ERR_INVALID, "hbc must be 0, 1 or 2");
     if (nsteps <= 0) return RTMHIP_OK;
     if (it0 < 0 || it0 + nsteps > shot->nt)
         return fail(RTMHIP_ERR_INVALID, "time range outside [0,%d)", shot->nt);
+    // a one-chunk history holds the layers of one receiver chunk only
+    if (hbc == 2 && (rc = check_chunk(shot, it0, it0 + nsteps - 1))) return rc;
     return with_format(shot, [&](auto t) {
         using T = decltype(t);
         return forward_loop<T>(shot, c, hbc, V2DT2, taper, ricker, as<T>(P), as<T>(PP),
@@ -1354,6 +1367,7 @@ int rtmhip_backward_steps(const rtmhip_shot *shot, int hbc, const float *V2DT2,
     if (!V2DT2 || !Ps || !PPs || !Pr || !PPr || !*Ps || !*PPs || !*Pr || !*PPr || !rcv_chunk ||
         !img || (hbc && (!taper || !upb)))
         return fail(RTMHIP_ERR_INVALID, "NULL pointer");
+    if (hbc < 0 || hbc > 2) return fail(RTMHIP_ERR_INVALID, "hbc must be 0, 1 or 2");
     if (nsteps <= 0) return RTMHIP_OK;
     if ((rc = check_chunk(shot, it_hi - nsteps + 1, it_hi))) return rc;
     return with_format(shot, [&](auto t) {

@@ -121,6 +121,44 @@ def test_driver_wave_format_and_backward_mode(workdir, cases, golden, oracle, fm
     assert np.array_equal(img.view(np.uint32), want.ravel().view(np.uint32))
 
 
+@pytest.mark.parametrize("fmt", ["f32", "f16"])
+def test_driver_upb_spill(workdir, cases, golden, oracle, fmt):
+    """RTM3D_UPB_SPILL=1: one receiver chunk of the HBC upper-border history
+    on the device, the chunks in host memory between the passes (the
+    reference spills them to files); the image is unchanged: the reference's
+    for FP32, the oracle's FP16 pipeline for FP16."""
+    print("driver upb spill", fmt, flush=True)
+    env = dict(os.environ, RTM3D_WAVE_FORMAT=fmt, RTM3D_UPB_SPILL="1")
+    p = dict(cases["small_mod"])
+    path = os.path.join(workdir, "input.json")
+    with open(path, "w") as f:
+        json.dump(p, f)
+    r = subprocess.run([DRIVER, path], cwd=workdir, capture_output=True, text=True,
+                       timeout=300, env=env)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    q = dict(cases["small_hbc"])
+    assert q["nt"] // q["ntstep"] >= 2
+    with open(path, "w") as f:
+        json.dump(q, f)
+    r = subprocess.run([DRIVER, path], cwd=workdir, capture_output=True, text=True,
+                       timeout=300, env=env)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    name = "RTMIMG_%s_NS2_NXNYNZ_%dx%dx%d_NT%d.bin.NP1" % (
+        q["mname"].upper(), q["nx"], q["ny"], q["nz"], q["nt"])
+    img = np.fromfile(os.path.join(workdir, "out", name), np.float32)
+    if fmt == "f32":
+        want = golden("small_hbc")["image"]
+    else:
+        vel = golden("small_mod")["vel"]
+        recs = oracle.pipeline_modeling(p, vel, np.float32, fmt)
+        want = oracle.pipeline_migration(q, golden("small_hbc")["vpe"], recs, np.float32, fmt)
+    assert np.array_equal(img.view(np.uint32), want.ravel().view(np.uint32))
+    bad = dict(os.environ, RTM3D_UPB_SPILL="2")
+    r = subprocess.run([DRIVER, path], cwd=workdir, capture_output=True, text=True,
+                       timeout=300, env=bad)
+    assert r.returncode == 1 and "RTM3D_UPB_SPILL" in r.stdout + r.stderr
+
+
 def test_driver_fmig(workdir, cases, golden):
     """Frequency-domain migration (fmig) through the driver: the reference's
     image bit for bit, with its running image across the two shots."""

@@ -69,18 +69,30 @@ def gpu_records(gpu, p, vel, fmt="f32"):
     return np.stack(out)
 
 
-def gpu_shot_image(gpu, S, s, records, hbc, fused):
+def gpu_shot_image(gpu, S, s, records, hbc, fused, spill=False):
+    """spill: the HBC history of one receiver chunk on the device (hbc mode
+    2), each chunk's layers kept on the host between the passes, as the
+    reference spills them to files."""
     b = S.blen
     F, R = gpu.FieldPair(S.shape, S.wdtype), gpu.FieldPair(S.shape, S.wdtype)
-    upb = (gpu.DeviceArray((S.nt, S.shape[0], S.shape[1], S.order // 2), S.wdtype)
-           if hbc else None)
+    mode = (2 if spill else 1) if hbc else 0
+    upb = (gpu.DeviceArray((S.ntstep if spill else S.nt, S.shape[0], S.shape[1], S.order // 2),
+                           S.wdtype) if hbc else None)
+    spilled = {}
     for itstep in range(0, S.nt, S.ntstep):
-        gpu.forward_steps(S.shot(s), hbc, S.V, S.coefs, S.g, S.src, F, upb, itstep, S.ntstep)
+        if spill:
+            upb.upload(np.full(upb.shape, 0x7FC0, np.uint16).view(S.wdtype)
+                       if S.wdtype == np.uint16 else np.full(upb.shape, np.nan, np.float32))
+        gpu.forward_steps(S.shot(s), mode, S.V, S.coefs, S.g, S.src, F, upb, itstep, S.ntstep)
+        if spill:
+            spilled[itstep] = upb.download()
     img = gpu.DeviceArray(S.shape).zero()
     rcv = gpu.DeviceArray(records.shape[1:])
     for k, itstep in reversed(list(enumerate(range(0, S.nt, S.ntstep)))):
         rcv.upload(records[k])
-        gpu.backward_steps(S.shot(s), hbc, S.V, S.coefs, S.g, F, R, upb, rcv, img,
+        if spill:
+            upb.upload(spilled[itstep])
+        gpu.backward_steps(S.shot(s), mode, S.V, S.coefs, S.g, F, R, upb, rcv, img,
                            itstep + S.ntstep - 1, S.ntstep, fused)
     out = img.download()
     for d in (F, R, img, rcv) + ((upb,) if upb else ()):
@@ -88,13 +100,13 @@ def gpu_shot_image(gpu, S, s, records, hbc, fused):
     return out
 
 
-def gpu_migration(gpu, p, vpe, records, fused=True, fmt="f32"):
+def gpu_migration(gpu, p, vpe, records, fused=True, fmt="f32", spill=False):
     hbc = p["boundary_condition"] != "rbc"
     S = Setup(gpu, p, vpe, fmt)
     total = gpu.DeviceArray(S.shape).zero()
     simg = gpu.DeviceArray(S.shape)
     for k, s in enumerate(_shots(p)):
-        simg.upload(gpu_shot_image(gpu, S, s, records[k], hbc, fused))
+        simg.upload(gpu_shot_image(gpu, S, s, records[k], hbc, fused, spill))
         gpu.stack(total, simg)
     b = S.blen
     out = gpu.DeviceArray(tuple(n - 2 * b for n in S.shape))
@@ -121,6 +133,45 @@ def test_migration_matches_reference(gpu, cases, golden, bc, fused):
     p = cases["small_" + bc]
     img = gpu_migration(gpu, p, golden("small_" + bc)["vpe"], golden("small_mod")["records"], fused)
     assert _bits_equal(img, golden("small_" + bc)["image"])
+
+
+@pytest.mark.parametrize("fused", [True, False, 2])
+def test_migration_one_chunk_history_matches_reference(gpu, cases, golden, fused):
+    """HBC history of one receiver chunk on the device (hbc mode 2, the
+    reference's per-chunk upper-border grid; the buffer is filled with NaN
+    before each forward chunk, so a layer read from the wrong slot shows)."""
+    print("migration one-chunk history", fused, flush=True)
+    p = cases["small_hbc"]
+    assert p["nt"] // p["ntstep"] >= 2
+    img = gpu_migration(gpu, p, golden("small_hbc")["vpe"], golden("small_mod")["records"],
+                        fused, spill=True)
+    assert _bits_equal(img, golden("small_hbc")["image"])
+
+
+@pytest.mark.parametrize("fmt", ["f16", "bf16"])
+def test_one_chunk_history_16bit(gpu, cases, golden, fmt):
+    """16-bit wavefields and history: mode 2 gives the whole-history image."""
+    print("one-chunk history", fmt, flush=True)
+    p = cases["small_hbc"]
+    vpe, rec = golden("small_hbc")["vpe"], golden("small_mod")["records"]
+    a = gpu_migration(gpu, p, vpe, rec, True, fmt)
+    b = gpu_migration(gpu, p, vpe, rec, True, fmt, spill=True)
+    assert _bits_equal(a, b)
+
+
+def test_one_chunk_history_rejects_bad_calls(gpu, cases, golden):
+    p = cases["small_hbc"]
+    S = Setup(gpu, p, golden("small_hbc")["vpe"])
+    s = _shots(p)[0]
+    F = gpu.FieldPair(S.shape)
+    upb = gpu.DeviceArray((S.ntstep, S.shape[0], S.shape[1], S.order // 2))
+    with pytest.raises(Exception, match="crosses a receiver chunk"):
+        gpu.forward_steps(S.shot(s), 2, S.V, S.coefs, S.g, S.src, F, upb, S.ntstep - 2, 4)
+    with pytest.raises(Exception, match="hbc must be"):
+        gpu.forward_steps(S.shot(s), 3, S.V, S.coefs, S.g, S.src, F, upb, 0, 2)
+    for d in (F, upb):
+        d.free()
+    S.free()
 
 
 def test_migration_matches_oracle_other_geometry(gpu, oracle, cases, golden):
