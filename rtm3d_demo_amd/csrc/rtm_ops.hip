@@ -616,8 +616,16 @@ __global__ void __launch_bounds__(kThreads)
 
 // Snapshot of a wavefield over the imaging range (all x, y; z in [b, nz-b)):
 // snap is [nx][ny][nz-2b] of element type S (float, or _Float16 rounded to
-// nearest-even).  One wave per (x, y) row, z on the lanes.
-template <typename T, typename S>
+// nearest-even).  One wave per (x, y) row, z on the lanes, V consecutive
+// cells per lane (V = 4 when every row start and both arrays allow the
+// vector access: one 16- or 8-byte load and one 8- or 16-byte store per
+// lane instead of 2-byte accesses; the values are the same).
+template <typename E, int N>
+struct alignas(sizeof(E) * N) SnapVec {
+    E v[N];
+};
+
+template <typename T, typename S, int V>
 __global__ void __launch_bounds__(kThreads)
     k_snap_store(const T *__restrict__ P, S *__restrict__ snap, Grid g, int b)
 {
@@ -630,7 +638,19 @@ __global__ void __launch_bounds__(kThreads)
     const int lz = g.nz - 2 * b;
     const T *__restrict__ r = P + ((long long)x * g.sx + (long long)y * g.sy + b);
     S *__restrict__ o = snap + (long long)row * lz;
-    for (int z = lane; z < lz; z += 64) o[z] = wst<S>(wld(r[z]));
+    if constexpr (V == 1) {
+        for (int z = lane; z < lz; z += 64) o[z] = wst<S>(wld(r[z]));
+    } else {
+        const SnapVec<T, V> *__restrict__ rv = reinterpret_cast<const SnapVec<T, V> *>(r);
+        SnapVec<S, V> *__restrict__ ov = reinterpret_cast<SnapVec<S, V> *>(o);
+        for (int k = lane; k < lz / V; k += 64) {
+            const SnapVec<T, V> a = rv[k];
+            SnapVec<S, V> c;
+#pragma unroll
+            for (int e = 0; e < V; ++e) c.v[e] = wst<S>(wld(a.v[e]));
+            ov[k] = c;
+        }
+    }
 }
 
 // Imaging condition from a snapshot: img += w * (snap * PR) on the imaging
@@ -1262,8 +1282,16 @@ hipError_t launch_snap_store(const T *P, S *snap, const Grid &g, int b, hipStrea
     if (g.nz <= 2 * b) return hipSuccess;
     if ((long long)g.nx * g.ny >= (1LL << 31)) return hipErrorInvalidValue;
     const long long rows = (long long)g.nx * g.ny;
-    hipLaunchKernelGGL((k_snap_store<T, S>), dim3((unsigned)((rows + 3) / 4)), dim3(kThreads), 0,
-                       s, P, snap, g, b);
+    const dim3 grid((unsigned)((rows + 3) / 4));
+    // vector form: every row start (x*sx + y*sy + b in P, row*(nz-2b) in
+    // snap) a multiple of 4 cells and both arrays aligned to 4 cells
+    const bool vec = (b % 4) == 0 && (g.sy % 4) == 0 && (g.sx % 4) == 0 &&
+                     ((g.nz - 2 * b) % 4) == 0 && ((uintptr_t)P % (4 * sizeof(T))) == 0 &&
+                     ((uintptr_t)snap % (4 * sizeof(S))) == 0;
+    if (vec)
+        hipLaunchKernelGGL((k_snap_store<T, S, 4>), grid, dim3(kThreads), 0, s, P, snap, g, b);
+    else
+        hipLaunchKernelGGL((k_snap_store<T, S, 1>), grid, dim3(kThreads), 0, s, P, snap, g, b);
     return hipGetLastError();
 }
 
