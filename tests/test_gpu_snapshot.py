@@ -16,14 +16,14 @@ SRC = (17, 19, 2)                      # interior coordinates
 RCV = (1, 2, 3, 3, 4, 4, 5)           # start x, y, depth, dist x, y, count x, y
 
 
-def _inputs(seed=3):
+def _inputs(seed=3, b=B, grid=GRID):
     rng = np.random.default_rng(seed)
-    V = rng.uniform(0.0, 0.15, GRID).astype(np.float32)
+    V = rng.uniform(0.0, 0.15, grid).astype(np.float32)
     coefs = np.zeros(27, np.float32)
     c8 = np.array([-1 / 560, 8 / 315, -1 / 5, 8 / 5, -205 / 72, 8 / 5, -1 / 5, 8 / 315, -1 / 560])
     for d, h in enumerate((10.0, 11.0, 9.0)):
         coefs[d * 9:(d + 1) * 9] = (c8 / (h * h)).astype(np.float32)
-    g = np.exp(-(np.sqrt(-np.log(0.9)) / B * np.arange(B)) ** 2).astype(np.float32)
+    g = np.exp(-(np.sqrt(-np.log(0.9)) / b * np.arange(b)) ** 2).astype(np.float32)
     ricker = rng.uniform(-1, 1, NT).astype(np.float32)
     rcv = rng.uniform(-1, 1, (RCV[5], RCV[6], NT)).astype(np.float32)
     return V, coefs, g, ricker, rcv
@@ -54,6 +54,35 @@ def test_snapshot_rtm_matches_oracle(gpu, oracle, wfmt, sfmt, k, fusion):
         for d in (dV, dg, dsrc, drcv, S, R, img, snaps):
             d.free()
     want = oracle.snapshot_migrate_shot(V, coefs, g, ricker, B, 8, NT, SRC,
+                                        oracle.RcvGeom(*RCV), rcv, k, sfmt,
+                                        None if wfmt == "f32" else wfmt)
+    assert np.abs(want).max() > 0
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+
+
+@pytest.mark.parametrize("sfmt", ["f32", "f16"])
+@pytest.mark.parametrize("wfmt", ["f32", "f16", "bf16"])
+def test_snapshot_rtm_aligned_rows_matches_oracle(gpu, oracle, wfmt, sfmt):
+    """Border 12 and nz 72: every snapshot row starts on a 4-cell boundary,
+    so the store takes its four-cells-per-lane form."""
+    print("snapshot rtm aligned rows", wfmt, sfmt, flush=True)
+    b, grid, k = 12, (36, 42, 72), 2
+    assert b % 4 == 0 and (grid[2] - 2 * b) % 4 == 0 and grid[2] % 4 == 0
+    V, coefs, g, ricker, rcv = _inputs(5, b, grid)
+    wd = gpu.WAVE_FORMATS[wfmt][1]
+    shot = gpu.make_shot(*grid, b, 8, NT, NT, tuple(c + b for c in SRC), gpu.RcvGeom(*RCV), wfmt)
+    dV, dg, dsrc, drcv = (gpu.DeviceArray.from_host(a) for a in (V, g, ricker, rcv))
+    S, R = gpu.FieldPair(grid, wd), gpu.FieldPair(grid, wd)
+    img = gpu.DeviceArray(grid).zero()
+    snaps = gpu.snap_buffer(shot, sfmt, k)
+    try:
+        gpu.snap_forward_steps(shot, dV, coefs, dg, dsrc, S, snaps, sfmt, k, 0, NT)
+        gpu.snap_backward_steps(shot, dV, coefs, dg, R, snaps, sfmt, k, drcv, img, NT - 1, NT)
+        got = img.download()
+    finally:
+        for d in (dV, dg, dsrc, drcv, S, R, img, snaps):
+            d.free()
+    want = oracle.snapshot_migrate_shot(V, coefs, g, ricker, b, 8, NT, SRC,
                                         oracle.RcvGeom(*RCV), rcv, k, sfmt,
                                         None if wfmt == "f32" else wfmt)
     assert np.abs(want).max() > 0
