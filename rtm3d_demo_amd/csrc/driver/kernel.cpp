@@ -147,7 +147,8 @@ void RTMKernel::initKernel()
                 const size_t es = wave_format_ == RTMHIP_WAVE_F32 ? 4 : 2;
                 upb_chunk_bytes_ = (size_t)p_.ntstep * layer * es;
                 upb_.createDeviceBuffer(wave_slots((size_t)p_.ntstep * layer));
-                host_upb_.resize(upb_chunk_bytes_ * (size_t)(p_.nt / p_.ntstep));
+                const size_t chunks = ((size_t)p_.nt + p_.ntstep - 1) / p_.ntstep;
+                host_upb_.resize(upb_chunk_bytes_ * chunks);
             }
         }
     }
@@ -300,10 +301,11 @@ void RTMKernel::rtmMigrate(const ShotPos &s, const DeviceGrid &v2dt2)
             if (p_.save_snapshots && (it - 1) % (int)p_.snapshot_step == 0)
                 snapshot(PP, s.sx, snapfile);
         }
-        if (upb_spill_)
-            hipcheck(rtmhip_copy_from_device(host_upb_.data() + upb_chunk_bytes_ * (itstep / ntstep),
-                                             upb_.ptr(), upb_chunk_bytes_),
+        if (upb_spill_) {
+            unsigned char *slot = host_upb_.data() + upb_chunk_bytes_ * (itstep / ntstep);
+            hipcheck(rtmhip_copy_from_device(slot, upb_.ptr(), upb_chunk_bytes_),
                      "upper-border chunk to host");
+        }
         progress((std::string("+[P0] ") + fwdtag).c_str(), s, itstep + ntstep, t0);
     }
     plat_.synchronize();
