@@ -345,6 +345,10 @@ def model_steps(shot, V, coefs, g, ricker, fields, rcv_chunk, it0, nsteps):
 
 
 def forward_steps(shot, hbc, V, coefs, g, ricker, fields, upb, it0, nsteps):
+    """rtmhip_forward_steps.  hbc: 0 RBC; 1 HBC with upb holding every
+    step's upper-border layer [nt][nxe][nye][order/2]; 2 HBC with upb
+    holding one receiver chunk's layers [ntstep][nxe][nye][order/2] (the
+    steps of the call must lie in one chunk)."""
     c = _f32(coefs)
     check(load().rtmhip_forward_steps(C.byref(shot), int(hbc), V.ptr, _host_ptr(c),
                                       g.ptr if g is not None else None, ricker.ptr,
@@ -354,6 +358,8 @@ def forward_steps(shot, hbc, V, coefs, g, ricker, fields, upb, it0, nsteps):
 
 def backward_steps(shot, hbc, V, coefs, g, src_fields, rcv_fields, upb, rcv_chunk, img, it_hi,
                    nsteps, fused=True):
+    """rtmhip_backward_steps; hbc and upb as for forward_steps (with hbc 2,
+    upb holds the layers of the receiver chunk being processed)."""
     c = _f32(coefs)
     check(load().rtmhip_backward_steps(C.byref(shot), int(hbc), V.ptr, _host_ptr(c),
                                        g.ptr if g is not None else None,
